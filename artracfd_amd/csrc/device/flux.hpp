@@ -1,0 +1,280 @@
+// flux.hpp - per-face numerical fluxes evaluated on the device.
+//
+// The convective flux is the characteristic-wise, flux-vector-split WENO
+// reconstruction of the solver (convective_flux.c:48-124 with the state
+// algebra of cfd_commons.c:79-246 and weno5.c:35-62 / weno3.c:35-55); the
+// viscous flux is its second-order central face flux (diffusive_flux.c:58-312).
+// Every expression keeps the reference's operand order and the translation
+// units are compiled with -ffp-contract=off, so a face flux is bit-identical
+// to the CPU evaluation (up to the sign of an exact zero).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace artra {
+
+constexpr int kComp = 5;
+
+struct FluxModel {
+    double gamma;
+    double cv;
+    double refMu;   // nondimensional viscosity scale (> 0 enables viscous flux)
+    double refT;
+    int roe;        // 1: Roe average, 0: arithmetic mean
+    int sw;         // 1: Steger-Warming split, 0: local Lax-Friedrichs
+};
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// Jiang-Shu fifth-order WENO of one characteristic component from the
+// five upwind-ordered values f0..f4 (value sought at the face after f2).
+__device__ __forceinline__ double weno5(double f0, double f1, double f2, double f3, double f4)
+{
+    const double eps = 1.0e-6;
+    const double IS0 = (13.0 / 12.0) * sq(f0 - 2.0 * f1 + f2) + (1.0 / 4.0) * sq(f0 - 4.0 * f1 + 3.0 * f2);
+    const double IS1 = (13.0 / 12.0) * sq(f1 - 2.0 * f2 + f3) + (1.0 / 4.0) * sq(f1 - f3);
+    const double IS2 = (13.0 / 12.0) * sq(f2 - 2.0 * f3 + f4) + (1.0 / 4.0) * sq(3.0 * f2 - 4.0 * f3 + f4);
+    const double a0 = (1.0 / 10.0) / sq(eps + IS0);
+    const double a1 = (6.0 / 10.0) / sq(eps + IS1);
+    const double a2 = (3.0 / 10.0) / sq(eps + IS2);
+    const double w0 = a0 / (a0 + a1 + a2);
+    const double w1 = a1 / (a0 + a1 + a2);
+    const double w2 = a2 / (a0 + a1 + a2);
+    const double q0 = (1.0 / 6.0) * (2.0 * f0 - 7.0 * f1 + 11.0 * f2);
+    const double q1 = (1.0 / 6.0) * (-f1 + 5.0 * f2 + 2.0 * f3);
+    const double q2 = (1.0 / 6.0) * (2.0 * f2 + 5.0 * f3 - f4);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+// Third-order WENO from f0..f2 (value sought at the face after f1).
+__device__ __forceinline__ double weno3(double f0, double f1, double f2)
+{
+    const double eps = 1.0e-6;
+    const double IS0 = sq(f1 - f0);
+    const double IS1 = sq(f2 - f1);
+    const double a0 = (1.0 / 3.0) / sq(eps + IS0);
+    const double a1 = (2.0 / 3.0) / sq(eps + IS1);
+    const double w0 = a0 / (a0 + a1);
+    const double w1 = a1 / (a0 + a1);
+    const double q0 = (1.0 / 2.0) * (-f0 + 3.0 * f1);
+    const double q1 = (1.0 / 2.0) * (f1 + f2);
+    return w0 * q0 + w1 * q1;
+}
+
+// Left eigenvector matrix of the Euler flux Jacobian along S at the face
+// state (u, v, w, c); rows are characteristic fields (acoustic -, entropy /
+// shear x3 in axis order, acoustic +).
+template <int S>
+__device__ __forceinline__ void left_vectors(double gamma, double u, double v, double w, double c, double L[5][5])
+{
+    const double q = 0.5 * (u * u + v * v + w * w);
+    const double b = (gamma - 1.0) / (2.0 * c * c);
+    const double d = 1.0 / (2.0 * c);
+    if (S == 0) {
+        L[0][0] = b * q + d * u;         L[0][1] = -b * u - d;            L[0][2] = -b * v;
+        L[0][3] = -b * w;                L[0][4] = b;
+        L[1][0] = -2.0 * b * q + 1.0;    L[1][1] = 2.0 * b * u;           L[1][2] = 2.0 * b * v;
+        L[1][3] = 2.0 * b * w;           L[1][4] = -2.0 * b;
+        L[2][0] = -2.0 * b * q * v;      L[2][1] = 2.0 * b * v * u;       L[2][2] = 2.0 * b * v * v + 1.0;
+        L[2][3] = 2.0 * b * w * v;       L[2][4] = -2.0 * b * v;
+        L[3][0] = -2.0 * b * q * w;      L[3][1] = 2.0 * b * w * u;       L[3][2] = 2.0 * b * w * v;
+        L[3][3] = 2.0 * b * w * w + 1.0; L[3][4] = -2.0 * b * w;
+        L[4][0] = b * q - d * u;         L[4][1] = -b * u + d;            L[4][2] = -b * v;
+        L[4][3] = -b * w;                L[4][4] = b;
+    } else if (S == 1) {
+        L[0][0] = b * q + d * v;         L[0][1] = -b * u;                L[0][2] = -b * v - d;
+        L[0][3] = -b * w;                L[0][4] = b;
+        L[1][0] = -2.0 * b * q * u;      L[1][1] = 2.0 * b * u * u + 1.0; L[1][2] = 2.0 * b * v * u;
+        L[1][3] = 2.0 * b * w * u;       L[1][4] = -2.0 * b * u;
+        L[2][0] = -2.0 * b * q + 1.0;    L[2][1] = 2.0 * b * u;           L[2][2] = 2.0 * b * v;
+        L[2][3] = 2.0 * b * w;           L[2][4] = -2.0 * b;
+        L[3][0] = -2.0 * b * q * w;      L[3][1] = 2.0 * b * w * u;       L[3][2] = 2.0 * b * w * v;
+        L[3][3] = 2.0 * b * w * w + 1.0; L[3][4] = -2.0 * b * w;
+        L[4][0] = b * q - d * v;         L[4][1] = -b * u;                L[4][2] = -b * v + d;
+        L[4][3] = -b * w;                L[4][4] = b;
+    } else {
+        L[0][0] = b * q + d * w;         L[0][1] = -b * u;                L[0][2] = -b * v;
+        L[0][3] = -b * w - d;            L[0][4] = b;
+        L[1][0] = -2.0 * b * q * u;      L[1][1] = 2.0 * b * u * u + 1.0; L[1][2] = 2.0 * b * v * u;
+        L[1][3] = 2.0 * b * w * u;       L[1][4] = -2.0 * b * u;
+        L[2][0] = -2.0 * b * q * v;      L[2][1] = 2.0 * b * v * u;       L[2][2] = 2.0 * b * v * v + 1.0;
+        L[2][3] = 2.0 * b * w * v;       L[2][4] = -2.0 * b * v;
+        L[3][0] = -2.0 * b * q + 1.0;    L[3][1] = 2.0 * b * u;           L[3][2] = 2.0 * b * v;
+        L[3][3] = 2.0 * b * w;           L[3][4] = -2.0 * b;
+        L[4][0] = b * q - d * w;         L[4][1] = -b * u;                L[4][2] = -b * v;
+        L[4][3] = -b * w + d;            L[4][4] = b;
+    }
+}
+
+// F = R (hp + hn) with R the right eigenvectors along S.  The reference
+// accumulates all 25 products from 0.0; the structural zeros and ones of R
+// contribute exact no-ops, so only the nonzero products are formed, in the
+// same column order, starting from 0.0.
+template <int S>
+__device__ __forceinline__ void inverse_project(double u, double v, double w, double hT, double c, const double h[5],
+                                                double F[5])
+{
+    const double q = 0.5 * (u * u + v * v + w * w);
+    const double vel[3] = {u, v, w};
+    const double un = vel[S];
+    // row 0: columns 0, 1+S, 4 are 1
+    F[0] = ((0.0 + h[0]) + h[1 + S]) + h[4];
+    // rows 1..3 (momentum m)
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const double r0 = (m == S) ? vel[m] - c : vel[m];
+        const double r4 = (m == S) ? vel[m] + c : vel[m];
+        double acc = 0.0 + r0 * h[0];
+        // column 1+t: entropy column (t == S) carries vel[m] only when m == S; shear column t carries 1 at m == t
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            if (t == S) {
+                if (m == S) acc = acc + vel[m] * h[1 + t];
+            } else if (t == m) {
+                acc = acc + h[1 + t];
+            }
+        }
+        F[1 + m] = acc + r4 * h[4];
+    }
+    // row 4
+    {
+        double acc = 0.0 + (hT - un * c) * h[0];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const double r = (t == S) ? vel[S] * vel[S] - q : vel[t];
+            acc = acc + r * h[1 + t];
+        }
+        F[4] = acc + (hT + un * c) * h[4];
+    }
+}
+
+// Convective numerical flux at the face between stencil nodes K0 and K0+1,
+// from the conservative states st[0..NS-1] (nodes i-2..i+3 for WENO5,
+// i-1..i+2 for WENO3; NS = 6 or 4).
+template <int S, int NS>
+__device__ __forceinline__ void convective_flux(const FluxModel &mdl, const double st[NS][5], double F[5])
+{
+    constexpr int K0 = (NS == 6) ? 2 : 1;
+    const double gamma = mdl.gamma;
+    // face average (cfd_commons.c:79-112)
+    const double *UL = st[K0];
+    const double *UR = st[K0 + 1];
+    const double rhoL = UL[0];
+    const double uL = UL[1] / UL[0], vL = UL[2] / UL[0], wL = UL[3] / UL[0];
+    const double hTL = (UL[4] / UL[0]) * gamma - 0.5 * (uL * uL + vL * vL + wL * wL) * (gamma - 1.0);
+    const double rhoR = UR[0];
+    const double uR = UR[1] / UR[0], vR = UR[2] / UR[0], wR = UR[3] / UR[0];
+    const double hTR = (UR[4] / UR[0]) * gamma - 0.5 * (uR * uR + vR * vR + wR * wR) * (gamma - 1.0);
+    double u, v, w, hT;
+    if (mdl.roe) {
+        const double D = sqrt(rhoR / rhoL);
+        u = (uL + D * uR) / (1.0 + D);
+        v = (vL + D * vR) / (1.0 + D);
+        w = (wL + D * wR) / (1.0 + D);
+        hT = (hTL + D * hTR) / (1.0 + D);
+    } else {
+        u = 0.5 * (uL + uR);
+        v = 0.5 * (vL + vR);
+        w = 0.5 * (wL + wR);
+        hT = 0.5 * (hTL + hTR);
+    }
+    const double c = sqrt((gamma - 1.0) * (hT - 0.5 * (u * u + v * v + w * w)));
+    const double vel[3] = {u, v, w};
+    const double un = vel[S];
+    // split eigenvalues (cfd_commons.c:113-148)
+    const double lam[5] = {un - c, un, un, un, un + c};
+    double lp[5], ln[5];
+    if (mdl.sw) {
+        const double eps = 1.0e-3;
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            lp[r] = 0.5 * (lam[r] + sqrt(lam[r] * lam[r] + eps * eps));
+            ln[r] = 0.5 * (lam[r] - sqrt(lam[r] * lam[r] + eps * eps));
+        }
+    } else {
+        const double star = fabs(lam[2]) + lam[4] - lam[2];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            lp[r] = 0.5 * (lam[r] + star);
+            ln[r] = 0.5 * (lam[r] - star);
+        }
+    }
+    double L[5][5];
+    left_vectors<S>(gamma, u, v, w, c, L);
+    // characteristic variables of the stencil, then WENO per field
+    double hsum[5];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        double W[NS];
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            double acc = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < 5; ++cc) acc = acc + L[r][cc] * st[m][cc];
+            W[m] = acc;
+        }
+        double hp, hn;
+        if (NS == 6) {
+            hp = weno5(lp[r] * W[0], lp[r] * W[1], lp[r] * W[2], lp[r] * W[3], lp[r] * W[4]);
+            hn = weno5(ln[r] * W[5], ln[r] * W[4], ln[r] * W[3], ln[r] * W[2], ln[r] * W[1]);
+        } else {
+            hp = weno3(lp[r] * W[0], lp[r] * W[1], lp[r] * W[2]);
+            hn = weno3(ln[r] * W[3], ln[r] * W[2], ln[r] * W[1]);
+        }
+        hsum[r] = hp + hn;
+    }
+    inverse_project<S>(u, v, w, hT, c, hsum, F);
+}
+
+// Viscous numerical flux at the face between node P and its +S neighbour E.
+// g[a] are the states at P -/+ the first tangential axis, g[b] along the
+// second, and the same for E: nb[0]=P-a, nb[1]=P+a, nb[2]=P-b, nb[3]=P+b,
+// nb[4]=E-a, nb[5]=E+a, nb[6]=E-b, nb[7]=E+b.  Tangential axes: X face
+// (y, z), Y face (x, z), Z face (x, y).
+template <int S>
+__device__ __forceinline__ void viscous_flux(const FluxModel &mdl, const double dd[3], const double P[5],
+                                             const double E[5], const double nb[8][5], double F[5])
+{
+    constexpr int TA = (S == 0) ? 1 : 0;
+    constexpr int TB = (S == 2) ? 1 : 2;
+    const double cv = mdl.cv;
+    const double velP[3] = {P[1] / P[0], P[2] / P[0], P[3] / P[0]};
+    const double velE[3] = {E[1] / E[0], E[2] / E[0], E[3] / E[0]};
+    const double TP = (P[4] - 0.5 * (P[1] * P[1] + P[2] * P[2] + P[3] * P[3]) / P[0]) / (P[0] * cv);
+    const double TE = (E[4] - 0.5 * (E[1] * E[1] + E[2] * E[2] + E[3] * E[3]) / E[0]) / (E[0] * cv);
+    // g[m][n] = d vel_m / d x_n
+    double g[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        g[m][S] = (velE[m] - velP[m]) * dd[S];
+        g[m][TA] = 0.25 * (nb[1][1 + m] / nb[1][0] + nb[5][1 + m] / nb[5][0] - nb[0][1 + m] / nb[0][0] -
+                           nb[4][1 + m] / nb[4][0]) * dd[TA];
+        g[m][TB] = 0.25 * (nb[3][1 + m] / nb[3][0] + nb[7][1 + m] / nb[7][0] - nb[2][1 + m] / nb[2][0] -
+                           nb[6][1 + m] / nb[6][0]) * dd[TB];
+    }
+    const double dT = (TE - TP) * dd[S];
+    const double uh = 0.5 * (velP[0] + velE[0]);
+    const double vh = 0.5 * (velP[1] + velE[1]);
+    const double wh = 0.5 * (velP[2] + velE[2]);
+    const double Ts = 0.5 * (TP + TE) * mdl.refT;
+    const double mu = mdl.refMu * (1.458e-6 * pow(Ts, 1.5) / (Ts + 110.4));
+    const double heatK = mdl.gamma * mdl.cv * mu / 0.71;
+    const double divV = g[0][0] + g[1][1] + g[2][2];
+    F[0] = 0.0;
+    if (S == 0) {
+        F[1] = mu * (g[0][0] + g[0][0] - (2.0 / 3.0) * divV);
+        F[2] = mu * (g[0][1] + g[1][0]);
+        F[3] = mu * (g[0][2] + g[2][0]);
+    } else if (S == 1) {
+        F[1] = mu * (g[1][0] + g[0][1]);
+        F[2] = mu * (g[1][1] + g[1][1] - (2.0 / 3.0) * divV);
+        F[3] = mu * (g[1][2] + g[2][1]);
+    } else {
+        F[1] = mu * (g[2][0] + g[0][2]);
+        F[2] = mu * (g[2][1] + g[1][2]);
+        F[3] = mu * (g[2][2] + g[2][2] - (2.0 / 3.0) * divV);
+    }
+    F[4] = heatK * dT + F[1] * uh + F[2] * vh + F[3] * wh;
+}
+
+}  // namespace artra

@@ -1,0 +1,72 @@
+/*
+ * artra_device.h - C interface between the host side and the HIP device
+ * side of libartra_hip.  All device memory is owned by an ArtraDevice; the
+ * host passes plain arrays (structure-of-arrays fields, flag masks, tables).
+ *
+ * Field layout on the device: one FP64 array per (time level, component),
+ * each covering the padded node grid in the solver's index order
+ * idx = (k * ny + j) * nx + i, i.e. U[level][component][idx].
+ */
+#ifndef ARTRA_DEVICE_H_
+#define ARTRA_DEVICE_H_
+
+#include <stdint.h>
+#include "artra_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ArtraDevice ArtraDevice;
+
+typedef struct {
+    int n[DIMS];             /* padded node counts */
+    int ng[DIMS];
+    int m[DIMS];             /* cells (after the at-least-two rule) */
+    int pin[DIMS][LIMIT];    /* interior range (ns[PIN]) */
+    int collapse;
+    int multidim;
+    int tScheme;
+    int sScheme;
+    int jacobMean;
+    int fluxSplit;
+    double gamma, gasR, cv, refMu, refT;
+    double d[DIMS], dd[DIMS];
+    int typeBC[NBC];
+    int normal[NBC][DIMS];
+    double varBC[NBC][VARBC];
+} ArtraDevParams;
+
+/* kernel timing categories */
+enum { ARTRA_T_SWEEP = 0, ARTRA_T_BC = 1, ARTRA_T_IB = 2, ARTRA_T_CFL = 3, ARTRA_T_COUNT = 4 };
+
+int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevice **out);
+void artra_dev_destroy(ArtraDevice *dev);
+int artra_dev_set_boundary(ArtraDevice *dev, const ArtraBcBox *boxes, int nbox, const int *group_start,
+                           int ngroup);
+int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *table);
+/* copy one time level: soa is [5][N] doubles */
+int artra_dev_upload(ArtraDevice *dev, int level, const double *soa);
+int artra_dev_download(ArtraDevice *dev, int level, double *soa);
+/* set every byte of the three levels to 0x01 (the solver's poison pattern) */
+int artra_dev_poison(ArtraDevice *dev);
+int artra_dev_treat_boundary(ArtraDevice *dev, int level);
+/* per-axis maxima of |u_s| + c over interior fluid nodes of level TO,
+ * folded with the given floors (body speeds) */
+int artra_dev_max_speed(ArtraDevice *dev, const double floor[DIMS], double vmax[DIMS]);
+/* one EvolveFluidDynamics step of size dt */
+int artra_dev_evolve(ArtraDevice *dev, double dt);
+int artra_dev_sync(ArtraDevice *dev);
+/* kernel timing: when on, every launch is bracketed by events on the
+ * solver's stream; totals are in milliseconds. */
+int artra_dev_set_timing(ArtraDevice *dev, int on);
+int artra_dev_get_timing(ArtraDevice *dev, double ms[ARTRA_T_COUNT], long launches[ARTRA_T_COUNT]);
+/* tuning knobs (0 keeps the default) */
+int artra_dev_set_tuning(ArtraDevice *dev, int march_segment, int xblock);
+const char *artra_dev_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARTRA_DEVICE_H_ */

@@ -1,0 +1,143 @@
+/*
+ * artra_host.h - internal interface of the host side of libartra_hip.
+ *
+ * The host side prepares a case exactly as the reference solver does before
+ * its time loop (case files -> Space/Model/Time, node flags, initial field,
+ * geometry), builds the static tables the device kernels consume (boundary
+ * boxes, immersed-boundary ghost stencils), and drives the device time loop.
+ * Everything here is plain C; the device side is reached through the
+ * functions declared in artra_device.h.
+ */
+#ifndef ARTRA_HOST_H_
+#define ARTRA_HOST_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include "artra_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+/* Record a formatted error message; returns the error code for chaining. */
+int artra_set_error(int code, const char *fmt, ...);
+const char *artra_error_text(void);
+
+/* ------------------------------------------------------------- utilities */
+/* Normalise a case-file line in place (case grammar of commons.c:26-84):
+ * strip the newline and everything after '#', collapse blanks. */
+void artra_parse_line(char *line);
+int artra_index(int k, int j, int i, int ny, int nx);
+int artra_map_node(Real s, Real smin, Real dds, int ng);
+int artra_confine(int n, int nmin, int nmax);
+Real artra_map_point(int n, Real smin, Real ds, int ng);
+int artra_in_box(int k, int j, int i, const int box[DIMS][LIMIT]);
+Real artra_dot(const Real a[DIMS], const Real b[DIMS]);
+Real artra_norm(const Real a[DIMS]);
+Real artra_dist2(const Real a[DIMS], const Real b[DIMS]);
+void artra_cross(const Real a[DIMS], const Real b[DIMS], Real out[DIMS]);
+void artra_normalize(int n, Real by, Real *v);
+void artra_orthogonal_space(const Real N[DIMS], Real Ta[DIMS], Real Tb[DIMS]);
+void artra_map_primitive(Real gamma, Real gasR, const Real U[DIMU], Real Uo[DIMUo]);
+void artra_map_conservative(Real gamma, const Real Uo[DIMUo], Real U[DIMU]);
+
+/* ------------------------------------------------------------ calculator */
+/* A compiled initial-condition expression (grammar of calculator.c:126-718). */
+typedef struct ArtraExpr ArtraExpr;
+int artra_expr_compile(const char *text, ArtraExpr **out);
+/* Evaluate with variables t, x, y, z; returns 0 on success.  On a domain
+ * error (ln of a non-positive number, division by zero, ...) the result is
+ * left at *ans unchanged and 1 is returned, as the reference does. */
+int artra_expr_eval(const ArtraExpr *e, Real t, Real x, Real y, Real z, Real *ans);
+void artra_expr_free(ArtraExpr *e);
+/* One-shot evaluation (used by tests): returns 0 on success. */
+int artra_calc(const char *text, Real t, Real x, Real y, Real z, Real *ans);
+
+/* -------------------------------------------------------------- case set-up */
+/* Read artracfd.case and artracfd.geo in dir (case_loader.c:32-253). */
+int artra_load_case(const char *dir, Time *time, Space *space, Model *model);
+/* Ghost counts, collapse flags, nondimensionalisation (cfd_parameters.c:28-186). */
+int artra_compute_parameters(Time *time, Space *space, Model *model);
+/* Index ranges, boundary normals, search path (domain_partition.c:19-150). */
+void artra_partition_domain(Space *space);
+/* Node and geometry storage (preprocess.c:47-59). */
+int artra_allocate(Space *space, Model *model);
+/* Initial field and geometry (initialization.c:38-162) up to, not including,
+ * the first boundary treatment. */
+int artra_initialize_domain(const char *dir, Time *time, Space *space, Model *model);
+/* Whole preparation: load, parameters, partition, allocate, initialise. */
+int artra_prepare_case(const char *dir, Time *time, Space *space, Model *model);
+/* Release everything artra_prepare_case allocated. */
+void artra_release_case(Space *space, Model *model, Time *time);
+
+/* --------------------------------------------------------------- geometry */
+int artra_read_stl(const char *fname, Polyhedron *poly);
+int artra_convert_polyhedron(Polyhedron *poly);
+void artra_transform_polyhedron(const Real O[DIMS], const Real scale[DIMS], const Real angle[DIMS],
+                                const Real offset[DIMS], Polyhedron *poly);
+void artra_compute_geometry_parameters(int collapse, Geometry *geo);
+Real artra_point_triangle_distance(const Real p[DIMS], const Real v0[DIMS], const Real e01[DIMS],
+                                   const Real e02[DIMS], Real para[DIMS]);
+int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, int *fid);
+void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS],
+                                  Real pm[DIMS], Real N[DIMS]);
+/* Node classification did/fid/lid/gst (immersed_boundary.c:65-264). */
+void artra_compute_geometric_field(Space *space, const Model *model);
+void artra_identify_geometry_state(Geometry *geo);
+
+/* ------------------------------------------------- device-side tables */
+/* One external-boundary box of TreatBoundary (boundary_treatment.c:51-63). */
+typedef struct {
+    int face;            /* PWB..PBB */
+    int layer;           /* r */
+    int box[DIMS][LIMIT];
+} ArtraBcBox;
+
+/* Boxes in the reference's order, empty boxes dropped.  Returns the count. */
+int artra_bc_boxes(const Space *space, ArtraBcBox *out, int cap);
+
+/* Launch groups: consecutive boxes that touch disjoint node sets and may run
+ * in one kernel.  group_start has ngroups+1 entries. */
+int artra_bc_groups(const Space *space, const ArtraBcBox *boxes, int nbox, int *group_start, int cap);
+
+/*
+ * Immersed-boundary ghost table.  For a stationary body the stencil of every
+ * ghost node is fixed, so the host precomputes, per ghost node and in the
+ * reference's exact summation order, the stencil node indices and weights of
+ * InverseDistanceWeighting (immersed_boundary.c:413-473) and the geometry of
+ * ReconstructFlow (immersed_boundary.c:357-412).
+ */
+typedef struct {
+    int node;            /* padded-grid index of the ghost node */
+    int kind;            /* 0: image-point reconstruction, 1: IDW of layer r-1 */
+    int first;           /* first stencil entry */
+    int count;           /* number of stencil entries */
+    int noslip;          /* cf > 0 */
+    int isothermal;      /* wall T >= 0 */
+    Real wallT;
+    Real wO;             /* weight of the boundary point (correction step) */
+    Real Vs[DIMS];       /* body velocity at the boundary point */
+    Real N[DIMS];
+    Real Ta[DIMS];
+    Real Tb[DIMS];
+} ArtraGhost;
+
+typedef struct {
+    int layers;          /* gl */
+    int *layer_start;    /* layers+1 offsets into ghosts (ghosts grouped by layer) */
+    ArtraGhost *ghosts;
+    int nghost;
+    int *stencil_node;
+    Real *stencil_w;
+    int nstencil;
+} ArtraGhostTable;
+
+int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTable *table);
+void artra_free_ghost_table(ArtraGhostTable *table);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARTRA_HOST_H_ */

@@ -1,0 +1,391 @@
+/*
+ * capi.c - the exported C-ABI of libartra_hip (declared in artra_hip.h).
+ *
+ * The resident solver keeps the field on the device between steps.  The
+ * drop-in symbols EvolveFluidDynamics and TreatBoundary keep one solver per
+ * node array, created on first use, and copy level TO (or tn) in and out
+ * around every call so that the caller's node array stays authoritative.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include <limits.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "artra_device.h"
+#include "artra_hip.h"
+#include "artra_host.h"
+
+struct ArtraHipSolver {
+    Space *space;
+    const Model *model;
+    ArtraDevice *dev;
+    ArtraGhostTable ghosts;
+    long N;
+    double *staging;          /* [5][N] host buffer for AoS <-> SoA */
+};
+
+struct ArtraCase {
+    Time time;
+    Space space;
+    Model model;
+};
+
+static int DevError(void)
+{
+    return artra_set_error(1, "%s", artra_dev_error());
+}
+
+const char *artra_hip_error(void)
+{
+    return artra_error_text();
+}
+
+/* -------------------------------------------------------- node array I/O */
+static void NodesToField(const Space *space, int tn, double *field, long N)
+{
+    const Node *node = space->node;
+    for (long q = 0; q < N; ++q) {
+        for (int c = 0; c < DIMU; ++c) field[c * N + q] = node[q].U[tn][c];
+    }
+}
+
+static void FieldToNodes(Space *space, int tn, const double *field, long N)
+{
+    Node *node = space->node;
+    for (long q = 0; q < N; ++q) {
+        for (int c = 0; c < DIMU; ++c) node[q].U[tn][c] = field[c * N + q];
+    }
+}
+
+static int Supported(const Space *space, const Model *model)
+{
+    if (0 != model->sState) return artra_set_error(2, "gravity source term is not supported");
+    if (0 != model->psi) return artra_set_error(2, "moving bodies are not supported");
+    for (int n = 0; n < space->geo.totN; ++n) {
+        const Polyhedron *p = space->geo.poly + n;
+        for (int s = 0; s < DIMS; ++s) {
+            if (0.0 != p->V[TO][s] || 0.0 != p->W[TO][s]) {
+                return artra_set_error(2, "body %d moves; only stationary bodies are supported", n);
+            }
+        }
+    }
+    if (model->refMu > 0.0 && OPTSPLIT == model->multidim && COLLAPSEN != space->part.collapse) {
+        return artra_set_error(2, "viscous flux on a collapsed split grid reads outside the grid");
+    }
+    return 0;
+}
+
+int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
+{
+    *out = NULL;
+    if (Supported(space, model)) return 2;
+    const Partition *part = &space->part;
+    ArtraHipSolver *s = calloc(1, sizeof *s);
+    if (NULL == s) return artra_set_error(1, "out of host memory");
+    s->space = space;
+    s->model = model;
+    s->N = (long)part->n[X] * part->n[Y] * part->n[Z];
+    s->staging = malloc((size_t)5 * s->N * sizeof(double));
+    uint8_t *fluid = malloc((size_t)s->N);
+    if (NULL == s->staging || NULL == fluid) {
+        free(fluid);
+        artra_hip_detach(s);
+        return artra_set_error(1, "out of host memory for %ld nodes", s->N);
+    }
+    for (long q = 0; q < s->N; ++q) fluid[q] = (0 == space->node[q].did);
+    ArtraDevParams par;
+    memset(&par, 0, sizeof par);
+    for (int d = 0; d < DIMS; ++d) {
+        par.n[d] = part->n[d];
+        par.ng[d] = part->ng[d];
+        par.m[d] = part->m[d];
+        par.pin[d][MIN] = part->ns[PIN][d][MIN];
+        par.pin[d][MAX] = part->ns[PIN][d][MAX];
+        par.d[d] = part->d[d];
+        par.dd[d] = part->dd[d];
+    }
+    par.collapse = part->collapse;
+    par.multidim = model->multidim;
+    par.tScheme = model->tScheme;
+    par.sScheme = model->sScheme;
+    par.jacobMean = model->jacobMean;
+    par.fluxSplit = model->fluxSplit;
+    par.gamma = model->gamma;
+    par.gasR = model->gasR;
+    par.cv = model->cv;
+    par.refMu = model->refMu;
+    par.refT = model->refT;
+    for (int p = 0; p < NBC; ++p) {
+        par.typeBC[p] = part->typeBC[p];
+        for (int d = 0; d < DIMS; ++d) par.normal[p][d] = part->N[p][d];
+        for (int v = 0; v < VARBC; ++v) par.varBC[p][v] = part->varBC[p][v];
+    }
+    const int rc = artra_dev_create(&par, fluid, &s->dev);
+    free(fluid);
+    if (rc) {
+        artra_hip_detach(s);
+        return DevError();
+    }
+    ArtraBcBox boxes[64];
+    int starts[65];
+    const int nbox = artra_bc_boxes(space, boxes, 64);
+    const int ngroup = (nbox >= 0) ? artra_bc_groups(space, boxes, nbox, starts, 64) : -1;
+    if (nbox < 0 || ngroup < 0) {
+        artra_hip_detach(s);
+        return artra_set_error(1, "boundary box table overflow");
+    }
+    if (artra_dev_set_boundary(s->dev, boxes, nbox, starts, ngroup)) {
+        artra_hip_detach(s);
+        return DevError();
+    }
+    if (artra_build_ghost_table(space, model, &s->ghosts)) {
+        artra_hip_detach(s);
+        return 1;
+    }
+    if (artra_dev_set_ghosts(s->dev, &s->ghosts) || artra_dev_poison(s->dev)) {
+        artra_hip_detach(s);
+        return DevError();
+    }
+    if (artra_hip_upload(s, TO)) {
+        artra_hip_detach(s);
+        return 1;
+    }
+    *out = s;
+    return 0;
+}
+
+int artra_hip_detach(ArtraHipSolver *s)
+{
+    if (NULL == s) return 0;
+    artra_dev_destroy(s->dev);
+    artra_free_ghost_table(&s->ghosts);
+    free(s->staging);
+    free(s);
+    return 0;
+}
+
+int artra_hip_treat_boundary(ArtraHipSolver *s, int tn)
+{
+    return artra_dev_treat_boundary(s->dev, tn) ? DevError() : 0;
+}
+
+int artra_hip_compute_dt(ArtraHipSolver *s, const Time *time, Real *dt)
+{
+    const Partition *part = &s->space->part;
+    const Geometry *geo = &s->space->geo;
+    /* body speeds first (solve.c:121-131) */
+    double floor[DIMS] = {0.0, 0.0, 0.0};
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *p = geo->poly + n;
+        const double V[DIMS] = {fabs(p->V[TO][X]) + fmax(fabs(p->W[TO][Y]), fabs(p->W[TO][Z])) * p->r,
+                                fabs(p->V[TO][Y]) + fmax(fabs(p->W[TO][Z]), fabs(p->W[TO][X])) * p->r,
+                                fabs(p->V[TO][Z]) + fmax(fabs(p->W[TO][X]), fabs(p->W[TO][Y])) * p->r};
+        for (int d = 0; d < DIMS; ++d) {
+            if (floor[d] < V[d]) floor[d] = V[d];
+        }
+    }
+    double vmax[DIMS];
+    if (artra_dev_max_speed(s->dev, floor, vmax)) return DevError();
+    const double tx = part->d[X] / vmax[X], ty = part->d[Y] / vmax[Y], tz = part->d[Z] / vmax[Z];
+    const double tyz = (ty < tz) ? ty : tz;
+    *dt = time->numCFL * ((tx < tyz) ? tx : tyz);
+    return 0;
+}
+
+int artra_hip_evolve(ArtraHipSolver *s, Real dt)
+{
+    return artra_dev_evolve(s->dev, dt) ? DevError() : 0;
+}
+
+int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_done, Real *dts)
+{
+    const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
+    Real rcInt = 0.0;
+    int done = 0;
+    while (time->now < time->end && time->stepC < time->stepN && done < max_steps) {
+        ++time->stepC;
+        Real dt;
+        if (artra_hip_compute_dt(s, time, &dt)) return 1;
+        if (rcInt + dt > tmInt) {
+            dt = tmInt - rcInt;
+            rcInt = 0.0;
+        } else {
+            rcInt = rcInt + dt;
+        }
+        time->now = time->now + dt;
+        if (time->now > time->end) {
+            dt = time->end - (time->now - dt);
+            time->now = time->end;
+        }
+        if (artra_hip_evolve(s, dt)) return 1;
+        if (dts) dts[done] = dt;
+        ++done;
+    }
+    if (steps_done) *steps_done = done;
+    return artra_hip_synchronize(s);
+}
+
+int artra_hip_download(ArtraHipSolver *s, int tn)
+{
+    if (artra_dev_download(s->dev, tn, s->staging)) return DevError();
+    FieldToNodes(s->space, tn, s->staging, s->N);
+    return 0;
+}
+
+int artra_hip_upload(ArtraHipSolver *s, int tn)
+{
+    NodesToField(s->space, tn, s->staging, s->N);
+    return artra_dev_upload(s->dev, tn, s->staging) ? DevError() : 0;
+}
+
+int artra_hip_read_field(ArtraHipSolver *s, int tn, Real *field)
+{
+    return artra_dev_download(s->dev, tn, field) ? DevError() : 0;
+}
+
+int artra_hip_write_field(ArtraHipSolver *s, int tn, const Real *field)
+{
+    return artra_dev_upload(s->dev, tn, field) ? DevError() : 0;
+}
+
+int artra_hip_synchronize(ArtraHipSolver *s)
+{
+    return artra_dev_sync(s->dev) ? DevError() : 0;
+}
+
+int artra_hip_set_timing(ArtraHipSolver *s, int on)
+{
+    return artra_dev_set_timing(s->dev, on) ? DevError() : 0;
+}
+
+int artra_hip_get_timing(ArtraHipSolver *s, double ms[4], long launches[4])
+{
+    return artra_dev_get_timing(s->dev, ms, launches) ? DevError() : 0;
+}
+
+int artra_hip_set_tuning(ArtraHipSolver *s, int march_segment)
+{
+    return artra_dev_set_tuning(s->dev, march_segment, 0) ? DevError() : 0;
+}
+
+/* ------------------------------------------------------------- drop-in */
+static struct {
+    Space *space;
+    Node *node;
+    const Model *model;
+    ArtraHipSolver *solver;
+} gDropIn;
+
+static ArtraHipSolver *DropInSolver(Space *space, const Model *model)
+{
+    if (gDropIn.solver && (gDropIn.space != space || gDropIn.node != space->node || gDropIn.model != model)) {
+        artra_hip_detach(gDropIn.solver);
+        gDropIn.solver = NULL;
+    }
+    if (NULL == gDropIn.solver) {
+        if (artra_hip_attach(space, model, &gDropIn.solver)) {
+            fprintf(stderr, "error: %s\n", artra_hip_error());
+            exit(EXIT_FAILURE);
+        }
+        gDropIn.space = space;
+        gDropIn.node = space->node;
+        gDropIn.model = model;
+    }
+    return gDropIn.solver;
+}
+
+void EvolveFluidDynamics(const Real dt, Space *space, const Model *model)
+{
+    ArtraHipSolver *s = DropInSolver(space, model);
+    if (artra_hip_upload(s, TO) || artra_hip_evolve(s, dt) || artra_hip_download(s, TO)) {
+        fprintf(stderr, "error: %s\n", artra_hip_error());
+        exit(EXIT_FAILURE);
+    }
+}
+
+void TreatBoundary(const int tn, Space *space, const Model *model)
+{
+    ArtraHipSolver *s = DropInSolver(space, model);
+    if (artra_hip_upload(s, tn) || artra_hip_treat_boundary(s, tn) || artra_hip_download(s, tn)) {
+        fprintf(stderr, "error: %s\n", artra_hip_error());
+        exit(EXIT_FAILURE);
+    }
+}
+
+/* ------------------------------------------------------ case preparation */
+int artra_case_open(const char *dir, ArtraCase **out)
+{
+    *out = NULL;
+    ArtraCase *c = calloc(1, sizeof *c);
+    if (NULL == c) return artra_set_error(1, "out of host memory");
+    const int rc = artra_prepare_case(dir, &c->time, &c->space, &c->model);
+    if (rc) {
+        artra_release_case(&c->space, &c->model, &c->time);
+        free(c);
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+void artra_case_close(ArtraCase *c)
+{
+    if (NULL == c) return;
+    if (gDropIn.space == &c->space) {
+        artra_hip_detach(gDropIn.solver);
+        memset(&gDropIn, 0, sizeof gDropIn);
+    }
+    artra_release_case(&c->space, &c->model, &c->time);
+    free(c);
+}
+
+Space *artra_case_space(ArtraCase *c) { return &c->space; }
+Model *artra_case_model(ArtraCase *c) { return &c->model; }
+Time *artra_case_time(ArtraCase *c) { return &c->time; }
+
+int artra_case_grid(const ArtraCase *c, int n[3], int ng[3], int pin[6])
+{
+    const Partition *part = &c->space.part;
+    for (int d = 0; d < DIMS; ++d) {
+        n[d] = part->n[d];
+        ng[d] = part->ng[d];
+        pin[2 * d] = part->ns[PIN][d][MIN];
+        pin[2 * d + 1] = part->ns[PIN][d][MAX];
+    }
+    return 0;
+}
+
+int artra_case_get_field(const ArtraCase *c, int tn, Real *field)
+{
+    const Partition *part = &c->space.part;
+    NodesToField(&c->space, tn, field, (long)part->n[X] * part->n[Y] * part->n[Z]);
+    return 0;
+}
+
+int artra_case_set_field(ArtraCase *c, int tn, const Real *field)
+{
+    const Partition *part = &c->space.part;
+    FieldToNodes(&c->space, tn, field, (long)part->n[X] * part->n[Y] * part->n[Z]);
+    return 0;
+}
+
+int artra_case_get_flags(const ArtraCase *c, int *flags)
+{
+    const Partition *part = &c->space.part;
+    const long N = (long)part->n[X] * part->n[Y] * part->n[Z];
+    for (long q = 0; q < N; ++q) {
+        const Node *nd = c->space.node + q;
+        flags[q] = nd->did;
+        flags[N + q] = nd->fid;
+        flags[2 * N + q] = nd->lid;
+        flags[3 * N + q] = nd->gst;
+    }
+    return 0;
+}
+
+int artra_calc_expression(const char *text, Real x, Real y, Real z, Real *ans)
+{
+    return artra_calc(text, 0.0, x, y, z, ans);
+}

@@ -1,0 +1,547 @@
+/*
+ * geometry.c - body geometry on the host: triangulated-surface tables,
+ * mass properties, closest-point queries and the node classification that
+ * marks bodies and their ghost layers on the grid.
+ *
+ *   artra_convert_polyhedron            computational_geometry.c:37-136
+ *   artra_transform_polyhedron          computational_geometry.c:162-261
+ *   artra_compute_geometry_parameters   computational_geometry.c:262-448
+ *   artra_point_in_polyhedron           computational_geometry.c:463-506
+ *   artra_point_triangle_distance       computational_geometry.c:511-688
+ *   closest point and pseudo-normal     computational_geometry.c:689-783
+ *   artra_compute_geometric_field       immersed_boundary.c:65-264
+ *
+ * Bodies are stationary in every supported case, so all of this runs once,
+ * before the time loop; the device only consumes the resulting tables.
+ */
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "artra_host.h"
+
+static const Real kPi = 3.14159265358979323846;
+
+/* --------------------------------------------------- surface tables */
+static int FindOrAddVertex(Polyhedron *poly, const Real v[DIMS])
+{
+    for (int n = 0; n < poly->vertN; ++n) {
+        if (v[X] == poly->v[n][X] && v[Y] == poly->v[n][Y] && v[Z] == poly->v[n][Z]) return n;
+    }
+    for (int s = 0; s < DIMS; ++s) poly->v[poly->vertN][s] = v[s];
+    return poly->vertN++;
+}
+
+/* Edges are keyed (larger vertex, smaller vertex); slot 2 holds the first
+ * face that uses the edge and slot 3 the second one. */
+static void AddEdgeOf(Polyhedron *poly, int a, int b, int face)
+{
+    const int hi = (a > b) ? a : b;
+    const int lo = (a > b) ? b : a;
+    for (int n = 0; n < poly->edgeN; ++n) {
+        if (hi == poly->e[n][0] && lo == poly->e[n][1]) {
+            poly->e[n][3] = face;
+            return;
+        }
+    }
+    poly->e[poly->edgeN][0] = hi;
+    poly->e[poly->edgeN][1] = lo;
+    poly->e[poly->edgeN][2] = face;
+    poly->edgeN++;
+}
+
+static int EdgeKeyCompare(const void *pa, const void *pb)
+{
+    const int *a = pa, *b = pb;
+    if (a[0] != b[0]) return (a[0] < b[0]) ? -1 : 1;
+    if (a[1] != b[1]) return (a[1] < b[1]) ? -1 : 1;
+    return 0;
+}
+
+int artra_convert_polyhedron(Polyhedron *poly)
+{
+    const int fN = poly->faceN;
+    poly->f = calloc(fN, sizeof(*poly->f));
+    poly->Nf = calloc(fN, sizeof(*poly->Nf));
+    poly->e = calloc(3 * fN, sizeof(*poly->e));
+    poly->Ne = calloc(3 * fN, sizeof(*poly->Ne));
+    poly->v = calloc(3 * fN, sizeof(*poly->v));
+    poly->Nv = calloc(3 * fN, sizeof(*poly->Nv));
+    if (!poly->f || !poly->Nf || !poly->e || !poly->Ne || !poly->v || !poly->Nv) {
+        return artra_set_error(1, "out of host memory for polyhedron tables");
+    }
+    poly->vertN = 0;
+    poly->edgeN = 0;
+    for (int n = 0; n < fN; ++n) {
+        poly->f[n][0] = FindOrAddVertex(poly, poly->facet[n].v0);
+        poly->f[n][1] = FindOrAddVertex(poly, poly->facet[n].v1);
+        poly->f[n][2] = FindOrAddVertex(poly, poly->facet[n].v2);
+        AddEdgeOf(poly, poly->f[n][0], poly->f[n][1], n);
+        AddEdgeOf(poly, poly->f[n][1], poly->f[n][2], n);
+        AddEdgeOf(poly, poly->f[n][2], poly->f[n][0], n);
+    }
+    /* edge keys are unique, so any correct sort gives the reference's order */
+    qsort(poly->e, poly->edgeN, sizeof(*poly->e), EdgeKeyCompare);
+    free(poly->facet);
+    poly->facet = NULL;
+    return 0;
+}
+
+static int FindEdgeOf(const Polyhedron *poly, int a, int b)
+{
+    const int key[2] = {(a > b) ? a : b, (a > b) ? b : a};
+    int lo = 0, hi = poly->edgeN - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) / 2;
+        const int c = EdgeKeyCompare(key, poly->e[mid]);
+        if (0 == c) return mid;
+        if (c > 0) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+static void TriangleOf(int fid, const Polyhedron *poly, Real v0[DIMS], Real v1[DIMS], Real v2[DIMS],
+                       Real e01[DIMS], Real e02[DIMS])
+{
+    for (int s = 0; s < DIMS; ++s) {
+        v0[s] = poly->v[poly->f[fid][0]][s];
+        v1[s] = poly->v[poly->f[fid][1]][s];
+        v2[s] = poly->v[poly->f[fid][2]][s];
+        e01[s] = v1[s] - v0[s];
+        e02[s] = v2[s] - v0[s];
+    }
+}
+
+/* ---------------------------------------------------- transformation */
+static void TransformPoints(const Real O[DIMS], const Real scale[DIMS], const Real rot[DIMS][DIMS],
+                            const Real offset[DIMS], Real box[DIMS][LIMIT], int count, Real (*v)[DIMS])
+{
+    for (int n = 0; n < count; ++n) {
+        Real t[DIMS];
+        for (int s = 0; s < DIMS; ++s) t[s] = (v[n][s] - O[s]) * scale[s];
+        for (int s = 0; s < DIMS; ++s) v[n][s] = artra_dot(rot[s], t);
+        for (int s = 0; s < DIMS; ++s) {
+            v[n][s] = v[n][s] + offset[s] + O[s];
+            box[s][MIN] = (box[s][MIN] < v[n][s]) ? box[s][MIN] : v[n][s];
+            box[s][MAX] = (box[s][MAX] > v[n][s]) ? box[s][MAX] : v[n][s];
+        }
+    }
+}
+
+static void RotateVectors(const Real rot[DIMS][DIMS], int count, Real (*N)[DIMS])
+{
+    for (int n = 0; n < count; ++n) {
+        const Real t[DIMS] = {N[n][X], N[n][Y], N[n][Z]};
+        for (int s = 0; s < DIMS; ++s) N[n][s] = artra_dot(rot[s], t);
+    }
+}
+
+static Real InertiaAbout(const Real a[DIMS], const Real I[DIMS][DIMS])
+{
+    return I[X][X] * a[X] * a[X] + I[Y][Y] * a[Y] * a[Y] + I[Z][Z] * a[Z] * a[Z] +
+           2.0 * I[X][Y] * a[X] * a[Y] + 2.0 * I[Y][Z] * a[Y] * a[Z] + 2.0 * I[Z][X] * a[Z] * a[X];
+}
+
+void artra_transform_polyhedron(const Real O[DIMS], const Real scale[DIMS], const Real angle[DIMS],
+                                const Real offset[DIMS], Polyhedron *poly)
+{
+    const Real Sx = sin(angle[X]), Sy = sin(angle[Y]), Sz = sin(angle[Z]);
+    const Real Cx = cos(angle[X]), Cy = cos(angle[Y]), Cz = cos(angle[Z]);
+    /* rotation about x, then y, then z */
+    const Real rot[DIMS][DIMS] = {
+        {Cy * Cz, -Cx * Sz + Sx * Sy * Cz, Sx * Sz + Cx * Sy * Cz},
+        {Cy * Sz, Cx * Cz + Sx * Sy * Sz, -Sx * Cz + Cx * Sy * Sz},
+        {-Sy, Sx * Cy, Cx * Cy}};
+    const Real inv[DIMS][DIMS] = {{rot[0][0], rot[1][0], rot[2][0]},
+                                  {rot[0][1], rot[1][1], rot[2][1]},
+                                  {rot[0][2], rot[1][2], rot[2][2]}};
+    const Real h = 1.0 / sqrt(2.0);
+    const Real axes[6][DIMS] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {h, h, 0}, {0, h, h}, {h, 0, h}};
+    for (int s = 0; s < DIMS; ++s) {
+        poly->box[s][MIN] = FLT_MAX;
+        poly->box[s][MAX] = FLT_MIN;
+    }
+    TransformPoints(O, scale, rot, offset, poly->box, poly->vertN, poly->v);
+    RotateVectors(rot, poly->faceN, poly->Nf);
+    RotateVectors(rot, poly->edgeN, poly->Ne);
+    RotateVectors(rot, poly->vertN, poly->Nv);
+    Real I[6];
+    for (int n = 0; n < 6; ++n) {
+        const Real a[DIMS] = {artra_dot(inv[X], axes[n]), artra_dot(inv[Y], axes[n]), artra_dot(inv[Z], axes[n])};
+        I[n] = InertiaAbout(a, (const Real (*)[DIMS])poly->I);
+    }
+    poly->I[X][X] = I[0];  poly->I[X][Y] = -I[3]; poly->I[X][Z] = -I[5];
+    poly->I[Y][X] = -I[3]; poly->I[Y][Y] = I[1];  poly->I[Y][Z] = -I[4];
+    poly->I[Z][X] = -I[5]; poly->I[Z][Y] = -I[4]; poly->I[Z][Z] = I[2];
+    Real Oc[1][DIMS] = {{poly->O[X], poly->O[Y], poly->O[Z]}};
+    TransformPoints(O, scale, rot, offset, poly->box, 1, Oc);
+    for (int s = 0; s < DIMS; ++s) poly->O[s] = Oc[0][s];
+}
+
+/* ------------------------------------------------------ mass properties */
+static void SphereParameters(int collapse, Polyhedron *poly)
+{
+    for (int s = 0; s < DIMS; ++s) {
+        poly->box[s][MIN] = poly->O[s] - poly->r;
+        poly->box[s][MAX] = poly->O[s] + poly->r;
+    }
+    Real k;
+    if (COLLAPSEN == collapse) {
+        poly->area = 4.0 * kPi * poly->r * poly->r;
+        poly->volume = (4.0 / 3.0) * kPi * poly->r * poly->r * poly->r;
+        k = 0.4;
+    } else {
+        poly->area = 2.0 * kPi * poly->r;
+        poly->volume = kPi * poly->r * poly->r;
+        k = 0.5;
+    }
+    k = k * poly->r * poly->r * poly->volume;
+    memset(poly->I, 0, sizeof poly->I);
+    poly->I[X][X] = poly->I[Y][Y] = poly->I[Z][Z] = k;
+}
+
+/* Surface-integral mass properties (Eberly, "Polyhedral mass properties")
+ * and angle-weighted vertex pseudo-normals (Baerentzen & Aanaes 2005). */
+static void PolyhedronParameters(int collapse, Polyhedron *poly)
+{
+    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], Nf[DIMS], ang[DIMS];
+    Real O[DIMS] = {0.0}, area = 0.0, volume = 0.0, I[6] = {0.0};
+    Real f[DIMS][DIMS], g[DIMS][DIMS], box[LIMIT][DIMS];
+    for (int s = 0; s < DIMS; ++s) {
+        box[MIN][s] = FLT_MAX;
+        box[MAX][s] = FLT_MIN;
+    }
+    memset(poly->Nv, 0, poly->vertN * sizeof(*poly->Nv));
+    for (int n = 0; n < poly->vertN; ++n) {
+        for (int s = 0; s < DIMS; ++s) {
+            box[MIN][s] = (box[MIN][s] < poly->v[n][s]) ? box[MIN][s] : poly->v[n][s];
+            box[MAX][s] = (box[MAX][s] > poly->v[n][s]) ? box[MAX][s] : poly->v[n][s];
+        }
+    }
+    for (int n = 0; n < poly->faceN; ++n) {
+        TriangleOf(n, poly, v0, v1, v2, e01, e02);
+        artra_cross(e01, e02, Nf);
+        for (int s = 0; s < DIMS; ++s) {
+            const Real t0 = v0[s] + v1[s];
+            f[0][s] = t0 + v2[s];
+            const Real t1 = v0[s] * v0[s];
+            const Real t2 = t1 + v1[s] * t0;
+            f[1][s] = t2 + v2[s] * f[0][s];
+            f[2][s] = v0[s] * t1 + v1[s] * t2 + v2[s] * f[1][s];
+            g[0][s] = f[1][s] + v0[s] * (f[0][s] + v0[s]);
+            g[1][s] = f[1][s] + v1[s] * (f[0][s] + v1[s]);
+            g[2][s] = f[1][s] + v2[s] * (f[0][s] + v2[s]);
+        }
+        area = area + artra_norm(Nf);
+        volume = volume + Nf[X] * f[0][X];
+        for (int s = 0; s < DIMS; ++s) O[s] = O[s] + Nf[s] * f[1][s];
+        I[0] = I[0] + Nf[X] * f[2][X];
+        I[1] = I[1] + Nf[Y] * f[2][Y];
+        I[2] = I[2] + Nf[Z] * f[2][Z];
+        I[3] = I[3] + Nf[X] * (v0[Y] * g[0][X] + v1[Y] * g[1][X] + v2[Y] * g[2][X]);
+        I[4] = I[4] + Nf[Y] * (v0[Z] * g[0][Y] + v1[Z] * g[1][Y] + v2[Z] * g[2][Y]);
+        I[5] = I[5] + Nf[Z] * (v0[X] * g[0][Z] + v1[X] * g[1][Z] + v2[X] * g[2][Z]);
+        artra_normalize(DIMS, artra_norm(Nf), Nf);
+        /* interior angles from the law of cosines */
+        const Real e12[DIMS] = {v2[X] - v1[X], v2[Y] - v1[Y], v2[Z] - v1[Z]};
+        const Real l[DIMS] = {artra_dot(e01, e01), artra_dot(e02, e02), artra_dot(e12, e12)};
+        ang[0] = acos((l[0] + l[1] - l[2]) / (2.0 * sqrt(l[0] * l[1])));
+        ang[1] = acos((l[0] + l[2] - l[1]) / (2.0 * sqrt(l[0] * l[2])));
+        ang[2] = kPi - ang[0] - ang[1];
+        for (int v = 0; v < POLYN; ++v) {
+            for (int s = 0; s < DIMS; ++s) {
+                poly->Nv[poly->f[n][v]][s] = poly->Nv[poly->f[n][v]][s] + ang[v] * Nf[s];
+            }
+        }
+        for (int s = 0; s < DIMS; ++s) poly->Nf[n][s] = Nf[s];
+    }
+    area = area * (1.0 / 2.0);
+    volume = volume * (1.0 / 6.0);
+    for (int s = 0; s < DIMS; ++s) O[s] = O[s] * (1.0 / 24.0);
+    for (int q = 0; q < 3; ++q) I[q] = I[q] * (1.0 / 60.0);
+    for (int q = 3; q < 6; ++q) I[q] = I[q] * (1.0 / 120.0);
+    for (int s = 0; s < DIMS; ++s) O[s] = O[s] / volume;
+    poly->area = (COLLAPSEN == collapse) ? area : area - 2.0 * volume;
+    poly->volume = volume;
+    for (int s = 0; s < DIMS; ++s) poly->O[s] = O[s];
+    poly->I[X][X] = I[1] + I[2] - volume * (O[Y] * O[Y] + O[Z] * O[Z]);
+    poly->I[X][Y] = -I[3] + volume * O[X] * O[Y];
+    poly->I[X][Z] = -I[5] + volume * O[Z] * O[X];
+    poly->I[Y][X] = poly->I[X][Y];
+    poly->I[Y][Y] = I[0] + I[2] - volume * (O[Z] * O[Z] + O[X] * O[X]);
+    poly->I[Y][Z] = -I[4] + volume * O[Y] * O[Z];
+    poly->I[Z][X] = poly->I[X][Z];
+    poly->I[Z][Y] = poly->I[Y][Z];
+    poly->I[Z][Z] = I[0] + I[1] - volume * (O[X] * O[X] + O[Y] * O[Y]);
+    for (int s = 0; s < DIMS; ++s) {
+        poly->box[s][MIN] = box[MIN][s];
+        poly->box[s][MAX] = box[MAX][s];
+    }
+    poly->r = sqrt(artra_dist2(box[MIN], box[MAX]));
+    for (int n = 0; n < poly->vertN; ++n) artra_normalize(DIMS, artra_norm(poly->Nv[n]), poly->Nv[n]);
+    for (int n = 0; n < poly->edgeN; ++n) {
+        for (int s = 0; s < DIMS; ++s) poly->Ne[n][s] = poly->Nf[poly->e[n][2]][s] + poly->Nf[poly->e[n][3]][s];
+        artra_normalize(DIMS, artra_norm(poly->Ne[n]), poly->Ne[n]);
+    }
+}
+
+void artra_compute_geometry_parameters(int collapse, Geometry *geo)
+{
+    for (int n = 0; n < geo->sphN; ++n) SphereParameters(collapse, geo->poly + n);
+    for (int n = geo->sphN; n < geo->totN; ++n) PolyhedronParameters(collapse, geo->poly + n);
+}
+
+/* ------------------------------------------------- closest point queries */
+/*
+ * Squared distance from p to the triangle v0 + s*e01 + t*e02 and the
+ * barycentric parameters (1-s-t, s, t) of the closest point, by the region
+ * classification of Eberly, "Distance between point and triangle in 3D".
+ */
+Real artra_point_triangle_distance(const Real p[DIMS], const Real v0[DIMS], const Real e01[DIMS],
+                                   const Real e02[DIMS], Real para[DIMS])
+{
+    const Real D[DIMS] = {v0[X] - p[X], v0[Y] - p[Y], v0[Z] - p[Z]};
+    const Real a = artra_dot(e01, e01), b = artra_dot(e01, e02), c = artra_dot(e02, e02);
+    const Real d = artra_dot(e01, D), e = artra_dot(e02, D), f = artra_dot(D, D);
+    const Real det = a * c - b * b;
+    Real s = b * e - c * d;
+    Real t = b * d - a * e;
+    Real dist;
+    /* the closest point on an edge parameterised from its start */
+    #define ON_E01(sv) do { s = (sv); dist = d * s + f; } while (0)
+    #define ON_E02(tv) do { t = (tv); dist = e * t + f; } while (0)
+    #define INTERIOR() (s * (a * s + b * t + 2.0 * d) + t * (b * s + c * t + 2.0 * e) + f)
+    if (s + t <= det) {
+        if (s < 0.0) {
+            if (t < 0.0) {                                    /* region 4 */
+                if (d < 0.0) {
+                    t = 0.0;
+                    if (-d >= a) { s = 1.0; dist = a + 2.0 * d + f; } else ON_E01(-d / a);
+                } else {
+                    s = 0.0;
+                    if (e >= 0.0) { t = 0.0; dist = f; }
+                    else if (-e >= c) { t = 1.0; dist = c + 2.0 * e + f; }
+                    else ON_E02(-e / c);
+                }
+            } else {                                          /* region 3 */
+                s = 0.0;
+                if (e >= 0.0) { t = 0.0; dist = f; }
+                else if (-e >= c) { t = 1.0; dist = c + 2.0 * e + f; }
+                else ON_E02(-e / c);
+            }
+        } else if (t < 0.0) {                                 /* region 5 */
+            t = 0.0;
+            if (d >= 0.0) { s = 0.0; dist = f; }
+            else if (-d >= a) { s = 1.0; dist = a + 2.0 * d + f; }
+            else ON_E01(-d / a);
+        } else {                                              /* region 0 */
+            s = s / det;
+            t = t / det;
+            dist = INTERIOR();
+        }
+    } else if (s < 0.0) {                                     /* region 2 */
+        const Real t0 = b + d, t1 = c + e;
+        if (t1 > t0) {
+            const Real num = t1 - t0, den = a - 2.0 * b + c;
+            if (num >= den) { s = 1.0; t = 0.0; dist = a + 2.0 * d + f; }
+            else { s = num / den; t = 1.0 - s; dist = INTERIOR(); }
+        } else {
+            s = 0.0;
+            if (t1 <= 0.0) { t = 1.0; dist = c + 2.0 * e + f; }
+            else if (e >= 0.0) { t = 0.0; dist = f; }
+            else ON_E02(-e / c);
+        }
+    } else if (t < 0.0) {                                     /* region 6 */
+        const Real t0 = b + e, t1 = a + d;
+        if (t1 > t0) {
+            const Real num = t1 - t0, den = a - 2.0 * b + c;
+            if (num >= den) { t = 1.0; s = 0.0; dist = c + 2.0 * e + f; }
+            else { t = num / den; s = 1.0 - t; dist = INTERIOR(); }
+        } else {
+            t = 0.0;
+            if (t1 <= 0.0) { s = 1.0; dist = a + 2.0 * d + f; }
+            else if (d >= 0.0) { s = 0.0; dist = f; }
+            else ON_E01(-d / a);
+        }
+    } else {                                                  /* region 1 */
+        const Real num = c + e - b - d;
+        if (num <= 0.0) { s = 0.0; t = 1.0; dist = c + 2.0 * e + f; }
+        else {
+            const Real den = a - 2.0 * b + c;
+            if (num >= den) { s = 1.0; t = 0.0; dist = a + 2.0 * d + f; }
+            else { s = num / den; t = 1.0 - s; dist = INTERIOR(); }
+        }
+    }
+    #undef ON_E01
+    #undef ON_E02
+    #undef INTERIOR
+    para[0] = 1.0 - s - t;
+    para[1] = s;
+    para[2] = t;
+    return (dist < 0.0) ? 0.0 : dist;
+}
+
+/* Closest point on facet fid and the matching face, edge or vertex
+ * pseudo-normal, chosen by exact tests on the barycentric parameters. */
+static void ClosestOnFacet(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS], Real N[DIMS])
+{
+    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
+    const int vid[3] = {poly->f[fid][0], poly->f[fid][1], poly->f[fid][2]};
+    TriangleOf(fid, poly, v0, v1, v2, e01, e02);
+    artra_point_triangle_distance(p, v0, e01, e02, para);
+    const Real *normal;
+    int edge;
+    if (0.0 == para[1]) {
+        if (0.0 == para[2]) {
+            for (int s = 0; s < DIMS; ++s) pi[s] = v0[s];
+            normal = poly->Nv[vid[0]];
+        } else if (1.0 == para[2]) {
+            for (int s = 0; s < DIMS; ++s) pi[s] = v2[s];
+            normal = poly->Nv[vid[2]];
+        } else {
+            for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[2] * e02[s];
+            edge = FindEdgeOf(poly, vid[0], vid[2]);
+            normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
+        }
+    } else if (1.0 == para[1]) {
+        for (int s = 0; s < DIMS; ++s) pi[s] = v1[s];
+        normal = poly->Nv[vid[1]];
+    } else if (0.0 == para[2]) {
+        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s];
+        edge = FindEdgeOf(poly, vid[0], vid[1]);
+        normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
+    } else if (0.0 == para[0]) {
+        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s] + para[2] * e02[s];
+        edge = FindEdgeOf(poly, vid[1], vid[2]);
+        normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
+    } else {
+        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s] + para[2] * e02[s];
+        normal = poly->Nf[fid];
+    }
+    for (int s = 0; s < DIMS; ++s) N[s] = normal[s];
+}
+
+int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, int *fid)
+{
+    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
+    Real best = FLT_MAX;
+    int cid = 0;
+    for (int n = 0; n < poly->faceN; ++n) {
+        TriangleOf(n, poly, v0, v1, v2, e01, e02);
+        const Real d2 = artra_point_triangle_distance(p, v0, e01, e02, para);
+        if (best > d2) { /* the first facet at the minimum wins */
+            best = d2;
+            cid = n;
+        }
+    }
+    *fid = cid;
+    Real pi[DIMS], N[DIMS];
+    ClosestOnFacet(p, cid, poly, pi, N);
+    const Real r[DIMS] = {p[X] - pi[X], p[Y] - pi[Y], p[Z] - pi[Z]};
+    return !(0.0 < artra_dot(r, N)); /* inside or on the surface */
+}
+
+/* Boundary point pi, image point pm = 2 pi - p and normal N of a ghost node. */
+void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS],
+                                  Real pm[DIMS], Real N[DIMS])
+{
+    if (0 >= poly->faceN) {
+        for (int s = 0; s < DIMS; ++s) N[s] = p[s] - poly->O[s];
+        Real dist = artra_norm(N);
+        artra_normalize(DIMS, dist, N);
+        dist = poly->r - dist;
+        for (int s = 0; s < DIMS; ++s) pi[s] = p[s] + dist * N[s];
+    } else {
+        ClosestOnFacet(p, fid, poly, pi, N);
+    }
+    for (int s = 0; s < DIMS; ++s) pm[s] = pi[s] + pi[s] - p[s];
+}
+
+/* ------------------------------------------------- node classification */
+/* Layer (1..gl) of the first search-path neighbour that satisfies the test,
+ * or 0.  sameAs != 0 looks for a node whose did equals `did`; otherwise for
+ * one whose did differs from it. */
+static int InterState(const Partition *part, const Node *node, int k, int j, int i, int did, int sameAs)
+{
+    for (int n = 0; n < part->pathSep[0]; ++n) {
+        const int kh = k + part->path[n][Z], jh = j + part->path[n][Y], ih = i + part->path[n][X];
+        if (!artra_in_box(kh, jh, ih, part->ns[PIN])) continue;
+        const int other = node[artra_index(kh, jh, ih, part->n[Y], part->n[X])].did;
+        if ((sameAs && other == did) || (!sameAs && other != did)) {
+            for (int r = 1; r <= part->gl; ++r) {
+                if (part->pathSep[r] > n) return r;
+            }
+        }
+    }
+    return 0;
+}
+
+void artra_compute_geometric_field(Space *space, const Model *model)
+{
+    (void)model;
+    const Partition *part = &space->part;
+    Node *node = space->node;
+    const Geometry *geo = &space->geo;
+    /* keep the previous domain field in gst (immersed_boundary.c:72-113) */
+    for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
+        for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
+            for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
+                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                const int gid = nd->did;
+                nd->gst = nd->did;
+                if (0 == gid) {
+                    nd->fid = 0;
+                    continue;
+                }
+                if (1 == geo->poly[gid - 1].state) continue;
+                if (0 < nd->lid) nd->did = 0;
+            }
+    /* nodes inside or on each body (immersed_boundary.c:128-182) */
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        if (1 == poly->state) continue;
+        int box[DIMS][LIMIT];
+        for (int s = 0; s < DIMS; ++s) {
+            const int lo = part->ns[PIN][s][MIN], hi = part->ns[PIN][s][MAX];
+            box[s][MIN] = artra_confine(artra_map_node(poly->box[s][MIN], part->domain[s][MIN], part->dd[s],
+                                                       part->ng[s]), lo, hi);
+            box[s][MAX] = artra_confine(artra_map_node(poly->box[s][MAX], part->domain[s][MIN], part->dd[s],
+                                                       part->ng[s]), lo, hi) + 1;
+        }
+        for (int k = box[Z][MIN]; k < box[Z][MAX]; ++k)
+            for (int j = box[Y][MIN]; j < box[Y][MAX]; ++j)
+                for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
+                    Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                    if (0 != nd->did) continue;
+                    const Real p[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                                          artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                                          artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+                    if (0 >= poly->faceN) {
+                        if (poly->r * poly->r >= artra_dist2(poly->O, p)) {
+                            nd->did = n + 1;
+                            nd->fid = 0;
+                        }
+                    } else {
+                        int fid = 0;
+                        if (artra_point_in_polyhedron(p, poly, &fid)) {
+                            nd->did = n + 1;
+                            nd->fid = fid;
+                        }
+                    }
+                }
+    }
+    /* interfacial and ghost layers (immersed_boundary.c:183-225).  Newly
+     * joined fluid nodes only arise for moving bodies, which are rejected
+     * when the case is loaded. */
+    for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
+        for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
+            for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
+                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                nd->lid = 0;
+                nd->gst = 0;
+                if (0 == nd->did) continue;
+                nd->lid = InterState(part, node, k, j, i, nd->did, 0);
+                if (0 < nd->lid) nd->gst = InterState(part, node, k, j, i, 0, 1);
+            }
+}

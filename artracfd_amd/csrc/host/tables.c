@@ -1,0 +1,292 @@
+/*
+ * tables.c - static tables consumed by the device boundary kernels.
+ *
+ * External boundary: TreatBoundary (boundary_treatment.c:28-65) visits, for
+ * r = 0..R and faces W E S N F B in that order, a box that extends r layers
+ * into the ghost region tangentially; later boxes read values that earlier
+ * boxes have just written at edges and corners.  artra_bc_boxes lists the
+ * boxes in that order; artra_bc_groups packs consecutive boxes whose node
+ * sets (written and read) do not interact, so one kernel launch can serve a
+ * whole group with results identical to the sequential order.
+ *
+ * Immersed boundary: for stationary bodies every ghost node's interpolation
+ * stencil is fixed.  artra_build_ghost_table replays the neighbour search of
+ * InverseDistanceWeighting (immersed_boundary.c:413-473) once, recording the
+ * stencil nodes and inverse-distance weights in summation order, and the
+ * boundary-point data of ReconstructFlow (immersed_boundary.c:357-412).
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "artra_host.h"
+
+/* --------------------------------------------------- external boundary */
+int artra_bc_boxes(const Space *space, ArtraBcBox *out, int cap)
+{
+    const Partition *part = &space->part;
+    const int R = (part->ng[X] > part->ng[Y]) ? ((part->ng[X] > part->ng[Z]) ? part->ng[X] : part->ng[Z])
+                                              : ((part->ng[Y] > part->ng[Z]) ? part->ng[Y] : part->ng[Z]);
+    int count = 0;
+    for (int r = 0; r <= R; ++r) {
+        for (int p = PWB; p <= PBB; ++p) {
+            /* the physical layer of a periodic face is computed, not treated */
+            if (0 == r && PERIODIC == part->typeBC[p]) continue;
+            ArtraBcBox b;
+            b.face = p;
+            b.layer = r;
+            int empty = 0;
+            for (int s = 0; s < DIMS; ++s) {
+                const int N = part->N[p][s];
+                const int reach = (r < part->ng[s]) ? r : part->ng[s];
+                b.box[s][MIN] = part->ns[p][s][MIN] + reach * (N - !N);
+                b.box[s][MAX] = part->ns[p][s][MAX] + reach * (N + !N) - (part->ng[s] < r) * (!!N);
+                if (b.box[s][MIN] >= b.box[s][MAX]) empty = 1;
+            }
+            if (empty) continue;
+            if (count >= cap) return -1;
+            out[count++] = b;
+        }
+    }
+    return count;
+}
+
+typedef struct {
+    int lo[DIMS];
+    int hi[DIMS];
+} Region;
+
+static int Overlap(const Region *a, const Region *b)
+{
+    for (int s = 0; s < DIMS; ++s) {
+        if (a->hi[s] <= b->lo[s] || b->hi[s] <= a->lo[s]) return 0;
+    }
+    return 1;
+}
+
+/* Nodes a box writes and the (up to two) shifted regions it reads. */
+static int BoxRegions(const Partition *part, const ArtraBcBox *b, Region *write, Region read[2])
+{
+    const int p = b->face, r = b->layer;
+    int shift[2][DIMS] = {{0}};
+    int nread = 0;
+    const int type = part->typeBC[p];
+    if (r > 0) {
+        if (SLIPWALL == type || NOSLIPWALL == type) {
+            for (int s = 0; s < DIMS; ++s) {
+                shift[0][s] = -r * part->N[p][s];
+                shift[1][s] = -2 * r * part->N[p][s];
+            }
+            nread = 2;
+        } else if (PERIODIC == type) {
+            for (int s = 0; s < DIMS; ++s) shift[0][s] = -part->m[s] * part->N[p][s];
+            nread = 1;
+        } else {
+            for (int s = 0; s < DIMS; ++s) shift[0][s] = -part->N[p][s];
+            nread = 1;
+        }
+    } else if (INFLOW != type) {
+        for (int s = 0; s < DIMS; ++s) shift[0][s] = -part->N[p][s];
+        nread = 1;
+    }
+    for (int s = 0; s < DIMS; ++s) {
+        write->lo[s] = b->box[s][MIN];
+        write->hi[s] = b->box[s][MAX];
+        for (int q = 0; q < nread; ++q) {
+            read[q].lo[s] = b->box[s][MIN] + shift[q][s];
+            read[q].hi[s] = b->box[s][MAX] + shift[q][s];
+        }
+    }
+    return nread;
+}
+
+int artra_bc_groups(const Space *space, const ArtraBcBox *boxes, int nbox, int *group_start, int cap)
+{
+    const Partition *part = &space->part;
+    int ngroup = 0;
+    int first = 0;
+    while (first < nbox) {
+        if (ngroup >= cap) return -1;
+        group_start[ngroup++] = first;
+        int last = first + 1;
+        while (last < nbox) {
+            Region wNew, rNew[2];
+            const int nrNew = BoxRegions(part, boxes + last, &wNew, rNew);
+            int clash = 0;
+            for (int q = first; q < last && !clash; ++q) {
+                Region wOld, rOld[2];
+                const int nrOld = BoxRegions(part, boxes + q, &wOld, rOld);
+                clash = Overlap(&wNew, &wOld);
+                for (int t = 0; t < nrNew && !clash; ++t) clash = Overlap(rNew + t, &wOld);
+                for (int t = 0; t < nrOld && !clash; ++t) clash = Overlap(rOld + t, &wNew);
+            }
+            if (clash) break;
+            ++last;
+        }
+        first = last;
+    }
+    group_start[ngroup] = nbox;
+    return ngroup;
+}
+
+/* --------------------------------------------------- immersed boundary */
+typedef struct {
+    int *node;
+    Real *w;
+    int n;
+    int cap;
+} StencilBuf;
+
+static int Push(StencilBuf *sb, int node, Real w)
+{
+    if (sb->n == sb->cap) {
+        const int cap = sb->cap ? 2 * sb->cap : 4096;
+        int *nn = realloc(sb->node, cap * sizeof(int));
+        Real *nw = realloc(sb->w, cap * sizeof(Real));
+        if (!nn || !nw) {
+            free(nn ? nn : sb->node);
+            free(nw ? nw : sb->w);
+            sb->node = NULL;
+            sb->w = NULL;
+            return artra_set_error(1, "out of host memory for ghost stencils");
+        }
+        sb->node = nn;
+        sb->w = nw;
+        sb->cap = cap;
+    }
+    sb->node[sb->n] = node;
+    sb->w[sb->n] = w;
+    sb->n++;
+    return 0;
+}
+
+static Real InverseDistance(Real tiny, Real d2)
+{
+    if (tiny > d2) d2 = tiny;
+    return 1.0 / d2;
+}
+
+/*
+ * Stencil search around node c for point p: the box of half-width h grows
+ * until it contains at least one node of the target set (did == did and, for
+ * layer >= 1, gst == layer), scanned k-j-i within the interior range.
+ * Appends the entries and returns the running weight sum in *wsum.
+ */
+static int SearchStencil(const Space *space, const int c[DIMS], const Real p[DIMS], int h, int did, int layer,
+                         StencilBuf *sb, int *count, Real *wsum)
+{
+    const Partition *part = &space->part;
+    const Node *node = space->node;
+    const int maxR = part->n[X] + part->n[Y] + part->n[Z];
+    *count = 0;
+    *wsum = 0.0;
+    for (int r = h; 0 == *count; ++r) {
+        if (r > maxR) return artra_set_error(1, "immersed boundary: no stencil node found");
+        for (int kh = -r; kh <= r; ++kh)
+            for (int jh = -r; jh <= r; ++jh)
+                for (int ih = -r; ih <= r; ++ih) {
+                    const int nh[DIMS] = {c[X] + ih, c[Y] + jh, c[Z] + kh};
+                    if (!artra_in_box(nh[Z], nh[Y], nh[X], part->ns[PIN])) continue;
+                    const int q = artra_index(nh[Z], nh[Y], nh[X], part->n[Y], part->n[X]);
+                    if (did != node[q].did) continue;
+                    if (layer > 0 && layer != node[q].gst) continue;
+                    const Real ph[DIMS] = {artra_map_point(nh[X], part->domain[X][MIN], part->d[X], part->ng[X]),
+                                           artra_map_point(nh[Y], part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                                           artra_map_point(nh[Z], part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+                    const Real w = InverseDistance(part->tinyL, artra_dist2(p, ph));
+                    if (Push(sb, q, w)) return 1;
+                    *wsum = *wsum + w;
+                    ++*count;
+                }
+    }
+    return 0;
+}
+
+int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTable *table)
+{
+    const Partition *part = &space->part;
+    const Node *node = space->node;
+    const Geometry *geo = &space->geo;
+    memset(table, 0, sizeof *table);
+    table->layers = part->gl;
+    table->layer_start = calloc(part->gl + 1, sizeof(int));
+    if (NULL == table->layer_start) return artra_set_error(1, "out of host memory");
+    StencilBuf sb = {0};
+    int cap = 0;
+    for (int r = 1; r <= part->gl; ++r) {
+        table->layer_start[r - 1] = table->nghost;
+        for (int n = 0; n < geo->totN; ++n) {
+            const Polyhedron *poly = geo->poly + n;
+            int box[DIMS][LIMIT];
+            for (int s = 0; s < DIMS; ++s) {
+                const int lo = part->ns[PIN][s][MIN], hi = part->ns[PIN][s][MAX];
+                box[s][MIN] = artra_confine(artra_map_node(poly->box[s][MIN], part->domain[s][MIN], part->dd[s],
+                                                           part->ng[s]), lo, hi);
+                box[s][MAX] = artra_confine(artra_map_node(poly->box[s][MAX], part->domain[s][MIN], part->dd[s],
+                                                           part->ng[s]), lo, hi) + 1;
+            }
+            for (int k = box[Z][MIN]; k < box[Z][MAX]; ++k)
+                for (int j = box[Y][MIN]; j < box[Y][MAX]; ++j)
+                    for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
+                        const int idx = artra_index(k, j, i, part->n[Y], part->n[X]);
+                        if (r != node[idx].gst || n + 1 != node[idx].did) continue;
+                        if (table->nghost == cap) {
+                            cap = cap ? 2 * cap : 1024;
+                            ArtraGhost *g = realloc(table->ghosts, cap * sizeof(ArtraGhost));
+                            if (NULL == g) {
+                                free(sb.node);
+                                free(sb.w);
+                                return artra_set_error(1, "out of host memory for ghost table");
+                            }
+                            table->ghosts = g;
+                        }
+                        ArtraGhost *gh = table->ghosts + table->nghost;
+                        memset(gh, 0, sizeof *gh);
+                        gh->node = idx;
+                        gh->first = sb.n;
+                        const Real pG[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                                               artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                                               artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+                        Real wsum = 0.0;
+                        if (model->ibmLayer >= r) {
+                            Real pO[DIMS], pI[DIMS], N[DIMS];
+                            artra_compute_geometric_data(pG, node[idx].fid, poly, pO, pI, N);
+                            const int nI[DIMS] = {
+                                artra_map_node(pI[X], part->domain[X][MIN], part->dd[X], part->ng[X]),
+                                artra_map_node(pI[Y], part->domain[Y][MIN], part->dd[Y], part->ng[Y]),
+                                artra_map_node(pI[Z], part->domain[Z][MIN], part->dd[Z], part->ng[Z])};
+                            if (SearchStencil(space, nI, pI, 2, 0, 0, &sb, &gh->count, &wsum)) return 1;
+                            gh->kind = 0;
+                            gh->wO = InverseDistance(part->tinyL, artra_dist2(pI, pO));
+                            /* body surface velocity V + W x (pO - O) */
+                            const Real arm[DIMS] = {pO[X] - poly->O[X], pO[Y] - poly->O[Y], pO[Z] - poly->O[Z]};
+                            artra_cross(poly->W[TO], arm, gh->Vs);
+                            for (int s = 0; s < DIMS; ++s) gh->Vs[s] = poly->V[TO][s] + gh->Vs[s];
+                            for (int s = 0; s < DIMS; ++s) gh->N[s] = N[s];
+                            artra_orthogonal_space(N, gh->Ta, gh->Tb);
+                            gh->noslip = (0.0 < poly->cf);
+                            gh->isothermal = !(0.0 > poly->T);
+                            gh->wallT = poly->T;
+                        } else {
+                            const int nG[DIMS] = {i, j, k};
+                            if (SearchStencil(space, nG, pG, 1, n + 1, r - 1, &sb, &gh->count, &wsum)) return 1;
+                            gh->kind = 1;
+                        }
+                        table->nghost++;
+                    }
+        }
+    }
+    table->layer_start[part->gl] = table->nghost;
+    table->stencil_node = sb.node;
+    table->stencil_w = sb.w;
+    table->nstencil = sb.n;
+    return 0;
+}
+
+void artra_free_ghost_table(ArtraGhostTable *table)
+{
+    free(table->layer_start);
+    free(table->ghosts);
+    free(table->stencil_node);
+    free(table->stencil_w);
+    memset(table, 0, sizeof *table);
+}
