@@ -85,20 +85,24 @@ __device__ __forceinline__ void LoadNode(const double *U, long N, int idx, doubl
     for (int c = 0; c < 5; ++c) out[c] = U[c * N + idx];
 }
 
-// Solution operator of one node (fluid_dynamics.c:243-274).
-template <bool VISC>
+// Solution operator of one node (fluid_dynamics.c:243-274).  FAST evaluates
+// the same expression with fused multiply-adds.
+template <bool VISC, bool FAST>
 __device__ __forceinline__ void UpdateNode(const StageArgs &sa, long N, int idx, const double FL[5], const double FR[5],
                                            const double FvL[5], const double FvR[5], const double Un[5])
 {
     double phi[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        phi[c] = VISC ? ((FL[c] - FR[c]) + FvR[c]) - FvL[c] : ((FL[c] - FR[c]) + 0.0) - 0.0;
+        if (FAST) phi[c] = VISC ? (FL[c] - FR[c]) + (FvR[c] - FvL[c]) : FL[c] - FR[c];
+        else phi[c] = VISC ? ((FL[c] - FR[c]) + FvR[c]) - FvL[c] : ((FL[c] - FR[c]) + 0.0) - 0.0;
     }
     if (sa.accumulate) {
         const double br = sa.b * sa.r;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) sa.Utm[c * N + idx] = sa.Utm[c * N + idx] + br * phi[c];
+        for (int c = 0; c < 5; ++c) {
+            sa.Utm[c * N + idx] = FAST ? fma(br, phi[c], sa.Utm[c * N + idx]) : sa.Utm[c * N + idx] + br * phi[c];
+        }
         return;
     }
     double Uo[5];
@@ -109,7 +113,25 @@ __device__ __forceinline__ void UpdateNode(const StageArgs &sa, long N, int idx,
         LoadNode(sa.Uto, N, idx, Uo);
     }
 #pragma unroll
-    for (int c = 0; c < 5; ++c) sa.Utm[c * N + idx] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
+    for (int c = 0; c < 5; ++c) {
+        if (FAST) sa.Utm[c * N + idx] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
+        else sa.Utm[c * N + idx] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
+    }
+}
+
+template <int S, int NS, bool FAST>
+__device__ __forceinline__ void ConvFlux(const FluxModel &mdl, const double st[NS][5], double F[5])
+{
+    if (FAST) artra::fast::convective_flux<S, NS>(mdl, st, F);
+    else artra::convective_flux<S, NS>(mdl, st, F);
+}
+
+template <int S, bool FAST>
+__device__ __forceinline__ void ViscFlux(const FluxModel &mdl, const double dd[3], const double P[5], const double E[5],
+                                         const double nb[8][5], double F[5])
+{
+    if (FAST) artra::fast::viscous_flux<S>(mdl, dd, P, E, nb, F);
+    else artra::viscous_flux<S>(mdl, dd, P, E, nb, F);
 }
 
 // Neighbour states of the viscous face stencil at (P, E = P + e_S).
@@ -131,7 +153,7 @@ __device__ __forceinline__ void LoadViscousNeighbours(const Grid &g, const doubl
 }
 
 // ------------------------------------------------------------------ sweeps
-template <int NS, bool VISC>
+template <int NS, bool VISC, bool FAST>
 __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxModel mdl)
 {
     constexpr int SL = (NS == 6) ? -2 : -1;
@@ -155,7 +177,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
 #pragma unroll
         for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, g.N, base + SL + m, st[m]);
         double F[5];
-        artra::convective_flux<0, NS>(mdl, st, F);
+        ConvFlux<0, NS, FAST>(mdl, st, F);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             sF[c][threadIdx.x] = F[c];
@@ -164,7 +186,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
         if (VISC) {
             double nb[8][5], Fv[5];
             LoadViscousNeighbours<0>(g, sa.Utn, base, nb);
-            artra::viscous_flux<0>(mdl, sa.dd, st[K0], st[K0 + 1], nb, Fv);
+            ViscFlux<0, FAST>(mdl, sa.dd, st[K0], st[K0 + 1], nb, Fv);
 #pragma unroll
             for (int c = 0; c < 5; ++c) sFv[VISC ? c : 0][threadIdx.x] = Fv[c];
         }
@@ -183,12 +205,12 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
                 FvL[c] = VISC ? sFv[VISC ? c : 0][threadIdx.x] : 0.0;
                 FvR[c] = VISC ? sFv[VISC ? c : 0][threadIdx.x + 1] : 0.0;
             }
-            UpdateNode<VISC>(sa, g.N, idx, FL, FR, FvL, FvR, Un);
+            UpdateNode<VISC, FAST>(sa, g.N, idx, FL, FR, FvL, FvR, Un);
         }
     }
 }
 
-template <int S, int NS, bool VISC>
+template <int S, int NS, bool VISC, bool FAST>
 __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArgs sa, FluxModel mdl, int segment)
 {
     constexpr int SL = (NS == 6) ? -2 : -1;
@@ -207,11 +229,11 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
 #pragma unroll
     for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, g.N, idx0 + (SL - 1 + m) * stride, st[m]);
     double FL[5], FvL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    artra::convective_flux<S, NS>(mdl, st, FL);
+    ConvFlux<S, NS, FAST>(mdl, st, FL);
     if (VISC) {
         double nb[8][5];
         LoadViscousNeighbours<S>(g, sa.Utn, idx0 - stride, nb);
-        artra::viscous_flux<S>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvL);
+        ViscFlux<S, FAST>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvL);
     }
     int idx = idx0;
     for (int s = s0; s < s1; ++s, idx += stride) {
@@ -222,13 +244,13 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
         }
         LoadNode(sa.Utn, g.N, idx + (SL + NS - 1) * stride, st[NS - 1]);
         double FR[5], FvR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        artra::convective_flux<S, NS>(mdl, st, FR);
+        ConvFlux<S, NS, FAST>(mdl, st, FR);
         if (VISC) {
             double nb[8][5];
             LoadViscousNeighbours<S>(g, sa.Utn, idx, nb);
-            artra::viscous_flux<S>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvR);
+            ViscFlux<S, FAST>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvR);
         }
-        if (sa.fluid[idx]) UpdateNode<VISC>(sa, g.N, idx, FL, FR, FvL, FvR, st[K0]);
+        if (sa.fluid[idx]) UpdateNode<VISC, FAST>(sa, g.N, idx, FL, FR, FvL, FvR, st[K0]);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             FL[c] = FR[c];
@@ -472,6 +494,12 @@ __global__ __launch_bounds__(256) void cfl_max(const double *U, const uint8_t *f
     }
 }
 
+// Plain streaming copy: the reference point for HBM counters and bandwidth.
+__global__ __launch_bounds__(256) void stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
+}
+
 __global__ void fill_u64(unsigned long long *p, int n, unsigned long long v)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -496,6 +524,7 @@ struct ArtraDevice {
     std::vector<int> layerStart;
     FluxModel mdl{};
     int segment = 32;
+    int exact = 0;                  // 1: reference operation order (bit-faithful), 0: fast arithmetic
     // timing
     bool timing = false;
     double ms[ARTRA_T_COUNT] = {0.0, 0.0, 0.0, 0.0};
@@ -553,7 +582,7 @@ int Harvest(ArtraDevice *dev)
     return 0;
 }
 
-template <int NS, bool VISC>
+template <int NS, bool VISC, bool FAST>
 void LaunchSweep(ArtraDevice *dev, int s, const StageArgs &sa)
 {
     const Grid &g = dev->g;
@@ -561,7 +590,7 @@ void LaunchSweep(ArtraDevice *dev, int s, const StageArgs &sa)
         const long faces = (long)(g.hi[0] - g.lo[0] + 1) * (g.hi[1] - g.lo[1]) * (g.hi[2] - g.lo[2]);
         const long blocks = (faces - 1 + (kXBlock - 2)) / (kXBlock - 1);
         if (blocks <= 0) return;
-        hipLaunchKernelGGL((sweep_x<NS, VISC>), dim3((unsigned)blocks), dim3(kXBlock), 0, dev->stream, g, sa,
+        hipLaunchKernelGGL((sweep_x<NS, VISC, FAST>), dim3((unsigned)blocks), dim3(kXBlock), 0, dev->stream, g, sa,
                            dev->mdl);
         return;
     }
@@ -572,9 +601,9 @@ void LaunchSweep(ArtraDevice *dev, int s, const StageArgs &sa)
     dim3 grid((ex + kMarchX - 1) / kMarchX, (et + kMarchT - 1) / kMarchT, (es + seg - 1) / seg);
     dim3 block(kMarchX, kMarchT);
     if (s == 1) {
-        hipLaunchKernelGGL((sweep_march<1, NS, VISC>), grid, block, 0, dev->stream, g, sa, dev->mdl, seg);
+        hipLaunchKernelGGL((sweep_march<1, NS, VISC, FAST>), grid, block, 0, dev->stream, g, sa, dev->mdl, seg);
     } else {
-        hipLaunchKernelGGL((sweep_march<2, NS, VISC>), grid, block, 0, dev->stream, g, sa, dev->mdl, seg);
+        hipLaunchKernelGGL((sweep_march<2, NS, VISC, FAST>), grid, block, 0, dev->stream, g, sa, dev->mdl, seg);
     }
 }
 
@@ -596,10 +625,19 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
         int slot;
         if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
         const bool visc = dev->mdl.refMu > 0.0;
+        const bool fast = !dev->exact;
         if (dev->par.sScheme == 1) {
-            if (visc) LaunchSweep<6, true>(dev, s, sa); else LaunchSweep<6, false>(dev, s, sa);
+            if (visc) {
+                if (fast) LaunchSweep<6, true, true>(dev, s, sa); else LaunchSweep<6, true, false>(dev, s, sa);
+            } else {
+                if (fast) LaunchSweep<6, false, true>(dev, s, sa); else LaunchSweep<6, false, false>(dev, s, sa);
+            }
         } else {
-            if (visc) LaunchSweep<4, true>(dev, s, sa); else LaunchSweep<4, false>(dev, s, sa);
+            if (visc) {
+                if (fast) LaunchSweep<4, true, true>(dev, s, sa); else LaunchSweep<4, true, false>(dev, s, sa);
+            } else {
+                if (fast) LaunchSweep<4, false, true>(dev, s, sa); else LaunchSweep<4, false, false>(dev, s, sa);
+            }
         }
         HIP_TRY(hipGetLastError());
         if (End(dev, slot)) return 1;
@@ -890,6 +928,40 @@ int artra_dev_get_timing(ArtraDevice *dev, double ms[ARTRA_T_COUNT], long launch
         ms[q] = dev->ms[q];
         launches[q] = dev->launches[q];
     }
+    return 0;
+}
+
+int artra_dev_copy_probe(long bytes, int iters, double *gbps)
+{
+    const long n = bytes / (long)sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipMalloc(&a, n * sizeof(double2)));
+    HIP_TRY(hipMalloc(&b, n * sizeof(double2)));
+    HIP_TRY(hipMemset(a, 0, n * sizeof(double2)));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+    const dim3 grid(cus * 8), block(256);
+    hipLaunchKernelGGL(stream_copy, grid, block, 0, 0, a, b, n);
+    HIP_TRY(hipEventRecord(e0, 0));
+    for (int q = 0; q < iters; ++q) hipLaunchKernelGGL(stream_copy, grid, block, 0, 0, a, b, n);
+    HIP_TRY(hipEventRecord(e1, 0));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = 2.0 * n * sizeof(double2) * iters / (ms * 1e-3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    return 0;
+}
+
+int artra_dev_set_exact(ArtraDevice *dev, int exact)
+{
+    dev->exact = (exact != 0);
     return 0;
 }
 

@@ -61,8 +61,13 @@ int artra_dev_sync(ArtraDevice *dev);
  * solver's stream; totals are in milliseconds. */
 int artra_dev_set_timing(ArtraDevice *dev, int on);
 int artra_dev_get_timing(ArtraDevice *dev, double ms[ARTRA_T_COUNT], long launches[ARTRA_T_COUNT]);
+/* arithmetic of the flux kernels: 1 keeps the reference's operation order
+ * (bit-faithful), 0 (default) uses the fused/reciprocal form */
+int artra_dev_set_exact(ArtraDevice *dev, int exact);
 /* tuning knobs (0 keeps the default) */
 int artra_dev_set_tuning(ArtraDevice *dev, int march_segment, int xblock);
+/* streaming copy of `bytes` bytes, iters times; returns read+write GB/s */
+int artra_dev_copy_probe(long bytes, int iters, double *gbps);
 const char *artra_dev_error(void);
 
 #ifdef __cplusplus

@@ -24,6 +24,7 @@ struct ArtraHipSolver {
     ArtraGhostTable ghosts;
     long N;
     double *staging;          /* [5][N] host buffer for AoS <-> SoA */
+    Real rcInt;               /* time since the last output instant (solve.c:64) */
 };
 
 struct ArtraCase {
@@ -152,6 +153,8 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         artra_hip_detach(s);
         return 1;
     }
+    const char *exact = getenv("ARTRA_EXACT");
+    if (exact && exact[0] == '1') artra_dev_set_exact(s->dev, 1);
     *out = s;
     return 0;
 }
@@ -202,7 +205,7 @@ int artra_hip_evolve(ArtraHipSolver *s, Real dt)
 int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_done, Real *dts)
 {
     const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
-    Real rcInt = 0.0;
+    Real rcInt = s->rcInt;
     int done = 0;
     while (time->now < time->end && time->stepC < time->stepN && done < max_steps) {
         ++time->stepC;
@@ -223,6 +226,7 @@ int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_don
         if (dts) dts[done] = dt;
         ++done;
     }
+    s->rcInt = rcInt;
     if (steps_done) *steps_done = done;
     return artra_hip_synchronize(s);
 }
@@ -268,6 +272,16 @@ int artra_hip_get_timing(ArtraHipSolver *s, double ms[4], long launches[4])
 int artra_hip_set_tuning(ArtraHipSolver *s, int march_segment)
 {
     return artra_dev_set_tuning(s->dev, march_segment, 0) ? DevError() : 0;
+}
+
+int artra_hip_set_arithmetic(ArtraHipSolver *s, int exact)
+{
+    return artra_dev_set_exact(s->dev, exact) ? DevError() : 0;
+}
+
+int artra_hip_copy_probe(long bytes, int iters, double *gbps)
+{
+    return artra_dev_copy_probe(bytes, iters, gbps) ? DevError() : 0;
 }
 
 /* ------------------------------------------------------------- drop-in */

@@ -54,7 +54,9 @@ def library():
         "artra_hip_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_get_timing": (ctypes.c_int, [vp, dp, lp]),
         "artra_hip_set_tuning": (ctypes.c_int, [vp, ctypes.c_int]),
+        "artra_hip_set_arithmetic": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_error": (ctypes.c_char_p, []),
+        "artra_hip_copy_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, dp]),
         "artra_case_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
         "artra_case_close": (None, [vp]),
         "artra_case_space": (vp, [vp]),
@@ -92,6 +94,13 @@ def calc(expr, x=0.0, y=0.0, z=0.0):
     out = ctypes.c_double(0.0)
     rc = library().artra_calc_expression(expr.encode(), x, y, z, ctypes.byref(out))
     return None if rc else out.value
+
+
+def copy_probe(nbytes=1 << 30, iters=20):
+    """Read+write GB/s of a plain device copy of nbytes bytes."""
+    out = ctypes.c_double()
+    _check(library().artra_hip_copy_probe(nbytes, iters, ctypes.byref(out)))
+    return out.value
 
 
 class Case:
@@ -173,12 +182,14 @@ class Solver:
     ``initial_boundary`` applies TreatBoundary(TO) after the upload, which is
     the last step of the solver's InitializeComputeDomain."""
 
-    def __init__(self, case, initial_boundary=True, segment=0):
+    def __init__(self, case, initial_boundary=True, segment=0, exact=None):
         self.case = case
         lib = library()
         h = ctypes.c_void_p()
         _check(lib.artra_hip_attach(case.space, case.model, ctypes.byref(h)))
         self._h = h
+        if exact is not None:
+            _check(lib.artra_hip_set_arithmetic(h, 1 if exact else 0))
         if segment:
             _check(lib.artra_hip_set_tuning(h, segment))
         if initial_boundary:

@@ -81,6 +81,8 @@ def main():
     cells = fluid_nodes(case)
     solver = Solver(case, segment=args.segment)
     init_s = time.perf_counter() - t_init
+    from artracfd_amd.lib import copy_probe
+    copy_gbs = copy_probe(1 << 30, 20)
     solver.march(args.warmup)
     solver.synchronize()
     solver.set_timing(True)
@@ -113,7 +115,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "sweep_x/sweep_march (directional WENO5 sweeps)",
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": sweep_n,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "measured_copy_gbs": round(copy_gbs, 1)},
         "kernel_ms": {k: round(v[0], 3) for k, v in tm.items()},
         "kernel_launches": {k: v[1] for k, v in tm.items()},
         "init_seconds": round(init_s, 2),

@@ -86,8 +86,22 @@ int artra_hip_synchronize(ArtraHipSolver *solver);
 int artra_hip_set_timing(ArtraHipSolver *solver, int on);
 int artra_hip_get_timing(ArtraHipSolver *solver, double ms[4], long launches[4]);
 
+/* Flux arithmetic.  exact = 1 evaluates every expression in the reference's
+ * operation order without contraction (results bit-identical to the CPU
+ * for inviscid cases); exact = 0, the default, uses fused multiply-adds,
+ * shared reciprocals, the structured characteristic projection and
+ * single-division WENO weights (rounding-level differences, far inside the
+ * 1e-10 relative-L2 parity contract).  The environment variable
+ * ARTRA_EXACT=1 selects the exact form at attach time. */
+int artra_hip_set_arithmetic(ArtraHipSolver *solver, int exact);
+
 /* Launch-shape knobs: column segment of the y/z sweeps (0 keeps the default). */
 int artra_hip_set_tuning(ArtraHipSolver *solver, int march_segment);
+
+/* Device copy bandwidth probe: copies `bytes` bytes `iters` times with a
+ * plain streaming kernel and reports read+write GB/s.  Used to calibrate
+ * HBM byte counters and to state the achievable copy bandwidth. */
+int artra_hip_copy_probe(long bytes, int iters, double *gbps);
 
 /* Message of the last failed call on this thread. */
 const char *artra_hip_error(void);
