@@ -384,6 +384,30 @@ __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
     StoreNode(ba.U, g.N, idx, out);
 }
 
+// Resolved boundary fill: each entry copies one interior node or writes the
+// inflow state of a face (boundary_treatment.c:66-185 replayed on the host).
+struct GatherArgs {
+    double *U;
+    long N;
+    double gamma;
+    double given[7][6];
+};
+
+__global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__restrict__ dst,
+                                                 const int *__restrict__ src, long count)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int d = dst[t], q = src[t];
+    double out[5];
+    if (q >= 0) {
+        LoadNode(ga.U, ga.N, q, out);
+    } else {
+        Conservative(ga.gamma, ga.given[-3 - q], out);
+    }
+    StoreNode(ga.U, ga.N, d, out);
+}
+
 // ------------------------------------------------------- immersed boundary
 struct GhostDev {
     int node, kind, first, count;
@@ -516,8 +540,13 @@ struct ArtraDevice {
     uint8_t *fluid = nullptr;
     unsigned long long *vmax = nullptr;
     hipStream_t stream = nullptr;
-    std::vector<BcGroup> groups;
+    std::vector<BcGroup> groups[4];          // mode 0: full fill; 1 + s: faces of axis s only
     BcArgs bc{};
+    int gatherResolved = 0;
+    int useGather = 1;
+    int *gDst[4] = {nullptr, nullptr, nullptr, nullptr};
+    int *gSrc[4] = {nullptr, nullptr, nullptr, nullptr};
+    long gCount[4] = {0, 0, 0, 0};
     GhostDev *ghosts = nullptr;
     int *snode = nullptr;
     double *sw = nullptr;
@@ -525,6 +554,7 @@ struct ArtraDevice {
     FluxModel mdl{};
     int segment = 32;
     int exact = 0;                  // 1: reference operation order (bit-faithful), 0: fast arithmetic
+    int partialFill = 1;            // axis-only fills between the stages of inviscid split sweeps
     // timing
     bool timing = false;
     double ms[ARTRA_T_COUNT] = {0.0, 0.0, 0.0, 0.0};
@@ -645,7 +675,9 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
     return 0;
 }
 
-int TreatBoundary(ArtraDevice *dev, int level)
+// mode 0: the whole external boundary; 1 + s: only the ghosts a sweep along
+// axis s reads (valid between the stages of an inviscid split sweep).
+int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
 {
     double *U = Level(dev, level);
     // immersed boundary first, layer by layer
@@ -660,9 +692,25 @@ int TreatBoundary(ArtraDevice *dev, int level)
         HIP_TRY(hipGetLastError());
         if (End(dev, slot)) return 1;
     }
+    if (dev->gatherResolved && dev->useGather) {
+        const long count = dev->gCount[mode];
+        if (count <= 0) return 0;
+        GatherArgs ga;
+        ga.U = U;
+        ga.N = dev->g.N;
+        ga.gamma = dev->par.gamma;
+        for (int p = 0; p < 7; ++p)
+            for (int q = 0; q < 6; ++q) ga.given[p][q] = dev->par.varBC[p][q];
+        int slot;
+        if (Begin(dev, ARTRA_T_BC, &slot)) return 1;
+        hipLaunchKernelGGL(bc_gather, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, dev->stream, ga,
+                           dev->gDst[mode], dev->gSrc[mode], count);
+        HIP_TRY(hipGetLastError());
+        return End(dev, slot);
+    }
     BcArgs ba = dev->bc;
     ba.U = U;
-    for (const BcGroup &grp : dev->groups) {
+    for (const BcGroup &grp : dev->groups[mode]) {
         if (grp.total <= 0) continue;
         int slot;
         if (Begin(dev, ARTRA_T_BC, &slot)) return 1;
@@ -674,15 +722,19 @@ int TreatBoundary(ArtraDevice *dev, int level)
 }
 
 // Runge-Kutta stages of one directional operator (fluid_dynamics.c:121-148).
+// Between the stages of an inviscid sweep along one axis only that axis's
+// ghosts are read, so only they are filled; the last stage fills everything,
+// exactly as TreatBoundary does after every stage.
 int Stages(ArtraDevice *dev, double dt, int p)
 {
+    const int mid = (p < 3 && !(dev->mdl.refMu > 0.0) && dev->partialFill) ? 1 + p : 0;
     if (dev->par.tScheme == 0) {
-        if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1)) return 1;
+        if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1, mid)) return 1;
         if (Sweep(dev, dt, 1.0 / 2.0, 1.0 / 2.0, 0, 1, 0, p) || TreatBoundary(dev, 0)) return 1;
         return 0;
     }
-    if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1)) return 1;
-    if (Sweep(dev, dt, 3.0 / 4.0, 1.0 / 4.0, 0, 1, 2, p) || TreatBoundary(dev, 2)) return 1;
+    if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1, mid)) return 1;
+    if (Sweep(dev, dt, 3.0 / 4.0, 1.0 / 4.0, 0, 1, 2, p) || TreatBoundary(dev, 2, mid)) return 1;
     if (Sweep(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0, p) || TreatBoundary(dev, 0)) return 1;
     return 0;
 }
@@ -754,20 +806,27 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->ghosts);
     hipFree(dev->snode);
     hipFree(dev->sw);
+    for (int m = 0; m < 4; ++m) {
+        hipFree(dev->gDst[m]);
+        hipFree(dev->gSrc[m]);
+    }
     if (dev->stream) hipStreamDestroy(dev->stream);
     delete dev;
 }
 
-int artra_dev_set_boundary(ArtraDevice *dev, const ArtraBcBox *boxes, int nbox, const int *group_start, int ngroup)
+int artra_dev_set_boundary(ArtraDevice *dev, int mode, const ArtraBcBox *boxes, int nbox, const int *group_start,
+                           int ngroup)
 {
-    dev->groups.clear();
+    if (mode < 0 || mode > 3) return Fail("bad boundary mode %d", mode);
+    std::vector<BcGroup> &groups = dev->groups[mode];
+    groups.clear();
     for (int q = 0; q < ngroup; ++q) {
         BcGroup grp{};
         long offset = 0;
         for (int b = group_start[q]; b < group_start[q + 1]; ++b) {
             if (grp.count == kMaxGroup) {
                 grp.total = offset;
-                dev->groups.push_back(grp);
+                groups.push_back(grp);
                 grp = BcGroup{};
                 offset = 0;
             }
@@ -784,9 +843,41 @@ int artra_dev_set_boundary(ArtraDevice *dev, const ArtraBcBox *boxes, int nbox, 
             offset += vol;
         }
         grp.total = offset;
-        if (grp.count) dev->groups.push_back(grp);
+        if (grp.count) groups.push_back(grp);
     }
     (void)nbox;
+    return 0;
+}
+
+int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
+{
+    for (int m = 0; m < 4; ++m) {
+        hipFree(dev->gDst[m]);
+        hipFree(dev->gSrc[m]);
+        dev->gDst[m] = dev->gSrc[m] = nullptr;
+        dev->gCount[m] = 0;
+    }
+    dev->gatherResolved = 0;
+    if (!gt->resolved) return 0;
+    for (int m = 0; m < 4; ++m) {
+        const long n = (m == 0) ? gt->count : gt->axis_count[m - 1];
+        const int *d = (m == 0) ? gt->dst : gt->axis_dst[m - 1];
+        const int *q = (m == 0) ? gt->src : gt->axis_src[m - 1];
+        dev->gCount[m] = n;
+        if (n == 0) continue;
+        HIP_TRY(hipMalloc(&dev->gDst[m], n * sizeof(int)));
+        HIP_TRY(hipMalloc(&dev->gSrc[m], n * sizeof(int)));
+        HIP_TRY(hipMemcpy(dev->gDst[m], d, n * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dev->gSrc[m], q, n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    dev->gatherResolved = 1;
+    return 0;
+}
+
+int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial)
+{
+    dev->useGather = (use_gather != 0);
+    dev->partialFill = (partial != 0);
     return 0;
 }
 

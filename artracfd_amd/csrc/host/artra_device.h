@@ -42,8 +42,15 @@ enum { ARTRA_T_SWEEP = 0, ARTRA_T_BC = 1, ARTRA_T_IB = 2, ARTRA_T_CFL = 3, ARTRA
 
 int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevice **out);
 void artra_dev_destroy(ArtraDevice *dev);
-int artra_dev_set_boundary(ArtraDevice *dev, const ArtraBcBox *boxes, int nbox, const int *group_start,
+/* box groups of the external boundary: mode 0 the full fill, 1 + s the
+ * faces of axis s only */
+int artra_dev_set_boundary(ArtraDevice *dev, int mode, const ArtraBcBox *boxes, int nbox, const int *group_start,
                            int ngroup);
+/* resolved fill tables (used instead of the box groups when resolved) */
+int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gather);
+/* use_gather = 0 forces the box groups; partial = 0 fills everything after
+ * every stage */
+int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial);
 int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *table);
 /* copy one time level: soa is [5][N] doubles */
 int artra_dev_upload(ArtraDevice *dev, int level, const double *soa);

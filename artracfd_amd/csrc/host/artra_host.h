@@ -102,6 +102,27 @@ int artra_bc_boxes(const Space *space, ArtraBcBox *out, int cap);
 int artra_bc_groups(const Space *space, const ArtraBcBox *boxes, int nbox, int *group_start, int cap);
 
 /*
+ * Resolved form of the external-boundary fill.  Replaying the box order on
+ * node labels shows, for outflow/periodic/inflow boundaries, that every
+ * node a fill writes ends up as a copy of one interior node or as the inflow
+ * state, independently of the order; the whole fill is then one gather.
+ * Wall reflections (or an order-dependent result) leave resolved = 0 and the
+ * grouped box kernels are used instead.
+ */
+typedef struct {
+    int resolved;
+    long count;
+    int *dst;            /* padded-grid index written */
+    int *src;            /* >= 0: interior source node; -3 - p: inflow state of face p */
+    long axis_count[DIMS];
+    int *axis_dst[DIMS]; /* the subset outside the interior range along one axis only */
+    int *axis_src[DIMS];
+} ArtraBcGather;
+
+int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, ArtraBcGather *out);
+void artra_free_bc_gather(ArtraBcGather *g);
+
+/*
  * Immersed-boundary ghost table.  For a stationary body the stencil of every
  * ghost node is fixed, so the host precomputes, per ghost node and in the
  * reference's exact summation order, the stencil node indices and weights of

@@ -78,6 +78,31 @@ static int Supported(const Space *space, const Model *model)
     return 0;
 }
 
+/* Box groups for the full fill and for each axis's faces, plus the resolved
+ * gather when the boundary types allow it. */
+static int SetBoundaryTables(ArtraHipSolver *s)
+{
+    ArtraBcBox boxes[64], axisBoxes[64];
+    int starts[65];
+    const int nbox = artra_bc_boxes(s->space, boxes, 64);
+    if (nbox < 0) return artra_set_error(1, "boundary box table overflow");
+    for (int mode = 0; mode < 4; ++mode) {
+        int n = 0;
+        for (int b = 0; b < nbox; ++b) {
+            const int axis = (boxes[b].face - PWB) / 2;
+            if (0 == mode || axis == mode - 1) axisBoxes[n++] = boxes[b];
+        }
+        const int ngroup = artra_bc_groups(s->space, axisBoxes, n, starts, 64);
+        if (ngroup < 0) return artra_set_error(1, "boundary group table overflow");
+        if (artra_dev_set_boundary(s->dev, mode, axisBoxes, n, starts, ngroup)) return DevError();
+    }
+    ArtraBcGather gather;
+    if (artra_bc_resolve(s->space, boxes, nbox, &gather)) return 1;
+    const int rc = artra_dev_set_gather(s->dev, &gather);
+    artra_free_bc_gather(&gather);
+    return rc ? DevError() : 0;
+}
+
 int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
 {
     *out = NULL;
@@ -129,17 +154,9 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         artra_hip_detach(s);
         return DevError();
     }
-    ArtraBcBox boxes[64];
-    int starts[65];
-    const int nbox = artra_bc_boxes(space, boxes, 64);
-    const int ngroup = (nbox >= 0) ? artra_bc_groups(space, boxes, nbox, starts, 64) : -1;
-    if (nbox < 0 || ngroup < 0) {
+    if (SetBoundaryTables(s)) {
         artra_hip_detach(s);
-        return artra_set_error(1, "boundary box table overflow");
-    }
-    if (artra_dev_set_boundary(s->dev, boxes, nbox, starts, ngroup)) {
-        artra_hip_detach(s);
-        return DevError();
+        return 1;
     }
     if (artra_build_ghost_table(space, model, &s->ghosts)) {
         artra_hip_detach(s);
@@ -272,6 +289,11 @@ int artra_hip_get_timing(ArtraHipSolver *s, double ms[4], long launches[4])
 int artra_hip_set_tuning(ArtraHipSolver *s, int march_segment)
 {
     return artra_dev_set_tuning(s->dev, march_segment, 0) ? DevError() : 0;
+}
+
+int artra_hip_set_fill_options(ArtraHipSolver *s, int use_gather, int partial)
+{
+    return artra_dev_set_fill_options(s->dev, use_gather, partial) ? DevError() : 0;
 }
 
 int artra_hip_set_arithmetic(ArtraHipSolver *s, int exact)

@@ -290,3 +290,148 @@ void artra_free_ghost_table(ArtraGhostTable *table)
     free(table->stencil_w);
     memset(table, 0, sizeof *table);
 }
+
+/* ------------------------------------------------ resolved boundary fill */
+/*
+ * Labels of the symbolic replay: a node label is
+ *   >= 0           "copy of interior node <label>" (an ns[PIN] node, which
+ *                  the boundary treatment never writes),
+ *   LABEL_CONST-p  "inflow state of face p",
+ *   LABEL_STALE    "value left from before this fill",
+ *   LABEL_MIXED    "function of several nodes" (wall reflections).
+ */
+enum { LABEL_STALE = -1, LABEL_MIXED = -2, LABEL_CONST = -3 };
+
+static int LabelOf(const int *label, const Partition *part, int k, int j, int i)
+{
+    if (artra_in_box(k, j, i, part->ns[PIN])) return artra_index(k, j, i, part->n[Y], part->n[X]);
+    return label[artra_index(k, j, i, part->n[Y], part->n[X])];
+}
+
+int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, ArtraBcGather *out)
+{
+    const Partition *part = &space->part;
+    const long N = (long)part->n[X] * part->n[Y] * part->n[Z];
+    memset(out, 0, sizeof *out);
+    int *label = malloc(N * sizeof(int));
+    if (NULL == label) return artra_set_error(1, "out of host memory");
+    for (long q = 0; q < N; ++q) label[q] = LABEL_STALE;
+    for (int b = 0; b < nbox; ++b) {
+        const ArtraBcBox *bx = boxes + b;
+        const int p = bx->face, r = bx->layer, type = part->typeBC[p];
+        const int *Nn = part->N[p];
+        for (int k = bx->box[Z][MIN]; k < bx->box[Z][MAX]; ++k)
+            for (int j = bx->box[Y][MIN]; j < bx->box[Y][MAX]; ++j)
+                for (int i = bx->box[X][MIN]; i < bx->box[X][MAX]; ++i) {
+                    int lab;
+                    if (r > 0) {
+                        if (SLIPWALL == type || NOSLIPWALL == type) lab = LABEL_MIXED;
+                        else if (PERIODIC == type)
+                            lab = LabelOf(label, part, k - part->m[Z] * Nn[Z], j - part->m[Y] * Nn[Y],
+                                          i - part->m[X] * Nn[X]);
+                        else lab = LabelOf(label, part, k - Nn[Z], j - Nn[Y], i - Nn[X]);
+                    } else if (INFLOW == type) {
+                        lab = LABEL_CONST - p;
+                    } else if (OUTFLOW == type) {
+                        lab = LabelOf(label, part, k - Nn[Z], j - Nn[Y], i - Nn[X]);
+                    } else {
+                        lab = LABEL_MIXED;
+                    }
+                    label[artra_index(k, j, i, part->n[Y], part->n[X])] = lab;
+                }
+    }
+    /* count entries; any written node that does not resolve disables the gather */
+    long count = 0;
+    int ok = 1;
+    for (long q = 0; q < N; ++q) {
+        const int lab = label[q];
+        if (LABEL_MIXED == lab) ok = 0;
+        if (lab >= 0 || lab <= LABEL_CONST) ++count;
+    }
+    /* a stale label only matters on a node that some box wrote */
+    if (ok) {
+        for (int b = 0; b < nbox && ok; ++b) {
+            const ArtraBcBox *bx = boxes + b;
+            for (int k = bx->box[Z][MIN]; k < bx->box[Z][MAX] && ok; ++k)
+                for (int j = bx->box[Y][MIN]; j < bx->box[Y][MAX] && ok; ++j)
+                    for (int i = bx->box[X][MIN]; i < bx->box[X][MAX]; ++i) {
+                        if (LABEL_STALE == label[artra_index(k, j, i, part->n[Y], part->n[X])]) {
+                            ok = 0;
+                            break;
+                        }
+                    }
+        }
+    }
+    if (!ok) {
+        free(label);
+        return 0;
+    }
+    out->dst = malloc(count * sizeof(int));
+    out->src = malloc(count * sizeof(int));
+    if (!out->dst || !out->src) {
+        free(label);
+        artra_free_bc_gather(out);
+        return artra_set_error(1, "out of host memory");
+    }
+    /* full table, then per-axis subsets: nodes outside the interior range
+     * along that axis only (the ghosts a sweep along it reads) */
+    long n = 0;
+    for (long q = 0; q < N; ++q) {
+        if (label[q] >= 0 || label[q] <= LABEL_CONST) {
+            out->dst[n] = (int)q;
+            out->src[n] = label[q];
+            ++n;
+        }
+    }
+    out->count = n;
+    for (int s = 0; s < DIMS; ++s) {
+        long m = 0;
+        for (long e = 0; e < n; ++e) {
+            const int q = out->dst[e];
+            const int c[DIMS] = {q % part->n[X], (q / part->n[X]) % part->n[Y], q / (part->n[X] * part->n[Y])};
+            int inside = 1;
+            for (int t = 0; t < DIMS; ++t) {
+                if (t == s) continue;
+                if (c[t] < part->ns[PIN][t][MIN] || c[t] >= part->ns[PIN][t][MAX]) inside = 0;
+            }
+            if (inside) ++m;
+        }
+        out->axis_dst[s] = malloc((m ? m : 1) * sizeof(int));
+        out->axis_src[s] = malloc((m ? m : 1) * sizeof(int));
+        if (!out->axis_dst[s] || !out->axis_src[s]) {
+            free(label);
+            artra_free_bc_gather(out);
+            return artra_set_error(1, "out of host memory");
+        }
+        long w = 0;
+        for (long e = 0; e < n; ++e) {
+            const int q = out->dst[e];
+            const int c[DIMS] = {q % part->n[X], (q / part->n[X]) % part->n[Y], q / (part->n[X] * part->n[Y])};
+            int inside = 1;
+            for (int t = 0; t < DIMS; ++t) {
+                if (t == s) continue;
+                if (c[t] < part->ns[PIN][t][MIN] || c[t] >= part->ns[PIN][t][MAX]) inside = 0;
+            }
+            if (inside) {
+                out->axis_dst[s][w] = q;
+                out->axis_src[s][w] = out->src[e];
+                ++w;
+            }
+        }
+        out->axis_count[s] = m;
+    }
+    out->resolved = 1;
+    free(label);
+    return 0;
+}
+
+void artra_free_bc_gather(ArtraBcGather *g)
+{
+    free(g->dst);
+    free(g->src);
+    for (int s = 0; s < DIMS; ++s) {
+        free(g->axis_dst[s]);
+        free(g->axis_src[s]);
+    }
+    memset(g, 0, sizeof *g);
+}

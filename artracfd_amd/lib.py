@@ -55,6 +55,7 @@ def library():
         "artra_hip_get_timing": (ctypes.c_int, [vp, dp, lp]),
         "artra_hip_set_tuning": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_arithmetic": (ctypes.c_int, [vp, ctypes.c_int]),
+        "artra_hip_set_fill_options": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "artra_hip_error": (ctypes.c_char_p, []),
         "artra_hip_copy_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, dp]),
         "artra_case_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
@@ -182,7 +183,7 @@ class Solver:
     ``initial_boundary`` applies TreatBoundary(TO) after the upload, which is
     the last step of the solver's InitializeComputeDomain."""
 
-    def __init__(self, case, initial_boundary=True, segment=0, exact=None):
+    def __init__(self, case, initial_boundary=True, segment=0, exact=None, gather=True, partial=True):
         self.case = case
         lib = library()
         h = ctypes.c_void_p()
@@ -190,6 +191,8 @@ class Solver:
         self._h = h
         if exact is not None:
             _check(lib.artra_hip_set_arithmetic(h, 1 if exact else 0))
+        if not (gather and partial):
+            _check(lib.artra_hip_set_fill_options(h, 1 if gather else 0, 1 if partial else 0))
         if segment:
             _check(lib.artra_hip_set_tuning(h, segment))
         if initial_boundary:

@@ -95,6 +95,15 @@ int artra_hip_get_timing(ArtraHipSolver *solver, double ms[4], long launches[4])
  * ARTRA_EXACT=1 selects the exact form at attach time. */
 int artra_hip_set_arithmetic(ArtraHipSolver *solver, int exact);
 
+/* Boundary-fill strategy.  use_gather = 1 (default) replaces the ordered box
+ * sequence by one gather wherever replaying the order on the host proves
+ * every ghost a copy of an interior node or an inflow state (outflow,
+ * periodic and inflow faces; walls keep the ordered boxes).  partial = 1
+ * (default) fills only the sweep axis's ghosts between the stages of an
+ * inviscid split sweep, the only ghosts those stages read.  Both leave the
+ * result unchanged; 0, 0 restores the literal fill after every stage. */
+int artra_hip_set_fill_options(ArtraHipSolver *solver, int use_gather, int partial);
+
 /* Launch-shape knobs: column segment of the y/z sweeps (0 keeps the default). */
 int artra_hip_set_tuning(ArtraHipSolver *solver, int march_segment);
 
