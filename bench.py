@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# Measured once with rocprofv3 on this workload (profiles/sweep_shockvortex256.md):
+# HBM bytes per cell-stage of the sweep kernels (FETCH_SIZE x 2.0 + WRITE_SIZE,
+# scale factors from a 1 GiB copy kernel) and FP64 flops per face (FMA = 2).
+TRAFFIC_B_PER_CELL_STAGE = 129.6
+FP64_FLOP_PER_CELL_STAGE = 1124.0
 METRIC = "Mcell-stage updates/s (256³ WENO5-RK3) + learned-hybrid vs plain at ≤1e-3 L1"
 
 
@@ -112,11 +117,16 @@ def main():
         "config": {"workload": f"{args.case}: shock-vortex interaction, {cells} interior fluid nodes, WENO5-LLF-RK3 "
                                f"split, inviscid, plain FP64 path", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(TRAFFIC_B_PER_CELL_STAGE * cells) if args.case == "shockvortex256" else None,
                      "kernel": "sweep_x/sweep_march (directional WENO5 sweeps)",
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": sweep_n,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "measured_copy_gbs": round(copy_gbs, 1)},
+                     "measured_copy_gbs": round(copy_gbs, 1),
+                     "fp64": {"achieved_tflops": round(FP64_FLOP_PER_CELL_STAGE * cells / avg_launch_s / 1e12, 2),
+                              "peak_tflops": FP64_PEAK_TFLOPS,
+                              "frac": round(FP64_FLOP_PER_CELL_STAGE * cells / avg_launch_s / 1e12 / FP64_PEAK_TFLOPS, 4),
+                              "flop_per_cell_stage": FP64_FLOP_PER_CELL_STAGE}},
         "kernel_ms": {k: round(v[0], 3) for k, v in tm.items()},
         "kernel_launches": {k: v[1] for k, v in tm.items()},
         "init_seconds": round(init_s, 2),
