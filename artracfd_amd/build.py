@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libartra_hip.so")
 ARCH = os.environ.get("ARTRA_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["case_io.c", "calculator.c", "setup.c", "geometry.c", "tables.c", "capi.c"]
-DEVICE_SOURCES = ["artra_device.hip"]
+DEVICE_SOURCES = ["artra_device.hip", "indicator.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",
               "-Wall", "-Wextra", "-Wno-unused-parameter"]

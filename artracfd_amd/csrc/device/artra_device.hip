@@ -20,12 +20,14 @@
 // compiled with -ffp-contract=off so results match the CPU bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <vector>
 
 #include "flux.hpp"
+#include "indicator.hpp"
 
 extern "C" {
 #include "artra_device.h"
@@ -259,6 +261,41 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
     }
 }
 
+
+// Features and labels of every face along S of level TO (training data).
+// Face q of a line is the face between nodes lo-1+q and lo+q along S.
+template <int S>
+__global__ __launch_bounds__(256) void sample_faces(Grid g, const double *U, FluxModel mdl, float *feat, float *label)
+{
+    constexpr int A = (S == 0) ? 1 : 0;
+    constexpr int B = (S == 2) ? 1 : 2;
+    const int faces = g.hi[S] - g.lo[S] + 1;
+    const int na = g.hi[A] - g.lo[A];
+    const long total = (long)faces * na * (g.hi[B] - g.lo[B]);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int q = (int)(t % faces);
+    const long line = t / faces;
+    int c[3];
+    c[S] = g.lo[S] - 1 + q;
+    c[A] = g.lo[A] + (int)(line % na);
+    c[B] = g.lo[B] + (int)(line / na);
+    const int stride = (S == 0) ? 1 : (S == 1) ? g.nx : g.nx * g.ny;
+    const int base = Index(g, c[0], c[1], c[2]);
+    double st[6][5];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) LoadNode(U, g.N, base + (m - 2) * stride, st[m]);
+    float x[artra::kFeat];
+    artra::face_features<6>(mdl.gamma, st, x);
+#pragma unroll
+    for (int k = 0; k < artra::kFeat; ++k) feat[t * artra::kFeat + k] = x[k];
+    artra::FaceState fs;
+    artra::face_state<S>(mdl, st[2], st[3], fs);
+    double F[5], lab = 0.0;
+    artra::weno_flux<S>(fs, st, F, &lab);
+    label[t] = (float)lab;
+}
+
 // ---------------------------------------------------------------- boundary
 struct BcBox {
     int lo[3], hi[3];
@@ -408,6 +445,291 @@ __global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__res
     StoreNode(ga.U, ga.N, d, out);
 }
 
+// ------------------------------------------------- fused directional sweep
+// All Runge-Kutta stages of one split directional operator for one grid line
+// per block: the line (ghosts included) is read from HBM once, the stages run
+// on it in LDS - face fluxes into an LDS face buffer, node updates in place,
+// the ghost ends refilled between stages by per-line copy rules - and the
+// interior is written back once.  Valid where a stage along S reads nothing
+// but its own line: inviscid, split, no immersed body, and a between-stage
+// fill that only copies along the line (derived from the resolved gather).
+// With HYB the learned indicator routes each face: every face first takes
+// the linear upwind flux, faces flagged troubled are queued in LDS and
+// recomputed with full WENO5 by the whole block, so WENO5 work stays dense
+// across lanes however the troubled faces are scattered.
+constexpr int kLineRules = 12;
+constexpr int kLineMaxK = 2;        // interior nodes per thread (K * blockDim >= interior)
+
+struct LineArgs {
+    double *U;                      // level TO: read at the start, written at the end
+    const uint8_t *fluid;
+    long N;
+    int n;                          // nodes along the line, ghosts included
+    int lo, hi;                     // interior range along the line
+    int n1;                         // interior extent of the first transverse axis
+    int lo1, lo2;                   // interior starts of the transverse axes
+    int stride, stride1, stride2;   // node-index strides along S and the transverse axes
+    int stages;
+    double a[3], b[3];
+    double r;                       // dt / ds
+    const double *dt;               // when set, the step's dt on the device (r = *dt * dd)
+    double dd;                      // 1 / ds
+    double gamma;
+    int nrules;
+    int rdst[kLineRules], rsrc[kLineRules];   // src >= 0: line position, src < 0: constant state -1 - src
+    double given[2][6];             // primitive states of constant (inflow) rules
+};
+
+struct LineHybrid {
+    const uint16_t *W1, *W2;        // bf16 weights
+    const float *bias;              // b1[32], b2[32], w3[32]
+    float b3, threshold;
+    unsigned long long *counts;     // [0] faces on the linear flux, [1] faces evaluated
+};
+
+// Blocks are dealt to the XCDs round robin; this bijection gives each XCD a
+// contiguous run of lines, so lines that share cache lines (adjacent x for
+// y and z lines) meet in the same L2.
+__device__ __forceinline__ long XcdLine(long b, long nb)
+{
+    const long q = nb / 8, rem = nb % 8;
+    const long x = b % 8, i = b / 8;
+    return (x < rem) ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
+}
+
+// LDS of one line: the states [5][n], the face fluxes [5][F] and (HYB) the
+// queue of troubled faces [F] with its counter.
+__host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F)
+{
+    return ((size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)(F + 2) * sizeof(int) + 15) / 16 * 16;
+}
+
+// All stages of the directional operator on one line, run by the T threads
+// tt = 0..T-1 (a multiple of the wavefront).  Every thread of the block must
+// call it with the same loop trip counts (same n, lo, hi, stages), because
+// the stages are separated by block barriers.  With write = false the line
+// is computed but not stored (padding slots of a batched block).
+template <int S, int NS, bool FAST, bool HYB, int K>
+__device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &mdl, const LineHybrid &hy, long line,
+                                         int tt, int T, char *mem, bool write, unsigned long long *tally,
+                                         const float *bias, const artra::MlpWave &frags)
+{
+    constexpr int SL = (NS == 6) ? 2 : 1;           // stencil nodes left of the face's left node
+    const int n = la.n, F = la.hi - la.lo + 1;
+    double *st = (double *)mem;                     // [5][n]
+    double *fb = st + 5 * n;                        // [5][F]
+    int *queue = (int *)(fb + 5 * F);               // [F]
+    int *queued = queue + F;
+    const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
+    const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
+    for (int p = tt; p < n; p += T) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) st[c * n + p] = la.U[c * la.N + base + (long)p * la.stride];
+    }
+    bool own[K], fl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = la.lo + tt + T * k;
+        own[k] = i < la.hi;
+        fl[k] = own[k] && la.fluid[base + (long)i * la.stride];
+    }
+    __syncthreads();
+    double U0[K][5];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = own[k] ? la.lo + tt + T * k : la.lo;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) U0[k][c] = st[c * n + i];
+    }
+    const int M = (F + T - 1) / T;
+    const double r = la.dt ? *la.dt * la.dd : la.r;
+    if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
+    for (int s = 0; s < la.stages; ++s) {
+        if (HYB) {
+            if (tt == 0) *queued = 0;
+            __syncthreads();
+        }
+        // face fluxes; the loop trip count is the same for every thread
+        for (int m = 0; m < M; ++m) {
+            const int q = tt + T * m;
+            const bool valid = q < F;
+            const int f = la.lo - 1 + (valid ? q : F - 1);   // face between f and f + 1
+            double Fx[5];
+            if (HYB) {
+                // the stencil is reduced to the face pair and the two linear
+                // combinations before the network runs, to keep registers low
+                float x[artra::kFeat];
+                double UL[5], UR[5], Up[5], Um[5];
+                {
+                    double w[6][5];
+#pragma unroll
+                    for (int j = 0; j < 6; ++j)
+#pragma unroll
+                        for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
+                    artra::face_features<6>(mdl.gamma, w, x);
+                    artra::linear_averages(w, Up, Um);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        UL[c] = w[2][c];
+                        UR[c] = w[3][c];
+                    }
+                }
+                const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
+                const bool troubled = valid && z > hy.threshold;
+                artra::FaceState fs;
+                artra::face_state<S>(mdl, UL, UR, fs);
+                artra::linear_flux_from<S>(fs, Up, Um, Fx);
+                const unsigned long long mask = __ballot(troubled);
+                const unsigned long long seen = __ballot(valid && write);
+                int slot = 0;
+                if (__lane_id() == 0) {
+                    if (mask) slot = atomicAdd(queued, __popcll(mask));
+                    const unsigned long long routed = seen & ~mask;
+                    atomicAdd(&tally[0], (unsigned long long)__popcll(routed));
+                    atomicAdd(&tally[1], (unsigned long long)__popcll(seen));
+                }
+                slot = __shfl(slot, 0);
+                if (troubled) {
+                    const unsigned long long below = mask & ((1ull << __lane_id()) - 1ull);
+                    queue[slot + __popcll(below)] = q;
+                }
+            } else {
+                double w[NS][5];
+#pragma unroll
+                for (int j = 0; j < NS; ++j)
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - SL + j];
+                ConvFlux<S, NS, FAST>(mdl, w, Fx);
+            }
+            if (valid) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+            }
+        }
+        __syncthreads();
+        if (HYB) {
+            const int nq = *queued;
+            for (int e = tt; e < nq; e += T) {
+                const int q = queue[e];
+                const int f = la.lo - 1 + q;
+                double w[6][5];
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
+                double Fx[5];
+                ConvFlux<S, 6, true>(mdl, w, Fx);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+            }
+            __syncthreads();
+        }
+        // node updates (fluid_dynamics.c:243-274); each thread reads and writes only its own nodes
+        const double a = la.a[s], b = la.b[s];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (!fl[k]) continue;
+            const int i = la.lo + tt + T * k;
+            const int q = i - la.lo;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double FL = fb[c * F + q], FR = fb[c * F + q + 1];
+                const double phi = FAST ? FL - FR : ((FL - FR) + 0.0) - 0.0;
+                const double Un = st[c * n + i];
+                const double Uo = (s == 0) ? Un : U0[k][c];
+                st[c * n + i] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
+            }
+        }
+        __syncthreads();
+        if (s + 1 < la.stages) {
+            // refill the line's ghost ends from the stage's interior
+            for (int e = tt; e < la.nrules; e += T) {
+                const int d = la.rdst[e], q = la.rsrc[e];
+                double v[5];
+                if (q >= 0) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) v[c] = st[c * n + q];
+                } else {
+                    Conservative(la.gamma, la.given[-1 - q], v);
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c) st[c * n + d] = v[c];
+            }
+            __syncthreads();
+        }
+    }
+    if (!write) return;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (!fl[k]) continue;
+        const int i = la.lo + tt + T * k;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[c * n + i];
+    }
+}
+
+// One line per block.
+template <int S, int NS, bool FAST, bool HYB, int K>
+__global__ __launch_bounds__(256) void rk_line(LineArgs la, FluxModel mdl, LineHybrid hy)
+{
+    extern __shared__ double lds[];
+    __shared__ unsigned long long tally[2];
+    __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
+    artra::MlpWave frags;
+    if (HYB) {
+        for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
+        if (threadIdx.x < 2) tally[threadIdx.x] = 0ull;
+        artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
+    }
+    const long line = XcdLine(blockIdx.x, gridDim.x);
+    LineBody<S, NS, FAST, HYB, K>(la, mdl, hy, line, threadIdx.x, blockDim.x, (char *)lds, true, tally, bias, frags);
+    if (HYB && threadIdx.x == 0) {
+        atomicAdd(hy.counts, tally[0]);
+        atomicAdd(hy.counts + 1, tally[1]);
+    }
+}
+
+// Many independent cases in one persistent launch: blocks of 256 threads
+// walk a tile list; a tile is P = 256 / T lines of one case, each run by T
+// threads (the tile shape picked per grid size).  Lines past the case's end
+// in its last tile are computed on a copy of its last line and not stored.
+struct BatchTile {
+    int c;                          // case
+    int T;                          // threads per line
+    int count;                      // lines in this tile (<= 256 / T)
+    int pad;
+    long first;                     // first line of the tile
+};
+
+template <int S, int NS, bool HYB>
+__global__ __launch_bounds__(256) void rk_batch(const BatchTile *tiles, int ntiles, const LineArgs *args,
+                                                FluxModel mdl, LineHybrid hy)
+{
+    extern __shared__ double lds[];
+    __shared__ unsigned long long tally[2];
+    __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
+    artra::MlpWave frags;
+    if (HYB) {
+        for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
+        if (threadIdx.x < 2) tally[threadIdx.x] = 0ull;
+        artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
+    }
+    for (long w = blockIdx.x; w < ntiles; w += gridDim.x) {
+        const BatchTile bt = tiles[w];
+        const LineArgs &la = args[bt.c];
+        const int slot = threadIdx.x / bt.T, tt = threadIdx.x % bt.T;
+        const bool write = slot < bt.count;
+        const long line = bt.first + (write ? slot : bt.count - 1);
+        char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1);
+        LineBody<S, NS, true, HYB, 2>(la, mdl, hy, line, tt, bt.T, mem, write, tally, bias, frags);
+        __syncthreads();
+    }
+    if (HYB && threadIdx.x == 0) {
+        atomicAdd(hy.counts, tally[0]);
+        atomicAdd(hy.counts + 1, tally[1]);
+    }
+}
+
 // ------------------------------------------------------- immersed boundary
 struct GhostDev {
     int node, kind, first, count;
@@ -518,6 +840,146 @@ __global__ __launch_bounds__(256) void cfl_max(const double *U, const uint8_t *f
     }
 }
 
+// ------------------------------------------------ batched cases: helpers
+// Per-case data of the batched time step.
+struct BatchCase {
+    const double *U;                // level TO
+    const uint8_t *fluid;
+    Grid g;
+    double d[3];                    // cell sizes
+    double cfl;
+    double gamma, gasR;
+};
+
+struct CflChunk {
+    int c;
+    int pad;
+    long start, count;              // interior nodes [start, start + count) of case c
+};
+
+// Per-axis maximum characteristic speed of every case (solve.c:108-153),
+// one chunk of one case per block iteration.
+__global__ __launch_bounds__(256) void cfl_batch(const CflChunk *chunks, int nchunks, const BatchCase *cases,
+                                                 unsigned long long *vmax)
+{
+    __shared__ double red[3][256];
+    for (long w = blockIdx.x; w < nchunks; w += gridDim.x) {
+        const CflChunk ch = chunks[w];
+        const BatchCase &bc = cases[ch.c];
+        const Grid &g = bc.g;
+        const int ex = g.hi[0] - g.lo[0], ey = g.hi[1] - g.lo[1];
+        double m[3] = {0.0, 0.0, 0.0};
+        for (long t = ch.start + threadIdx.x; t < ch.start + ch.count; t += blockDim.x) {
+            const int i = g.lo[0] + (int)(t % ex);
+            const int j = g.lo[1] + (int)((t / ex) % ey);
+            const int k = g.lo[2] + (int)(t / ((long)ex * ey));
+            const int idx = Index(g, i, j, k);
+            if (!bc.fluid[idx]) continue;
+            double Un[5], Uo[6];
+            LoadNode(bc.U, g.N, idx, Un);
+            Primitive(bc.gamma, bc.gasR, Un, Uo);
+            const double c = sqrt(bc.gamma * bc.gasR * Uo[5]);
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                const double V = fabs(Uo[s + 1]) + c;
+                if (m[s] < V) m[s] = V;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) red[s][threadIdx.x] = m[s];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) {
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    const double o = red[s][threadIdx.x + h];
+                    if (red[s][threadIdx.x] < o) red[s][threadIdx.x] = o;
+                }
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                atomicMax(vmax + 3 * ch.c + s, (unsigned long long)__double_as_longlong(red[s][0]));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Time step of every case from its speeds, with the solver loop's
+// rectification to output instants and the end time (solve.c:65-78).
+// clock[c] = {now, end, rcInt, tmInt}; half[c] receives dt / 2, the step of
+// each Strang-split directional operator.
+__global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long *vmax, double *clock, double *dt,
+                         double *half)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const BatchCase &bc = cases[c];
+    double *ck = clock + 4 * c;
+    if (!(ck[0] < ck[1])) {         // reached its end time: the case stays as it is
+        dt[c] = 0.0;
+        half[c] = 0.0;
+        return;
+    }
+    double v[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) v[s] = __longlong_as_double((long long)vmax[3 * c + s]);
+    const double tx = bc.d[0] / v[0], ty = bc.d[1] / v[1], tz = bc.d[2] / v[2];
+    const double tyz = (ty < tz) ? ty : tz;
+    double h = bc.cfl * ((tx < tyz) ? tx : tyz);
+    double now = ck[0], rc = ck[2];
+    const double end = ck[1], tm = ck[3];
+    if (rc + h > tm) {
+        h = tm - rc;
+        rc = 0.0;
+    } else {
+        rc = rc + h;
+    }
+    now = now + h;
+    if (now > end) {
+        h = end - (now - h);
+        now = end;
+    }
+    ck[0] = now;
+    ck[2] = rc;
+    dt[c] = h;
+    half[c] = 0.5 * h;
+}
+
+struct GatherCase {
+    double *U;                      // level TO
+    long N;
+    double gamma;
+    double given[7][6];
+};
+
+// The resolved boundary fill of every case (bc_gather for many cases).
+__global__ __launch_bounds__(256) void gather_batch(const GatherCase *cases, const int *__restrict__ cid,
+                                                    const int *__restrict__ dst, const int *__restrict__ src,
+                                                    long count)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const GatherCase &gc = cases[cid[t]];
+    const int d = dst[t], q = src[t];
+    double out[5];
+    if (q >= 0) {
+        LoadNode(gc.U, gc.N, q, out);
+    } else {
+        Conservative(gc.gamma, gc.given[-3 - q], out);
+    }
+    StoreNode(gc.U, gc.N, d, out);
+}
+
+__global__ void fill_i32(int *p, long n, int v)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
 // Plain streaming copy: the reference point for HBM counters and bandwidth.
 __global__ __launch_bounds__(256) void stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n)
 {
@@ -555,6 +1017,22 @@ struct ArtraDevice {
     int segment = 32;
     int exact = 0;                  // 1: reference operation order (bit-faithful), 0: fast arithmetic
     int partialFill = 1;            // axis-only fills between the stages of inviscid split sweeps
+    int hybrid = 0;                 // learned-indicator routing of WENO5 faces
+    float *mlpParams = nullptr;     // FP32 biases/output vector (device)
+    uint16_t *mlpW1 = nullptr, *mlpW2 = nullptr;
+    float mlpThreshold = 0.0f;
+    float mlpB3 = 0.0f;
+    unsigned long long *hybCounts = nullptr;
+    float *mlpBias = nullptr;       // b1, b2, w3 contiguous (96 FP32)
+    // fused directional sweeps (rk_line)
+    int fused = 1;                  // allowed by the caller
+    int allFluid = 0;               // every interior node is fluid
+    struct LineRules {
+        int ok = 0;
+        int count = 0;
+        int dst[kLineRules], src[kLineRules];
+        double given[2][6];
+    } lines[3];
     // timing
     bool timing = false;
     double ms[ARTRA_T_COUNT] = {0.0, 0.0, 0.0, 0.0};
@@ -721,12 +1199,213 @@ int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
     return 0;
 }
 
+// Launch shape of rk_line for axis p: threads, interior nodes per thread, LDS bytes.
+struct LineShape {
+    int T = 0, K = 0;
+    size_t lds = 0;
+    long lines = 0;
+};
+
+LineShape LineShapeOf(const ArtraDevice *dev, int p)
+{
+    const Grid &g = dev->g;
+    const int ext[3] = {g.nx, g.ny, g.nz};
+    const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
+    LineShape sh;
+    const int interior = g.hi[p] - g.lo[p];
+    if (interior <= 0) return sh;
+    sh.K = (interior <= 256) ? 1 : 2;
+    const int per = (interior + sh.K - 1) / sh.K;
+    sh.T = ((per + 63) / 64) * 64;
+    if (sh.T > 256) return LineShape{};
+    const int F = interior + 1;
+    sh.lds = LineLdsBytes(ext[p], F);
+    sh.lines = (long)(g.hi[t1] - g.lo[t1]) * (g.hi[t2] - g.lo[t2]);
+    return sh;
+}
+
+// Whether one directional operator along p can run as one fused rk_line launch.
+bool LineEligible(const ArtraDevice *dev, int p)
+{
+    if (!dev->fused || !dev->lines[p].ok || !dev->allFluid) return false;
+    if (dev->layerStart.back() != 0) return false;               // immersed-boundary ghosts present
+    if (dev->mdl.refMu > 0.0 || dev->par.multidim != 0) return false;
+    if (!dev->gatherResolved || !dev->useGather || !dev->partialFill) return false;
+    const LineShape sh = LineShapeOf(dev, p);
+    return sh.T > 0 && sh.lines > 0 && sh.lds <= 64 * 1024;
+}
+
+template <int S, int NS, bool FAST, bool HYB>
+void LaunchLineK(ArtraDevice *dev, const LineShape &sh, const LineArgs &la, const LineHybrid &hy)
+{
+    if (sh.K == 1) {
+        hipLaunchKernelGGL((rk_line<S, NS, FAST, HYB, 1>), dim3((unsigned)sh.lines), dim3(sh.T), sh.lds, dev->stream,
+                           la, dev->mdl, hy);
+    } else {
+        hipLaunchKernelGGL((rk_line<S, NS, FAST, HYB, 2>), dim3((unsigned)sh.lines), dim3(sh.T), sh.lds, dev->stream,
+                           la, dev->mdl, hy);
+    }
+}
+
+template <int S>
+void LaunchLineS(ArtraDevice *dev, const LineShape &sh, const LineArgs &la, const LineHybrid &hy)
+{
+    const bool fast = !dev->exact;
+    if (dev->par.sScheme == 1) {
+        if (dev->hybrid) LaunchLineK<S, 6, true, true>(dev, sh, la, hy);
+        else if (fast) LaunchLineK<S, 6, true, false>(dev, sh, la, hy);
+        else LaunchLineK<S, 6, false, false>(dev, sh, la, hy);
+    } else {
+        if (fast) LaunchLineK<S, 4, true, false>(dev, sh, la, hy);
+        else LaunchLineK<S, 4, false, false>(dev, sh, la, hy);
+    }
+}
+
+// All RK stages of the directional operator along p in one launch; the
+// result lands in TO, whose full boundary fill follows.
+int LineSweep(ArtraDevice *dev, double dt, int p)
+{
+    const Grid &g = dev->g;
+    const LineShape sh = LineShapeOf(dev, p);
+    const int ext[3] = {g.nx, g.ny, g.nz};
+    const int stride[3] = {1, g.nx, g.nx * g.ny};
+    const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
+    LineArgs la{};
+    la.U = Level(dev, 0);
+    la.fluid = dev->fluid;
+    la.N = g.N;
+    la.n = ext[p];
+    la.lo = g.lo[p];
+    la.hi = g.hi[p];
+    la.n1 = g.hi[t1] - g.lo[t1];
+    la.lo1 = g.lo[t1];
+    la.lo2 = g.lo[t2];
+    la.stride = stride[p];
+    la.stride1 = stride[t1];
+    la.stride2 = stride[t2];
+    if (dev->par.tScheme == 0) {
+        la.stages = 2;
+        la.a[0] = 0.0; la.b[0] = 1.0;
+        la.a[1] = 1.0 / 2.0; la.b[1] = 1.0 / 2.0;
+    } else {
+        la.stages = 3;
+        la.a[0] = 0.0; la.b[0] = 1.0;
+        la.a[1] = 3.0 / 4.0; la.b[1] = 1.0 / 4.0;
+        la.a[2] = 1.0 / 3.0; la.b[2] = 2.0 / 3.0;
+    }
+    la.r = dt * dev->par.dd[p];
+    la.dt = nullptr;
+    la.dd = dev->par.dd[p];
+    la.gamma = dev->par.gamma;
+    const ArtraDevice::LineRules &R = dev->lines[p];
+    la.nrules = R.count;
+    for (int e = 0; e < R.count; ++e) {
+        la.rdst[e] = R.dst[e];
+        la.rsrc[e] = R.src[e];
+    }
+    memcpy(la.given, R.given, sizeof la.given);
+    LineHybrid hy{};
+    if (dev->hybrid) {
+        hy.W1 = dev->mlpW1;
+        hy.W2 = dev->mlpW2;
+        hy.bias = dev->mlpBias;
+        hy.b3 = dev->mlpB3;
+        hy.threshold = dev->mlpThreshold;
+        hy.counts = dev->hybCounts;
+    }
+    int slot;
+    if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
+    if (p == 0) LaunchLineS<0>(dev, sh, la, hy);
+    else if (p == 1) LaunchLineS<1>(dev, sh, la, hy);
+    else LaunchLineS<2>(dev, sh, la, hy);
+    HIP_TRY(hipGetLastError());
+    return End(dev, slot);
+}
+
+// Per-line copy rules of the between-stage fill along axis p, read off the
+// resolved gather of the faces normal to p: every interior line must see the
+// same rules (destination position along the line <- source position on the
+// same line, or a constant state), covering every ghost the stencil reads.
+void DeriveLineRules(ArtraDevice *dev, const ArtraBcGather *gt)
+{
+    const Grid &g = dev->g;
+    const int ext[3] = {g.nx, g.ny, g.nz};
+    const int half = (dev->par.sScheme == 1) ? 3 : 2;
+    for (int p = 0; p < 3; ++p) {
+        ArtraDevice::LineRules &R = dev->lines[p];
+        R = ArtraDevice::LineRules{};
+        if (!gt->resolved || g.hi[p] - g.lo[p] <= 0) continue;
+        const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
+        const long n1 = g.hi[t1] - g.lo[t1], n2 = g.hi[t2] - g.lo[t2];
+        if (n1 <= 0 || n2 <= 0) continue;
+        std::vector<std::vector<std::pair<int, int>>> per((size_t)(n1 * n2));
+        int faces[2] = {-1, -1}, nfaces = 0;
+        bool ok = true;
+        for (long e = 0; e < gt->axis_count[p] && ok; ++e) {
+            const int d = gt->axis_dst[p][e], q = gt->axis_src[p][e];
+            const int c[3] = {d % g.nx, (d / g.nx) % g.ny, d / (g.nx * g.ny)};
+            if (c[t1] < g.lo[t1] || c[t1] >= g.hi[t1] || c[t2] < g.lo[t2] || c[t2] >= g.hi[t2]) continue;
+            int src;
+            if (q >= 0) {
+                const int cq[3] = {q % g.nx, (q / g.nx) % g.ny, q / (g.nx * g.ny)};
+                if (cq[t1] != c[t1] || cq[t2] != c[t2] || cq[p] < g.lo[p] || cq[p] >= g.hi[p]) ok = false;
+                src = cq[p];
+            } else {
+                const int face = -3 - q;
+                int slot = -1;
+                for (int k = 0; k < nfaces; ++k)
+                    if (faces[k] == face) slot = k;
+                if (slot < 0) {
+                    if (nfaces == 2) {
+                        ok = false;
+                        break;
+                    }
+                    slot = nfaces;
+                    faces[nfaces++] = face;
+                }
+                src = -1 - slot;
+            }
+            per[(size_t)((c[t1] - g.lo[t1]) + n1 * (c[t2] - g.lo[t2]))].emplace_back(c[p], src);
+        }
+        if (!ok) continue;
+        for (auto &v : per) std::sort(v.begin(), v.end());
+        const auto &ref = per[0];
+        for (const auto &v : per) {
+            if (v != ref) {
+                ok = false;
+                break;
+            }
+        }
+        if (!ok || (int)ref.size() > kLineRules) continue;
+        for (int pos = g.lo[p] - half; pos < g.hi[p] + half; ++pos) {
+            if (pos >= g.lo[p] && pos < g.hi[p]) continue;
+            bool hit = pos >= 0 && pos < ext[p];
+            if (hit) {
+                hit = false;
+                for (const auto &r : ref) hit = hit || r.first == pos;
+            }
+            ok = ok && hit;
+        }
+        if (!ok) continue;
+        R.count = (int)ref.size();
+        for (int e = 0; e < R.count; ++e) {
+            R.dst[e] = ref[e].first;
+            R.src[e] = ref[e].second;
+        }
+        for (int k = 0; k < nfaces; ++k)
+            for (int v = 0; v < 6; ++v) R.given[k][v] = dev->par.varBC[faces[k]][v];
+        R.ok = 1;
+    }
+}
+
 // Runge-Kutta stages of one directional operator (fluid_dynamics.c:121-148).
 // Between the stages of an inviscid sweep along one axis only that axis's
 // ghosts are read, so only they are filled; the last stage fills everything,
 // exactly as TreatBoundary does after every stage.
 int Stages(ArtraDevice *dev, double dt, int p)
 {
+    if (p < 3 && LineEligible(dev, p)) return LineSweep(dev, dt, p) || TreatBoundary(dev, 0);
+    if (dev->hybrid) return Fail("the learned indicator is on but axis %d cannot run fused", p);
     const int mid = (p < 3 && !(dev->mdl.refMu > 0.0) && dev->partialFill) ? 1 + p : 0;
     if (dev->par.tScheme == 0) {
         if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1, mid)) return 1;
@@ -791,6 +1470,15 @@ int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevic
         for (int q = 0; q < 6; ++q) ba.face[p].given[q] = par->varBC[p][q];
     }
     dev->layerStart.assign(1, 0);
+    dev->allFluid = 1;
+    for (int k = g.lo[2]; k < g.hi[2] && dev->allFluid; ++k)
+        for (int j = g.lo[1]; j < g.hi[1] && dev->allFluid; ++j)
+            for (int i = g.lo[0]; i < g.hi[0]; ++i) {
+                if (!fluid[((long)k * g.ny + j) * g.nx + i]) {
+                    dev->allFluid = 0;
+                    break;
+                }
+            }
     *out = dev;
     return 0;
 }
@@ -810,6 +1498,11 @@ void artra_dev_destroy(ArtraDevice *dev)
         hipFree(dev->gDst[m]);
         hipFree(dev->gSrc[m]);
     }
+    hipFree(dev->mlpParams);
+    hipFree(dev->mlpW1);
+    hipFree(dev->mlpW2);
+    hipFree(dev->hybCounts);
+    hipFree(dev->mlpBias);
     if (dev->stream) hipStreamDestroy(dev->stream);
     delete dev;
 }
@@ -858,6 +1551,7 @@ int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
         dev->gCount[m] = 0;
     }
     dev->gatherResolved = 0;
+    DeriveLineRules(dev, gt);
     if (!gt->resolved) return 0;
     for (int m = 0; m < 4; ++m) {
         const long n = (m == 0) ? gt->count : gt->axis_count[m - 1];
@@ -1050,6 +1744,115 @@ int artra_dev_copy_probe(long bytes, int iters, double *gbps)
     return 0;
 }
 
+int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float threshold)
+{
+    if (params == nullptr) {
+        dev->hybrid = 0;
+        return 0;
+    }
+    if (dev->par.sScheme != 1) return Fail("the learned indicator routes WENO5 faces; this case uses WENO3");
+    if (dev->exact) return Fail("the learned indicator runs with the fast arithmetic only");
+    static const int order[9][6] = {{2, 1, 0, 0, 1, 2},    {2, 1, 1, 2, -1, -1},   {2, 0, 0, 2, -1, -1},
+                                    {1, 0, 0, 1, -1, -1},  {-1, -1, -1, -1, -1, -1}, {2, 2, -1, -1, -1, -1},
+                                    {-1, -1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1}, {0, 0, -1, -1, -1, -1}};
+    const int c = dev->par.collapse;
+    for (int q = 0; c >= 0 && c <= 8 && q < 6 && order[c][q] >= 0; ++q) {
+        if (!LineEligible(dev, order[c][q])) {
+            return Fail("the learned indicator runs inside the fused line sweeps: it needs an inviscid split case "
+                        "without immersed bodies whose line fill copies along lines and whose lines fit in LDS "
+                        "(axis %d does not qualify)", order[c][q]);
+        }
+    }
+    if (!dev->mlpParams) {
+        HIP_TRY(hipMalloc(&dev->mlpParams, ARTRA_MLP_PARAMS * sizeof(float)));
+        HIP_TRY(hipMalloc(&dev->mlpW1, 512 * sizeof(uint16_t)));
+        HIP_TRY(hipMalloc(&dev->mlpW2, 1024 * sizeof(uint16_t)));
+        HIP_TRY(hipMalloc(&dev->mlpBias, 96 * sizeof(float)));
+        HIP_TRY(hipMalloc(&dev->hybCounts, 2 * sizeof(unsigned long long)));
+    }
+    std::vector<uint16_t> w1(512), w2(1024);
+    for (int q = 0; q < 512; ++q) w1[q] = artra::f32_to_bf16(params[ARTRA_MLP_W1 + q]);
+    for (int q = 0; q < 1024; ++q) w2[q] = artra::f32_to_bf16(params[ARTRA_MLP_W2 + q]);
+    std::vector<float> bias(96);
+    for (int q = 0; q < 32; ++q) {
+        bias[q] = params[ARTRA_MLP_B1 + q];
+        bias[32 + q] = params[ARTRA_MLP_B2 + q];
+        bias[64 + q] = params[ARTRA_MLP_W3 + q];
+    }
+    HIP_TRY(hipMemcpy(dev->mlpParams, params, ARTRA_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dev->mlpW1, w1.data(), 512 * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dev->mlpW2, w2.data(), 1024 * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dev->mlpBias, bias.data(), 96 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dev->hybCounts, 0, 2 * sizeof(unsigned long long)));
+    dev->mlpB3 = params[ARTRA_MLP_B3];
+    dev->mlpThreshold = threshold;
+    dev->hybrid = 1;
+    return 0;
+}
+
+int artra_dev_set_fused(ArtraDevice *dev, int fused)
+{
+    if (!fused && dev->hybrid) return Fail("the learned indicator needs the fused line sweeps");
+    dev->fused = (fused != 0);
+    return 0;
+}
+
+int artra_dev_fused_axes(ArtraDevice *dev)
+{
+    int mask = 0;
+    for (int p = 0; p < 3; ++p)
+        if (LineEligible(dev, p)) mask |= 1 << p;
+    return mask;
+}
+
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset)
+{
+    if (!dev->hybCounts) {
+        counts[0] = counts[1] = 0;
+        return 0;
+    }
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    HIP_TRY(hipMemcpy(counts, dev->hybCounts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (reset) HIP_TRY(hipMemset(dev->hybCounts, 0, 2 * sizeof(unsigned long long)));
+    return 0;
+}
+
+long artra_dev_face_count(ArtraDevice *dev, int axis)
+{
+    const Grid &g = dev->g;
+    long n = g.hi[axis] - g.lo[axis] + 1;
+    for (int t = 0; t < 3; ++t) {
+        if (t != axis) n *= g.hi[t] - g.lo[t];
+    }
+    return n;
+}
+
+int artra_dev_sample_faces(ArtraDevice *dev, int axis, float *feat, float *label)
+{
+    const long n = artra_dev_face_count(dev, axis);
+    if (n <= 0) return 0;
+    // the six-node stencil of the faces lo-1/2 .. hi-1/2 spans nodes lo-3 .. hi+2
+    const int extent = (axis == 0) ? dev->g.nx : (axis == 1) ? dev->g.ny : dev->g.nz;
+    if (dev->g.lo[axis] - 3 < 0 || dev->g.hi[axis] + 2 > extent - 1) {
+        return Fail("axis %d has too few ghost layers for the six-node face stencil", axis);
+    }
+    float *df = nullptr, *dl = nullptr;
+    HIP_TRY(hipMalloc(&df, n * artra::kFeat * sizeof(float)));
+    HIP_TRY(hipMalloc(&dl, n * sizeof(float)));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (axis == 0) hipLaunchKernelGGL(sample_faces<0>, dim3(blocks), dim3(256), 0, dev->stream, dev->g, Level(dev, 0), dev->mdl, df, dl);
+    else if (axis == 1) hipLaunchKernelGGL(sample_faces<1>, dim3(blocks), dim3(256), 0, dev->stream, dev->g, Level(dev, 0), dev->mdl, df, dl);
+    else hipLaunchKernelGGL(sample_faces<2>, dim3(blocks), dim3(256), 0, dev->stream, dev->g, Level(dev, 0), dev->mdl, df, dl);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(dev->stream);
+    if (e == hipSuccess) e = hipMemcpy(feat, df, n * artra::kFeat * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(label, dl, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(df);
+    hipFree(dl);
+    if (e != hipSuccess) return Fail("face sampling failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
 int artra_dev_set_exact(ArtraDevice *dev, int exact)
 {
     dev->exact = (exact != 0);
@@ -1060,6 +1863,519 @@ int artra_dev_set_tuning(ArtraDevice *dev, int march_segment, int xblock)
 {
     (void)xblock;
     if (march_segment > 0) dev->segment = march_segment;
+    return 0;
+}
+
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched cases
+struct ArtraBatch {
+    std::vector<ArtraDevice *> cases;
+    int C = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int> threads;               // threads per line of each case (tuned)
+    int grid = 0;                           // persistent blocks of the line launches
+    LineArgs *dArgs[3] = {nullptr, nullptr, nullptr};
+    BatchTile *dTiles[3] = {nullptr, nullptr, nullptr};
+    int ntiles[3] = {0, 0, 0};
+    size_t lds[3] = {0, 0, 0};
+    BatchCase *dCases = nullptr;
+    CflChunk *dChunks = nullptr;
+    int nchunks = 0;
+    GatherCase *dGather = nullptr;
+    int *gCase = nullptr, *gDst = nullptr, *gSrc = nullptr;
+    long gCount = 0;
+    unsigned long long *vmax = nullptr;
+    double *clock = nullptr, *dt = nullptr, *half = nullptr;
+    int hybrid = 0;
+    LineHybrid hy{};
+    float *mlpBias = nullptr;
+    uint16_t *mlpW1 = nullptr, *mlpW2 = nullptr;
+    unsigned long long *counts = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graphHybrid = -1;
+    int cus = 256;
+};
+
+namespace {
+
+const int kBatchOrder[6] = {2, 1, 0, 0, 1, 2};
+
+void BatchDropGraph(ArtraBatch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Tiles of axis p for the current threads-per-line choice, in case order.
+int BatchBuildTiles(ArtraBatch *b, int p)
+{
+    std::vector<BatchTile> tiles;
+    size_t lds = 0;
+    for (int c = 0; c < b->C; ++c) {
+        const ArtraDevice *dev = b->cases[c];
+        const Grid &g = dev->g;
+        const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
+        const long lines = (long)(g.hi[t1] - g.lo[t1]) * (g.hi[t2] - g.lo[t2]);
+        const int T = b->threads[c], P = 256 / T;
+        const int ext[3] = {g.nx, g.ny, g.nz};
+        lds = std::max(lds, (size_t)P * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1));
+        for (long first = 0; first < lines; first += P) {
+            BatchTile bt{};
+            bt.c = c;
+            bt.T = T;
+            bt.count = (int)std::min<long>(P, lines - first);
+            bt.first = first;
+            tiles.push_back(bt);
+        }
+    }
+    if (lds > 64 * 1024) return Fail("batched lines need %zu bytes of LDS per block (64 KB at most)", lds);
+    hipFree(b->dTiles[p]);
+    b->dTiles[p] = nullptr;
+    HIP_TRY(hipMalloc(&b->dTiles[p], tiles.size() * sizeof(BatchTile)));
+    HIP_TRY(hipMemcpy(b->dTiles[p], tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice));
+    b->ntiles[p] = (int)tiles.size();
+    b->lds[p] = lds;
+    BatchDropGraph(b);
+    return 0;
+}
+
+void BatchLaunchLines(ArtraBatch *b, int p, hipStream_t st)
+{
+    const unsigned grid = (unsigned)std::max(1, std::min(b->grid, b->ntiles[p]));
+    const bool hyb = b->hybrid != 0;
+    const FluxModel mdl = b->cases[0]->mdl;
+#define ARTRA_BATCH_LAUNCH(S)                                                                                   \
+    do {                                                                                                        \
+        if (hyb)                                                                                                \
+            hipLaunchKernelGGL((rk_batch<S, 6, true>), dim3(grid), dim3(256), b->lds[p], st, b->dTiles[p],      \
+                               b->ntiles[p], b->dArgs[p], mdl, b->hy);                                          \
+        else                                                                                                    \
+            hipLaunchKernelGGL((rk_batch<S, 6, false>), dim3(grid), dim3(256), b->lds[p], st, b->dTiles[p],     \
+                               b->ntiles[p], b->dArgs[p], mdl, b->hy);                                          \
+    } while (0)
+    if (p == 0) ARTRA_BATCH_LAUNCH(0);
+    else if (p == 1) ARTRA_BATCH_LAUNCH(1);
+    else ARTRA_BATCH_LAUNCH(2);
+#undef ARTRA_BATCH_LAUNCH
+}
+
+// One time step of every case: speeds, dt, then Z Y X X Y Z, each operator a
+// persistent launch over all cases' lines followed by every case's fill.
+int BatchStep(ArtraBatch *b, hipStream_t st)
+{
+    hipLaunchKernelGGL(fill_u64, dim3((3 * b->C + 255) / 256), dim3(256), 0, st, b->vmax, 3 * b->C, 0ull);
+    hipLaunchKernelGGL(cfl_batch, dim3((unsigned)std::min(b->nchunks, 4 * b->cus)), dim3(256), 0, st, b->dChunks,
+                       b->nchunks, b->dCases, b->vmax);
+    hipLaunchKernelGGL(dt_batch, dim3((b->C + 63) / 64), dim3(64), 0, st, b->dCases, b->C, b->vmax, b->clock, b->dt,
+                       b->half);
+    for (int q = 0; q < 6; ++q) {
+        BatchLaunchLines(b, kBatchOrder[q], st);
+        if (b->gCount > 0) {
+            hipLaunchKernelGGL(gather_batch, dim3((unsigned)((b->gCount + 255) / 256)), dim3(256), 0, st, b->dGather,
+                               b->gCase, b->gDst, b->gSrc, b->gCount);
+        }
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *clock, const double *cfl,
+                           ArtraBatch **out)
+{
+    *out = nullptr;
+    if (count < 1) return Fail("a batch needs at least one case");
+    const ArtraDevice *d0 = devs[0];
+    for (int c = 0; c < count; ++c) {
+        const ArtraDevice *dev = devs[c];
+        if (dev->par.collapse != 0) return Fail("batched case %d: only three-dimensional cases are batched", c);
+        for (int p = 0; p < 3; ++p) {
+            if (!LineEligible(dev, p)) {
+                return Fail("batched case %d: axis %d cannot run as fused line sweeps (inviscid, split, no "
+                            "immersed body, resolvable fills and lines that fit in LDS are required)", c, p);
+            }
+        }
+        if (dev->par.sScheme != 1 || dev->exact) return Fail("batched case %d: WENO5 with the fast arithmetic only", c);
+        if (dev->par.tScheme != d0->par.tScheme || dev->mdl.gamma != d0->mdl.gamma || dev->mdl.roe != d0->mdl.roe ||
+            dev->mdl.sw != d0->mdl.sw) {
+            return Fail("batched case %d: every case must share the time scheme and flux model", c);
+        }
+        if (dev->g.hi[0] - dev->g.lo[0] > 512 || dev->g.hi[1] - dev->g.lo[1] > 512 || dev->g.hi[2] - dev->g.lo[2] > 512)
+            return Fail("batched case %d: lines longer than 512 interior nodes", c);
+    }
+    ArtraBatch *b = new ArtraBatch();
+    b->cases.assign(devs, devs + count);
+    b->C = count;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        b->cus = prop.multiProcessorCount;
+    }
+    b->grid = 4 * b->cus;
+    auto fail = [&](int rc) {
+        artra_dev_batch_destroy(b);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(Fail("stream creation failed"));
+    // threads per line: the smallest wavefront multiple that covers the line with two nodes per lane
+    b->threads.resize(count);
+    for (int c = 0; c < count; ++c) {
+        const Grid &g = devs[c]->g;
+        int longest = 0;
+        for (int p = 0; p < 3; ++p) longest = std::max(longest, g.hi[p] - g.lo[p]);
+        int T = 64;
+        while (2 * T < longest) T *= 2;
+        b->threads[c] = std::min(T, 256);
+    }
+    // line arguments of every case and axis
+    std::vector<LineArgs> hostArgs[3];
+    for (int p = 0; p < 3; ++p) {
+        std::vector<LineArgs> args(count);
+        for (int c = 0; c < count; ++c) {
+            ArtraDevice *dv = devs[c];
+            const Grid &g = dv->g;
+            const int ext[3] = {g.nx, g.ny, g.nz};
+            const int stride[3] = {1, g.nx, g.nx * g.ny};
+            const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
+            LineArgs la{};
+            la.U = Level(dv, 0);
+            la.fluid = dv->fluid;
+            la.N = g.N;
+            la.n = ext[p];
+            la.lo = g.lo[p];
+            la.hi = g.hi[p];
+            la.n1 = g.hi[t1] - g.lo[t1];
+            la.lo1 = g.lo[t1];
+            la.lo2 = g.lo[t2];
+            la.stride = stride[p];
+            la.stride1 = stride[t1];
+            la.stride2 = stride[t2];
+            if (dv->par.tScheme == 0) {
+                la.stages = 2;
+                la.a[0] = 0.0; la.b[0] = 1.0;
+                la.a[1] = 1.0 / 2.0; la.b[1] = 1.0 / 2.0;
+            } else {
+                la.stages = 3;
+                la.a[0] = 0.0; la.b[0] = 1.0;
+                la.a[1] = 3.0 / 4.0; la.b[1] = 1.0 / 4.0;
+                la.a[2] = 1.0 / 3.0; la.b[2] = 2.0 / 3.0;
+            }
+            la.r = 0.0;
+            la.dd = dv->par.dd[p];
+            la.gamma = dv->par.gamma;
+            const ArtraDevice::LineRules &R = dv->lines[p];
+            la.nrules = R.count;
+            for (int e = 0; e < R.count; ++e) {
+                la.rdst[e] = R.dst[e];
+                la.rsrc[e] = R.src[e];
+            }
+            memcpy(la.given, R.given, sizeof la.given);
+            args[c] = la;
+        }
+        if (hipMalloc(&b->dArgs[p], count * sizeof(LineArgs)) != hipSuccess) return fail(Fail("out of device memory"));
+        hostArgs[p] = args;
+    }
+    if (hipMalloc(&b->vmax, 3 * count * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&b->clock, 4 * count * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b->dt, count * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b->half, count * sizeof(double)) != hipSuccess) {
+        return fail(Fail("out of device memory"));
+    }
+    for (int p = 0; p < 3; ++p) {
+        // every operator of a case advances by half of the case's step, read on the device
+        for (int c = 0; c < count; ++c) hostArgs[p][c].dt = b->half + c;
+        if (hipMemcpy(b->dArgs[p], hostArgs[p].data(), count * sizeof(LineArgs), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(Fail("upload of the line arguments failed"));
+    }
+    if (hipMemcpy(b->clock, clock, 4 * count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(Fail("upload of the case clocks failed"));
+    for (int p = 0; p < 3; ++p)
+        if (BatchBuildTiles(b, p)) return fail(1);
+    // CFL chunks and per-case data
+    std::vector<BatchCase> bcs(count);
+    std::vector<CflChunk> chunks;
+    std::vector<GatherCase> gcs(count);
+    for (int c = 0; c < count; ++c) {
+        ArtraDevice *dv = devs[c];
+        BatchCase &x = bcs[c];
+        x.U = Level(dv, 0);
+        x.fluid = dv->fluid;
+        x.g = dv->g;
+        for (int s = 0; s < 3; ++s) x.d[s] = dv->par.d[s];
+        x.cfl = cfl[c];
+        x.gamma = dv->par.gamma;
+        x.gasR = dv->par.gasR;
+        const Grid &g = dv->g;
+        const long total = (long)(g.hi[0] - g.lo[0]) * (g.hi[1] - g.lo[1]) * (g.hi[2] - g.lo[2]);
+        const long per = 256L * 16;
+        for (long st = 0; st < total; st += per) chunks.push_back(CflChunk{c, 0, st, std::min(per, total - st)});
+        GatherCase &gc = gcs[c];
+        gc.U = Level(dv, 0);
+        gc.N = g.N;
+        gc.gamma = dv->par.gamma;
+        for (int q = 0; q < 7; ++q)
+            for (int v = 0; v < 6; ++v) gc.given[q][v] = dv->par.varBC[q][v];
+        b->gCount += dv->gCount[0];
+    }
+    b->nchunks = (int)chunks.size();
+    if (hipMalloc(&b->dCases, count * sizeof(BatchCase)) != hipSuccess ||
+        hipMalloc(&b->dChunks, chunks.size() * sizeof(CflChunk)) != hipSuccess ||
+        hipMalloc(&b->dGather, count * sizeof(GatherCase)) != hipSuccess) {
+        return fail(Fail("out of device memory"));
+    }
+    if (hipMemcpy(b->dCases, bcs.data(), count * sizeof(BatchCase), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->dChunks, chunks.data(), chunks.size() * sizeof(CflChunk), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->dGather, gcs.data(), count * sizeof(GatherCase), hipMemcpyHostToDevice) != hipSuccess) {
+        return fail(Fail("upload of the batch tables failed"));
+    }
+    if (b->gCount > 0) {
+        if (hipMalloc(&b->gCase, b->gCount * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->gDst, b->gCount * sizeof(int)) != hipSuccess ||
+            hipMalloc(&b->gSrc, b->gCount * sizeof(int)) != hipSuccess) {
+            return fail(Fail("out of device memory"));
+        }
+        long off = 0;
+        for (int c = 0; c < count; ++c) {
+            const long n = devs[c]->gCount[0];
+            if (n == 0) continue;
+            if (hipMemcpy(b->gDst + off, devs[c]->gDst[0], n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess ||
+                hipMemcpy(b->gSrc + off, devs[c]->gSrc[0], n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess) {
+                return fail(Fail("copy of the boundary tables failed"));
+            }
+            hipLaunchKernelGGL(fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->gCase + off, n, c);
+            off += n;
+        }
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(Fail("boundary table set-up failed"));
+    }
+    *out = b;
+    return 0;
+}
+
+void artra_dev_batch_destroy(ArtraBatch *b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    BatchDropGraph(b);
+    for (int p = 0; p < 3; ++p) {
+        hipFree(b->dArgs[p]);
+        hipFree(b->dTiles[p]);
+    }
+    hipFree(b->dCases);
+    hipFree(b->dChunks);
+    hipFree(b->dGather);
+    hipFree(b->gCase);
+    hipFree(b->gDst);
+    hipFree(b->gSrc);
+    hipFree(b->vmax);
+    hipFree(b->clock);
+    hipFree(b->dt);
+    hipFree(b->half);
+    hipFree(b->mlpBias);
+    hipFree(b->mlpW1);
+    hipFree(b->mlpW2);
+    hipFree(b->counts);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float threshold)
+{
+    BatchDropGraph(b);
+    if (!params) {
+        b->hybrid = 0;
+        return 0;
+    }
+    if (!b->mlpBias) {
+        HIP_TRY(hipMalloc(&b->mlpBias, 96 * sizeof(float)));
+        HIP_TRY(hipMalloc(&b->mlpW1, 512 * sizeof(uint16_t)));
+        HIP_TRY(hipMalloc(&b->mlpW2, 1024 * sizeof(uint16_t)));
+        HIP_TRY(hipMalloc(&b->counts, 2 * sizeof(unsigned long long)));
+    }
+    std::vector<uint16_t> w1(512), w2(1024);
+    for (int q = 0; q < 512; ++q) w1[q] = artra::f32_to_bf16(params[ARTRA_MLP_W1 + q]);
+    for (int q = 0; q < 1024; ++q) w2[q] = artra::f32_to_bf16(params[ARTRA_MLP_W2 + q]);
+    std::vector<float> bias(96);
+    for (int q = 0; q < 32; ++q) {
+        bias[q] = params[ARTRA_MLP_B1 + q];
+        bias[32 + q] = params[ARTRA_MLP_B2 + q];
+        bias[64 + q] = params[ARTRA_MLP_W3 + q];
+    }
+    HIP_TRY(hipMemcpy(b->mlpW1, w1.data(), 512 * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->mlpW2, w2.data(), 1024 * sizeof(uint16_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->mlpBias, bias.data(), 96 * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
+    b->hy.W1 = b->mlpW1;
+    b->hy.W2 = b->mlpW2;
+    b->hy.bias = b->mlpBias;
+    b->hy.b3 = params[ARTRA_MLP_B3];
+    b->hy.threshold = threshold;
+    b->hy.counts = b->counts;
+    b->hybrid = 1;
+    return 0;
+}
+
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int reset)
+{
+    counts[0] = counts[1] = 0;
+    if (!b->counts) return 0;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(counts, b->counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    if (reset) HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
+    return 0;
+}
+
+// steps time steps of every case; with use_graph the step is captured once
+// in a HIP graph and replayed.  ms (optional) receives the wall time of the
+// steps measured with events on the batch stream.
+int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = 0;
+    if (use_graph && (!b->exec || b->graphHybrid != b->hybrid)) {
+        BatchDropGraph(b);
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int r1 = BatchStep(b, b->stream);
+        const hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
+        if (r1) return 1;
+        HIP_TRY(ce);
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graphHybrid = b->hybrid;
+    }
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    for (int q = 0; q < steps && !rc; ++q) {
+        if (use_graph) {
+            const hipError_t e = hipGraphLaunch(b->exec, b->stream);
+            if (e != hipSuccess) rc = Fail("graph launch failed: %s", hipGetErrorString(e));
+        } else {
+            rc = BatchStep(b, b->stream);
+        }
+    }
+    HIP_TRY(hipEventRecord(e1, b->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float el = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+    if (ms) *ms = el;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int artra_dev_batch_clock(ArtraBatch *b, double *clock)
+{
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(clock, b->clock, 4 * b->C * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Autotuner: for every grid size in the batch (cases of equal line length)
+// time the three directional launches with each admissible threads-per-line
+// choice (the tile shape: 256 / T lines of T threads per block), keep the
+// fastest; then time whole steps for persistent grids of 2, 4 and 8 blocks
+// per compute unit.  The field is restored afterwards (the tuning steps run
+// on a scratch copy of level TO in level TN of every case).
+int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_out)
+{
+    if (reps < 1) reps = 1;
+    // save TO of every case into TN, restore at the end
+    for (ArtraDevice *dv : b->cases) {
+        HIP_TRY(hipMemcpyAsync(Level(dv, 1), Level(dv, 0), 5 * dv->g.N * sizeof(double), hipMemcpyDeviceToDevice,
+                               b->stream));
+    }
+    std::vector<double> clockSave(4 * b->C);
+    HIP_TRY(hipMemcpyAsync(clockSave.data(), b->clock, clockSave.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    // one dt so the line launches have a valid step
+    if (BatchStep(b, b->stream)) return 1;
+    // distinct longest-line lengths
+    std::vector<int> lengths;
+    for (ArtraDevice *dv : b->cases) {
+        int L = 0;
+        for (int p = 0; p < 3; ++p) L = std::max(L, dv->g.hi[p] - dv->g.lo[p]);
+        if (std::find(lengths.begin(), lengths.end(), L) == lengths.end()) lengths.push_back(L);
+    }
+    const std::vector<int> saved = b->threads;
+    for (int L : lengths) {
+        double best = 1e300;
+        int bestT = 0;
+        for (int T = 64; T <= 256; T *= 2) {
+            if (2 * T < L) continue;            // two nodes per lane at most
+            if (T > 64 && T / 2 >= L + 1) break; // a whole wavefront idle
+            for (int c = 0; c < b->C; ++c) {
+                int Lc = 0;
+                for (int p = 0; p < 3; ++p) Lc = std::max(Lc, b->cases[c]->g.hi[p] - b->cases[c]->g.lo[p]);
+                b->threads[c] = (Lc == L) ? T : saved[c];
+            }
+            bool ok = true;
+            for (int p = 0; p < 3 && ok; ++p) ok = BatchBuildTiles(b, p) == 0;
+            if (!ok) continue;
+            HIP_TRY(hipEventRecord(e0, b->stream));
+            for (int r = 0; r < reps; ++r)
+                for (int p = 0; p < 3; ++p) BatchLaunchLines(b, p, b->stream);
+            HIP_TRY(hipEventRecord(e1, b->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipGetLastError());
+            float el = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+            if (el < best) {
+                best = el;
+                bestT = T;
+            }
+        }
+        if (bestT == 0) return Fail("no admissible tile shape for lines of %d nodes", L);
+        for (int c = 0; c < b->C; ++c) {
+            int Lc = 0;
+            for (int p = 0; p < 3; ++p) Lc = std::max(Lc, b->cases[c]->g.hi[p] - b->cases[c]->g.lo[p]);
+            if (Lc == L) b->threads[c] = bestT;
+        }
+    }
+    for (int p = 0; p < 3; ++p)
+        if (BatchBuildTiles(b, p)) return 1;
+    double bestG = 1e300;
+    int grid = b->grid;
+    for (int m = 2; m <= 8; m *= 2) {
+        b->grid = m * b->cus;
+        HIP_TRY(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; ++r)
+            if (BatchStep(b, b->stream)) return 1;
+        HIP_TRY(hipEventRecord(e1, b->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float el = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&el, e0, e1));
+        if (el < bestG) {
+            bestG = el;
+            grid = b->grid;
+        }
+    }
+    b->grid = grid;
+    BatchDropGraph(b);
+    // restore the fields and clocks
+    for (ArtraDevice *dv : b->cases) {
+        HIP_TRY(hipMemcpyAsync(Level(dv, 0), Level(dv, 1), 5 * dv->g.N * sizeof(double), hipMemcpyDeviceToDevice,
+                               b->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(b->clock, clockSave.data(), clockSave.size() * sizeof(double), hipMemcpyHostToDevice,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->counts) HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (int c = 0; c < b->C; ++c)
+        if (threads_out) threads_out[c] = b->threads[c];
+    if (grid_out) *grid_out = b->grid;
     return 0;
 }
 

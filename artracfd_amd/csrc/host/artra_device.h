@@ -12,6 +12,7 @@
 
 #include <stdint.h>
 #include "artra_host.h"
+#include "artra_hip.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -71,6 +72,46 @@ int artra_dev_get_timing(ArtraDevice *dev, double ms[ARTRA_T_COUNT], long launch
 /* arithmetic of the flux kernels: 1 keeps the reference's operation order
  * (bit-faithful), 0 (default) uses the fused/reciprocal form */
 int artra_dev_set_exact(ArtraDevice *dev, int exact);
+/* ------------------------------------------------ learned indicator */
+/* FP32 parameter vector of the 12-32-32-1 MLP (layout of indicator.hip) */
+#define ARTRA_MLP_PARAMS 1633
+#define ARTRA_MLP_W1 0
+#define ARTRA_MLP_B1 512
+#define ARTRA_MLP_W2 544
+#define ARTRA_MLP_B2 1568
+#define ARTRA_MLP_W3 1600
+#define ARTRA_MLP_B3 1632
+
+/* route WENO5 faces through the indicator (params NULL disables) */
+int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float threshold);
+/* faces sent to the linear flux and faces evaluated since the last reset */
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset);
+int artra_dev_set_fused(ArtraDevice *dev, int fused);
+int artra_dev_fused_axes(ArtraDevice *dev);
+/* faces along one axis of the interior lines, and their features/labels on level TO */
+long artra_dev_face_count(ArtraDevice *dev, int axis);
+int artra_dev_sample_faces(ArtraDevice *dev, int axis, float *feat, float *label);
+/* training / gradient / inference on the device (indicator.hip) */
+int artra_mlp_train_dev(float *params, const float *feat, const float *label, long n, const ArtraMlpTraining *opt,
+                        float *loss_history);
+int artra_mlp_gradient_dev(const float *params, const float *feat, const float *label, long n, float tau, float wpos,
+                           float *grad, float *loss);
+int artra_mlp_infer_dev(const float *params, const float *feat, long n, float *logit);
+const char *artra_mlp_error(void);
+
+/* batched cases: many independent cases advanced together, one persistent
+ * launch per directional operator over the lines of every case */
+typedef struct ArtraBatch ArtraBatch;
+/* clock[c] = {now, end, rcInt, tmInt} of case c; cfl[c] its CFL number */
+int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *clock, const double *cfl,
+                           ArtraBatch **out);
+void artra_dev_batch_destroy(ArtraBatch *b);
+int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float threshold);
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int reset);
+int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms);
+int artra_dev_batch_clock(ArtraBatch *b, double *clock);
+int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads, int *grid);
+
 /* tuning knobs (0 keeps the default) */
 int artra_dev_set_tuning(ArtraDevice *dev, int march_segment, int xblock);
 /* streaming copy of `bytes` bytes, iters times; returns read+write GB/s */

@@ -306,6 +306,68 @@ int artra_hip_copy_probe(long bytes, int iters, double *gbps)
     return artra_dev_copy_probe(bytes, iters, gbps) ? DevError() : 0;
 }
 
+/* ------------------------------------------- learned troubled-face indicator */
+static int MlpError(void)
+{
+    return artra_set_error(1, "%s", artra_mlp_error());
+}
+
+int artra_hip_face_count(ArtraHipSolver *s, int axis, long *count)
+{
+    if (axis < 0 || axis > 2) return artra_set_error(1, "bad axis %d", axis);
+    *count = artra_dev_face_count(s->dev, axis);
+    return 0;
+}
+
+int artra_hip_sample_faces(ArtraHipSolver *s, int axis, float *features, float *labels)
+{
+    if (axis < 0 || axis > 2) return artra_set_error(1, "bad axis %d", axis);
+    return artra_dev_sample_faces(s->dev, axis, features, labels) ? DevError() : 0;
+}
+
+int artra_hip_set_indicator(ArtraHipSolver *s, const float *params, float logit_threshold)
+{
+    return artra_dev_set_indicator(s->dev, params, logit_threshold) ? DevError() : 0;
+}
+
+int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[2], int reset)
+{
+    return artra_dev_hybrid_counts(s->dev, counts, reset) ? DevError() : 0;
+}
+
+int artra_hip_set_fused(ArtraHipSolver *s, int fused)
+{
+    return artra_dev_set_fused(s->dev, fused) ? DevError() : 0;
+}
+
+int artra_hip_fused_axes(ArtraHipSolver *s, int *axes)
+{
+    *axes = artra_dev_fused_axes(s->dev);
+    return 0;
+}
+
+int artra_mlp_train(float *params, const float *features, const float *labels, long n,
+                    const ArtraMlpTraining *options, float *loss_history)
+{
+    if (n < options->batch || options->batch <= 0 || options->batch % 64) {
+        return artra_set_error(1, "training needs at least one batch of a multiple of 64 samples");
+    }
+    return artra_mlp_train_dev(params, features, labels, n, options, loss_history) ? MlpError() : 0;
+}
+
+int artra_mlp_gradient(const float *params, const float *features, const float *labels, long n, float tau,
+                       float wpos, float *grad, float *loss)
+{
+    if (n <= 0) return artra_set_error(1, "no samples");
+    return artra_mlp_gradient_dev(params, features, labels, n, tau, wpos, grad, loss) ? MlpError() : 0;
+}
+
+int artra_mlp_logits(const float *params, const float *features, long n, float *logits)
+{
+    if (n <= 0) return artra_set_error(1, "no samples");
+    return artra_mlp_infer_dev(params, features, n, logits) ? MlpError() : 0;
+}
+
 /* ------------------------------------------------------------- drop-in */
 static struct {
     Space *space;
@@ -425,3 +487,111 @@ int artra_calc_expression(const char *text, Real x, Real y, Real z, Real *ans)
 {
     return artra_calc(text, 0.0, x, y, z, ans);
 }
+
+/* ------------------------------------------------------------ batched cases */
+struct ArtraHipBatch {
+    ArtraBatch *b;
+    int count;
+    ArtraHipSolver **solvers;
+    Time **times;
+};
+
+int artra_hip_batch_create(ArtraHipSolver *const *solvers, Time *const *times, int count, ArtraHipBatch **out)
+{
+    *out = NULL;
+    if (count < 1) return artra_set_error(1, "a batch needs at least one case");
+    ArtraHipBatch *hb = calloc(1, sizeof *hb);
+    ArtraDevice **devs = malloc(count * sizeof *devs);
+    double *clock = malloc(4 * count * sizeof *clock);
+    double *cfl = malloc(count * sizeof *cfl);
+    hb->solvers = malloc(count * sizeof *hb->solvers);
+    hb->times = malloc(count * sizeof *hb->times);
+    if (!devs || !clock || !cfl || !hb->solvers || !hb->times) {
+        free(devs); free(clock); free(cfl); free(hb->solvers); free(hb->times); free(hb);
+        return artra_set_error(1, "out of host memory");
+    }
+    int rc = 0;
+    for (int c = 0; c < count && !rc; ++c) {
+        const Time *t = times[c];
+        if (solvers[c]->space->geo.totN > 0) rc = artra_set_error(1, "batched case %d has bodies; batches take fluid-only cases", c);
+        devs[c] = solvers[c]->dev;
+        clock[4 * c + 0] = t->now;
+        clock[4 * c + 1] = t->end;
+        clock[4 * c + 2] = solvers[c]->rcInt;
+        clock[4 * c + 3] = (INT_MAX == t->dataW[PROSD]) ? t->end : t->end / (Real)(t->dataW[PROSD]);
+        cfl[c] = t->numCFL;
+        hb->solvers[c] = solvers[c];
+        hb->times[c] = times[c];
+    }
+    if (!rc && artra_dev_batch_create(devs, count, clock, cfl, &hb->b)) rc = DevError();
+    free(devs);
+    free(clock);
+    free(cfl);
+    if (rc) {
+        free(hb->solvers);
+        free(hb->times);
+        free(hb);
+        return rc;
+    }
+    hb->count = count;
+    *out = hb;
+    return 0;
+}
+
+int artra_hip_batch_destroy(ArtraHipBatch *hb)
+{
+    if (!hb) return 0;
+    artra_dev_batch_destroy(hb->b);
+    free(hb->solvers);
+    free(hb->times);
+    free(hb);
+    return 0;
+}
+
+int artra_hip_batch_tune(ArtraHipBatch *hb, int reps, int *threads, int *grid)
+{
+    return artra_dev_batch_tune(hb->b, reps, threads, grid) ? DevError() : 0;
+}
+
+int artra_hip_batch_set_indicator(ArtraHipBatch *hb, const float *params, float logit_threshold)
+{
+    return artra_dev_batch_set_indicator(hb->b, params, logit_threshold) ? DevError() : 0;
+}
+
+int artra_hip_batch_counts(ArtraHipBatch *hb, unsigned long long counts[2], int reset)
+{
+    return artra_dev_batch_counts(hb->b, counts, reset) ? DevError() : 0;
+}
+
+int artra_hip_batch_march(ArtraHipBatch *hb, int steps, int use_graph, double *ms)
+{
+    for (int c = 0; c < hb->count; ++c) {
+        const Time *t = hb->times[c];
+        if (t->stepC + steps > t->stepN) {
+            return artra_set_error(1, "batched case %d has %d steps left, fewer than %d", c, t->stepN - t->stepC,
+                                   steps);
+        }
+    }
+    if (artra_dev_batch_march(hb->b, steps, use_graph, ms)) return DevError();
+    double *clock = malloc(4 * hb->count * sizeof *clock);
+    if (!clock) return artra_set_error(1, "out of host memory");
+    if (artra_dev_batch_clock(hb->b, clock)) {
+        free(clock);
+        return DevError();
+    }
+    for (int c = 0; c < hb->count; ++c) {
+        hb->times[c]->now = clock[4 * c + 0];
+        hb->times[c]->stepC += steps;
+        hb->solvers[c]->rcInt = clock[4 * c + 2];
+    }
+    free(clock);
+    return 0;
+}
+
+int artra_case_clock(const ArtraCase *c, double *now, int *step)
+{
+    *now = c->time.now;
+    *step = c->time.stepC;
+    return 0;
+}
+

@@ -16,6 +16,14 @@ TO, TN, TM = 0, 1, 2
 _lib = None
 
 
+MLP_PARAMS = 1633
+
+
+class MlpTraining(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_int), ("batch", ctypes.c_int), ("lr", ctypes.c_float), ("tau", ctypes.c_float),
+                ("wpos", ctypes.c_float)]
+
+
 class ArtraError(RuntimeError):
     """A libartra_hip call returned a non-zero code."""
 
@@ -39,6 +47,7 @@ def library():
     lib = ctypes.CDLL(path)
     vp, ip, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     lp = ctypes.POINTER(ctypes.c_long)
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "artra_hip_attach": (ctypes.c_int, [vp, vp, ctypes.POINTER(vp)]),
         "artra_hip_detach": (ctypes.c_int, [vp]),
@@ -56,6 +65,8 @@ def library():
         "artra_hip_set_tuning": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_arithmetic": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_fill_options": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "artra_hip_set_fused": (ctypes.c_int, [vp, ctypes.c_int]),
+        "artra_hip_fused_axes": (ctypes.c_int, [vp, ip]),
         "artra_hip_error": (ctypes.c_char_p, []),
         "artra_hip_copy_probe": (ctypes.c_int, [ctypes.c_long, ctypes.c_int, dp]),
         "artra_case_open": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(vp)]),
@@ -67,8 +78,16 @@ def library():
         "artra_case_get_field": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "artra_case_set_field": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "artra_case_get_flags": (ctypes.c_int, [vp, ip]),
+        "artra_case_clock": (ctypes.c_int, [vp, dp, ip]),
         "artra_calc_expression": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, dp]),
+        "artra_hip_face_count": (ctypes.c_int, [vp, ctypes.c_int, lp]),
+        "artra_hip_sample_faces": (ctypes.c_int, [vp, ctypes.c_int, fp, fp]),
+        "artra_hip_set_indicator": (ctypes.c_int, [vp, fp, ctypes.c_float]),
+        "artra_hip_hybrid_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
+        "artra_mlp_train": (ctypes.c_int, [fp, fp, fp, ctypes.c_long, ctypes.POINTER(MlpTraining), fp]),
+        "artra_mlp_gradient": (ctypes.c_int, [fp, fp, fp, ctypes.c_long, ctypes.c_float, ctypes.c_float, fp, fp]),
+        "artra_mlp_logits": (ctypes.c_int, [fp, fp, ctypes.c_long, fp]),
         "EvolveFluidDynamics": (None, [ctypes.c_double, vp, vp]),
         "TreatBoundary": (None, [ctypes.c_int, vp, vp]),
     }
@@ -87,6 +106,39 @@ def _check(code):
 
 def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def mlp_train(params, features, labels, epochs=3, batch=4096, lr=3e-3, tau=0.1, wpos=4.0):
+    """Train the indicator on the device; returns (params, loss per epoch)."""
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    y = np.ascontiguousarray(labels, dtype=np.float32)
+    hist = np.zeros(epochs, dtype=np.float32)
+    opt = MlpTraining(epochs, batch, lr, tau, wpos)
+    _check(library().artra_mlp_train(_fptr(p), _fptr(f), _fptr(y), len(y), ctypes.byref(opt), _fptr(hist)))
+    return p, hist
+
+
+def mlp_gradient(params, features, labels, tau, wpos):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    y = np.ascontiguousarray(labels, dtype=np.float32)
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _check(library().artra_mlp_gradient(_fptr(p), _fptr(f), _fptr(y), len(y), tau, wpos, _fptr(g), _fptr(loss)))
+    return g, float(loss[0])
+
+
+def mlp_logits(params, features):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float32)
+    out = np.zeros(len(f), dtype=np.float32)
+    _check(library().artra_mlp_logits(_fptr(p), _fptr(f), len(f), _fptr(out)))
+    return out
 
 
 def calc(expr, x=0.0, y=0.0, z=0.0):
@@ -160,6 +212,12 @@ class Case:
         library().artra_case_get_flags(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return out.reshape(4, self.n[2], self.n[1], self.n[0])
 
+    def clock(self):
+        """(now, steps taken) of the case's time loop."""
+        now, step = ctypes.c_double(), ctypes.c_int()
+        library().artra_case_clock(self._h, ctypes.byref(now), ctypes.byref(step))
+        return now.value, step.value
+
     def interior(self, arr):
         """Restrict a [..., nz, ny, nx] array to the interior range."""
         (x0, x1), (y0, y1), (z0, z1) = self.pin
@@ -183,7 +241,7 @@ class Solver:
     ``initial_boundary`` applies TreatBoundary(TO) after the upload, which is
     the last step of the solver's InitializeComputeDomain."""
 
-    def __init__(self, case, initial_boundary=True, segment=0, exact=None, gather=True, partial=True):
+    def __init__(self, case, initial_boundary=True, segment=0, exact=None, gather=True, partial=True, fused=True):
         self.case = case
         lib = library()
         h = ctypes.c_void_p()
@@ -195,6 +253,8 @@ class Solver:
             _check(lib.artra_hip_set_fill_options(h, 1 if gather else 0, 1 if partial else 0))
         if segment:
             _check(lib.artra_hip_set_tuning(h, segment))
+        if not fused:
+            _check(lib.artra_hip_set_fused(h, 0))
         if initial_boundary:
             self.treat_boundary(TO)
 
@@ -246,6 +306,34 @@ class Solver:
         _check(library().artra_hip_get_timing(self._h, ms, n))
         names = ("sweep", "boundary", "immersed", "cfl")
         return {names[q]: (ms[q], n[q]) for q in range(4)}
+
+    def fused_axes(self):
+        """Axes (0 x, 1 y, 2 z) whose directional operator runs as one fused launch."""
+        m = ctypes.c_int()
+        _check(library().artra_hip_fused_axes(self._h, ctypes.byref(m)))
+        return [p for p in range(3) if m.value >> p & 1]
+
+    def sample_faces(self, axis):
+        """Features [n][12] and labels [n] of every face along an axis (level TO)."""
+        n = ctypes.c_long()
+        _check(library().artra_hip_face_count(self._h, axis, ctypes.byref(n)))
+        feat = np.zeros((n.value, 12), dtype=np.float32)
+        lab = np.zeros(n.value, dtype=np.float32)
+        _check(library().artra_hip_sample_faces(self._h, axis, _fptr(feat), _fptr(lab)))
+        return feat, lab
+
+    def set_indicator(self, params, logit_threshold=0.0):
+        """Route WENO5 faces with the learned indicator (None: plain WENO5)."""
+        if params is None:
+            _check(library().artra_hip_set_indicator(self._h, None, 0.0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(library().artra_hip_set_indicator(self._h, _fptr(p), float(logit_threshold)))
+
+    def hybrid_counts(self, reset=False):
+        c = (ctypes.c_ulonglong * 2)()
+        _check(library().artra_hip_hybrid_counts(self._h, c, 1 if reset else 0))
+        return int(c[0]), int(c[1])
 
     def close(self):
         if self._h:

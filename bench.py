@@ -5,16 +5,22 @@ Workload: the 256^3 shock-vortex interaction (cases/shockvortex256: Mach 1.5
 shock at x = 0.5, Gaussian isentropic vortex of strength 0.3 and radius 0.075
 at (0.75, 1, 1), domain [0,2]^3, inviscid, WENO5 + LLF + RK3 with Strang
 splitting, CFL 0.6), plain FP64 path.  A step is one full solver time step:
-the device CFL reduction and dt readback, 18 directional sweeps and 18
-boundary fills.  The metric counts cell-stage updates: interior fluid nodes x
-18 x steps / wall seconds.
+the device CFL reduction and dt readback, six directional operators of three
+RK stages each, and their boundary fills.  The metric counts cell-stage
+updates: interior fluid nodes x 18 x steps / wall seconds.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case NAME] [--no-cpu]
+The learned-hybrid path (committed indicator weights) is timed the same way
+on the same workload, and both paths are run to t = 0.6 to report the
+relative L1 density difference of the hybrid against plain WENO5.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case NAME] [--no-cpu] [--no-hybrid]
 
 Prints one JSON line.  The roofline object is for the dominant kernel family
 (the directional sweeps), from HIP events on the solver's stream over the
-timed region; algorithmic bytes are 80 B per cell-stage for the first RK3
-stage (read U_tn, write U_tm) and 120 B for the other two (also U_to).
+timed region.  Algorithmic bytes per cell-stage are those of a stage-by-stage
+sweep (80 B for the first RK3 stage: read U_tn, write U_tm; 120 B for the
+other two, which also read U_to; 106.7 B on average); a fused launch covers
+three cell-stages per cell.
 """
 import argparse
 import json
@@ -22,15 +28,18 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# Measured once with rocprofv3 on this workload (profiles/sweep_shockvortex256.md):
-# HBM bytes per cell-stage of the sweep kernels (FETCH_SIZE x 2.0 + WRITE_SIZE,
-# scale factors from a 1 GiB copy kernel) and FP64 flops per face (FMA = 2).
-TRAFFIC_B_PER_CELL_STAGE = 129.6
+# Measured with rocprofv3 on this workload (profiles/rk_line_shockvortex256.md):
+# HBM bytes per cell-stage of the fused sweep kernel (FETCH_SIZE x 2.0 +
+# WRITE_SIZE, scale factors from a 1 GiB copy kernel; None until profiled) and
+# FP64 flops per cell-stage of the plain WENO5 flux and update (FMA = 2).
+TRAFFIC_B_PER_CELL_STAGE = None
 FP64_FLOP_PER_CELL_STAGE = 1124.0
 METRIC = "Mcell-stage updates/s (256³ WENO5-RK3) + learned-hybrid vs plain at ≤1e-3 L1"
 
@@ -67,6 +76,37 @@ def cpu_baseline(seconds_budget=12.0):
                       f"oracle/artra_oracle.c single thread"}
 
 
+def timed(solver, warmup, steps):
+    solver.march(warmup)
+    solver.synchronize()
+    solver.set_timing(True)
+    solver.hybrid_counts(reset=True)
+    t0 = time.perf_counter()
+    done, _ = solver.march(steps)
+    solver.synchronize()
+    elapsed = time.perf_counter() - t0
+    solver.set_timing(False)
+    return done, elapsed, solver.timing(), solver.hybrid_counts()
+
+
+def run_to_end(case_dir, params=None, threshold=0.0):
+    """Density at the case's end time (t = 0.6) and the wall time of the march."""
+    from artracfd_amd import Case, Solver
+    case = Case(case_dir)
+    solver = Solver(case)
+    if params is not None:
+        solver.set_indicator(params, threshold)
+    solver.synchronize()
+    t0 = time.perf_counter()
+    steps, _ = solver.march()
+    solver.synchronize()
+    el = time.perf_counter() - t0
+    rho = case.interior(solver.field())[0].copy()
+    solver.close()
+    case.close()
+    return rho, steps, el
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,32 +115,38 @@ def main():
     ap.add_argument("--case", default="shockvortex256")
     ap.add_argument("--segment", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--unfused", action="store_true", help="stage-by-stage sweeps instead of the fused line sweeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is a "
                                    "design (see DESIGN.md), so --gpus must be 1"}))
         sys.exit(2)
     from artracfd_amd import Case, Solver
+    case_dir = os.path.join(ROOT, "cases", args.case)
     t_init = time.perf_counter()
-    case = Case(os.path.join(ROOT, "cases", args.case))
+    case = Case(case_dir)
     cells = fluid_nodes(case)
-    solver = Solver(case, segment=args.segment)
+    solver = Solver(case, segment=args.segment, fused=not args.unfused)
+    fused_axes = solver.fused_axes()
     init_s = time.perf_counter() - t_init
     from artracfd_amd.lib import copy_probe
     copy_gbs = copy_probe(1 << 30, 20)
-    solver.march(args.warmup)
-    solver.synchronize()
-    solver.set_timing(True)
-    t0 = time.perf_counter()
-    done, dts = solver.march(args.steps)
-    solver.synchronize()
-    elapsed = time.perf_counter() - t0
-    tm = solver.timing()
+    done, elapsed, tm, _ = timed(solver, args.warmup, args.steps)
+    solver.close()
+    case.close()
     value = cells * 18 * done / elapsed / 1e6
     sweep_ms, sweep_n = tm["sweep"]
-    bytes_per_launch = cells * stage_bytes(1)
+    stages_per_launch = 18.0 * done / max(sweep_n, 1)
+    bytes_per_launch = cells * stage_bytes(1) * stages_per_launch
     avg_launch_s = sweep_ms / 1e3 / max(sweep_n, 1)
     achieved = bytes_per_launch / avg_launch_s / 1e9
+    flops_per_launch = FP64_FLOP_PER_CELL_STAGE * cells * stages_per_launch
+    traffic = None
+    if TRAFFIC_B_PER_CELL_STAGE is not None and args.case == "shockvortex256" and not args.unfused:
+        traffic = round(TRAFFIC_B_PER_CELL_STAGE * cells * stages_per_launch)
+    kernel = ("rk_line (fused directional sweep, all RK stages of one line in LDS)" if fused_axes
+              else "sweep_x/sweep_march (stage-by-stage directional sweeps)")
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -115,24 +161,44 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{args.case}: shock-vortex interaction, {cells} interior fluid nodes, WENO5-LLF-RK3 "
-                               f"split, inviscid, plain FP64 path", "parallelism": "single GPU"},
+                               f"split, inviscid, plain FP64 path", "parallelism": "single GPU",
+                   "fused_axes": fused_axes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(TRAFFIC_B_PER_CELL_STAGE * cells) if args.case == "shockvortex256" else None,
-                     "kernel": "sweep_x/sweep_march (directional WENO5 sweeps)",
-                     "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": sweep_n,
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": kernel, "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": sweep_n,
+                     "cell_stages_per_launch": round(cells * stages_per_launch),
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch),
                      "measured_copy_gbs": round(copy_gbs, 1),
-                     "fp64": {"achieved_tflops": round(FP64_FLOP_PER_CELL_STAGE * cells / avg_launch_s / 1e12, 2),
+                     "fp64": {"achieved_tflops": round(flops_per_launch / avg_launch_s / 1e12, 2),
                               "peak_tflops": FP64_PEAK_TFLOPS,
-                              "frac": round(FP64_FLOP_PER_CELL_STAGE * cells / avg_launch_s / 1e12 / FP64_PEAK_TFLOPS, 4),
+                              "frac": round(flops_per_launch / avg_launch_s / 1e12 / FP64_PEAK_TFLOPS, 4),
                               "flop_per_cell_stage": FP64_FLOP_PER_CELL_STAGE}},
         "kernel_ms": {k: round(v[0], 3) for k, v in tm.items()},
         "kernel_launches": {k: v[1] for k, v in tm.items()},
         "init_seconds": round(init_s, 2),
     }
-    solver.close()
-    case.close()
+    if not args.no_hybrid and not args.unfused:
+        from artracfd_amd import indicator
+        params, meta = indicator.load()
+        thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
+        case = Case(case_dir)
+        solver = Solver(case)
+        solver.set_indicator(params, thr)
+        hdone, helapsed, htm, (lin, seen) = timed(solver, args.warmup, args.steps)
+        solver.close()
+        case.close()
+        hvalue = cells * 18 * hdone / helapsed / 1e6
+        rho_p, steps_p, el_p = run_to_end(case_dir)
+        rho_h, steps_h, el_h = run_to_end(case_dir, params, thr)
+        out["hybrid"] = {
+            "value": round(hvalue, 3), "unit": "Mcell-stage updates/s", "vs_plain": round(hvalue / value, 3),
+            "ms_per_step": round(helapsed * 1e3 / max(hdone, 1), 4),
+            "linear_face_fraction": round(lin / max(seen, 1), 4),
+            "rel_l1_rho_vs_plain_at_end": float(np.abs(rho_h - rho_p).sum() / np.abs(rho_p).sum()),
+            "end_time_steps": {"plain": steps_p, "hybrid": steps_h},
+            "end_time_seconds": {"plain": round(el_p, 3), "hybrid": round(el_h, 3)},
+            "logit_threshold": thr, "kernel_ms": {k: round(v[0], 3) for k, v in htm.items()},
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -104,6 +104,16 @@ int artra_hip_set_arithmetic(ArtraHipSolver *solver, int exact);
  * result unchanged; 0, 0 restores the literal fill after every stage. */
 int artra_hip_set_fill_options(ArtraHipSolver *solver, int use_gather, int partial);
 
+/* Fused directional sweeps.  fused = 1 (default) runs all RK stages of a
+ * split directional operator as one launch per grid line with the line
+ * held in LDS, wherever a stage along that axis reads nothing but its own
+ * line (inviscid, split, no immersed body, between-stage fill copying along
+ * the line, line fits in LDS); the final level TO is identical, the
+ * intermediate levels TN and TM are left as scratch.  *axes receives a bit
+ * mask of the axes that qualify. */
+int artra_hip_set_fused(ArtraHipSolver *solver, int fused);
+int artra_hip_fused_axes(ArtraHipSolver *solver, int *axes);
+
 /* Launch-shape knobs: column segment of the y/z sweeps (0 keeps the default). */
 int artra_hip_set_tuning(ArtraHipSolver *solver, int march_segment);
 
@@ -114,6 +124,84 @@ int artra_hip_copy_probe(long bytes, int iters, double *gbps);
 
 /* Message of the last failed call on this thread. */
 const char *artra_hip_error(void);
+
+/* ------------------------------------------------------------ batched cases */
+/*
+ * Many independent cases advanced together.  Each time step is one device
+ * CFL reduction over every case, one kernel computing every case's dt (with
+ * the time loop's rectification, solve.c:65-78), then the split directional
+ * operators Z Y X X Y Z, each as ONE persistent launch over the lines of
+ * every case (blocks walk a tile list; a tile is 256 / T lines of one case,
+ * T threads per line) followed by every case's boundary fill.  Nothing of a
+ * step needs the host, so a step can be captured once in a HIP graph and
+ * replayed.  No reference counterpart: an addition of this library.
+ *
+ * Cases must be three-dimensional, fluid-only, inviscid, split WENO5 with
+ * the fast arithmetic and share the time scheme and flux model; they run in
+ * lockstep and each must have `steps` steps left.  A case whose end time
+ * falls inside the marched steps stops changing (dt = 0) from then on.
+ */
+typedef struct ArtraHipBatch ArtraHipBatch;
+int artra_hip_batch_create(ArtraHipSolver *const *solvers, Time *const *times, int count, ArtraHipBatch **out);
+int artra_hip_batch_destroy(ArtraHipBatch *batch);
+/* Autotuner: per grid size the threads per line (tile shape), then the
+ * persistent grid size; threads[count] and *grid receive the choice.  The
+ * fields and clocks are restored afterwards. */
+int artra_hip_batch_tune(ArtraHipBatch *batch, int reps, int *threads, int *grid);
+/* Learned indicator for every case (NULL: plain WENO5). */
+int artra_hip_batch_set_indicator(ArtraHipBatch *batch, const float *params, float logit_threshold);
+int artra_hip_batch_counts(ArtraHipBatch *batch, unsigned long long counts[2], int reset);
+/* steps lockstep time steps; use_graph replays one captured step; *ms the
+ * device time of the steps (may be NULL).  Each case's Time advances. */
+int artra_hip_batch_march(ArtraHipBatch *batch, int steps, int use_graph, double *ms);
+
+/* ------------------------------------------- learned troubled-face indicator */
+/*
+ * A 12-32-32-1 MLP (ReLU, BF16 weights and activations on MFMA, FP32
+ * accumulation) reads, per face, the six-node stencils of density and
+ * pressure normalised by their face average, and routes the face either to
+ * full WENO5 or to the linear fifth-order upwind flux.  FP64 is kept for the
+ * flux, the characteristic projection and the update.  No reference
+ * counterpart: an addition of this library.
+ *
+ * Parameters: one FP32 vector of ARTRA_MLP_PARAMS values,
+ *   W1[16][32] (rows 12..15 zero), b1[32], W2[32][32], b2[32], w3[32], b3,
+ * weights stored [input][output].
+ */
+#define ARTRA_MLP_PARAMS 1633
+
+typedef struct {
+    int epochs;          /* passes over the samples */
+    int batch;           /* samples per optimiser step (multiple of 64) */
+    float lr;            /* peak Adam step size, cosine-decayed to zero */
+    float tau;           /* a face is troubled when its label exceeds tau */
+    float wpos;          /* loss weight of troubled samples */
+} ArtraMlpTraining;
+
+/* Faces along an axis of the interior lines, and their 12 features and
+ * WENO5 weight-deviation labels (max over fields and sides of
+ * sum_k |omega_k - C_k|) computed on level TO. */
+int artra_hip_face_count(ArtraHipSolver *solver, int axis, long *count);
+int artra_hip_sample_faces(ArtraHipSolver *solver, int axis, float *features, float *labels);
+
+/* Use the indicator for the WENO5 sweeps: faces whose logit exceeds
+ * logit_threshold get WENO5, the others the linear flux.  params NULL
+ * restores plain WENO5.  The indicator runs inside the fused sweeps, so
+ * every swept axis must qualify for them (see artra_hip_set_fused). */
+int artra_hip_set_indicator(ArtraHipSolver *solver, const float *params, float logit_threshold);
+/* counts[0] faces sent to the linear flux, counts[1] faces evaluated */
+int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[2], int reset);
+
+/* Train in place on the device (Adam, weighted logistic loss of
+ * label > tau); loss_history[epoch] receives the last batch loss per epoch
+ * (may be NULL).  Deterministic for identical inputs. */
+int artra_mlp_train(float *params, const float *features, const float *labels, long n,
+                    const ArtraMlpTraining *options, float *loss_history);
+/* Gradient of the mean weighted loss over n samples (device kernels). */
+int artra_mlp_gradient(const float *params, const float *features, const float *labels, long n, float tau,
+                       float wpos, float *grad, float *loss);
+/* Logits of n feature rows (device kernels). */
+int artra_mlp_logits(const float *params, const float *features, long n, float *logits);
 
 /* ------------------------------------------------ 3. case preparation */
 
@@ -133,6 +221,8 @@ int artra_case_grid(const ArtraCase *c, int n[3], int ng[3], int pin[6]);
 int artra_case_get_field(const ArtraCase *c, int tn, Real *field);
 int artra_case_set_field(ArtraCase *c, int tn, const Real *field);
 int artra_case_get_flags(const ArtraCase *c, int *flags);
+/* current time and step count of the case's clock */
+int artra_case_clock(const ArtraCase *c, double *now, int *step);
 /* Evaluate an initial-condition expression (t = 0); returns 0 on success. */
 int artra_calc_expression(const char *text, Real x, Real y, Real z, Real *ans);
 

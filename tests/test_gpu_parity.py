@@ -135,3 +135,47 @@ def case_interior(name, arr):
     g = golden(name)
     (x0, x1), (y0, y1), (z0, z1) = g["pin"]
     return arr[..., z0:z1, y0:y1, x0:x1]
+
+
+def _variant(tmp_path, name, numerical):
+    """Copy of a case directory with its numerical section replaced."""
+    import shutil
+    d = tmp_path / f"{name}_{''.join(map(str, numerical))}"
+    shutil.copytree(case_dir(name), d)
+    text = open(d / "artracfd.case").read()
+    head, rest = text.split("numerical begin\n", 1)
+    _, tail = rest.split("numerical end\n", 1)
+    body = "".join(f"{v}\n" for v in numerical)
+    open(d / "artracfd.case", "w").write(head + "numerical begin\n" + body + "numerical end\n" + tail)
+    return str(d)
+
+
+FUSED = [("sod", None), ("shockvortex64", None), ("sod", (0, 1, 0, 0, 0, 0, 1)), ("sod", (1, 0, 0, 0, 0, 0, 1)),
+         ("sod", (1, 1, 0, 1, 0, 0, 1)), ("sod", (1, 1, 0, 0, 1, 0, 1)), ("shockvortex64", (0, 0, 0, 1, 1, 0, 1))]
+
+
+@pytest.mark.parametrize("name, numerical", FUSED)
+@pytest.mark.parametrize("arith", ["exact", "fast"])
+def test_fused_line_sweeps_match_stage_by_stage_sweeps(tmp_path, name, numerical, arith):
+    """One launch per directional operator with the line in LDS (rk_line)
+    against the stage-by-stage sweeps with their between-stage fills: the
+    final level must agree bit for bit (RK3/RK2, WENO5/WENO3, mean/Roe,
+    LLF/Steger-Warming)."""
+    print(f"fused {name} {numerical} {arith}", flush=True)
+    from artracfd_amd import Case, Solver
+    d = case_dir(name) if numerical is None else _variant(tmp_path, name, numerical)
+    out = []
+    for fused in (True, False):
+        case = Case(d)
+        solver = Solver(case, exact=(arith == "exact"), fused=fused)
+        axes = solver.fused_axes()
+        if fused:
+            assert axes == [0, 1, 2], axes
+        else:
+            assert axes == []
+        done, dts = solver.march(5)
+        out.append((case.interior(solver.field()), dts))
+        solver.close()
+        case.close()
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][0], out[1][0])
