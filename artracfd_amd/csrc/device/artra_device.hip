@@ -578,7 +578,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const bool troubled = valid && z > hy.threshold;
                 artra::FaceState fs;
                 artra::face_state<S>(mdl, UL, UR, fs);
-                artra::linear_flux_from<S>(fs, Up, Um, Fx);
+                if (mdl.sw) artra::linear_flux_from<S>(fs, Up, Um, Fx);
+                else artra::linear_flux_llf<S>(mdl.gamma, fs, Up, Um, Fx);
                 const unsigned long long mask = __ballot(troubled);
                 const unsigned long long seen = __ballot(valid && write);
                 int slot = 0;

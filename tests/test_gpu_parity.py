@@ -9,11 +9,12 @@ steps within 1e-13 relative.  Inviscid runs are expected bit-identical; the
 viscous flux calls pow(), whose device and glibc results may differ in the
 last bit, so viscous runs agree to rounding level instead."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
 
-from conftest import case_dir, golden, rel_l2
+from conftest import GOLDEN, case_dir, golden, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -179,3 +180,36 @@ def test_fused_line_sweeps_match_stage_by_stage_sweeps(tmp_path, name, numerical
         case.close()
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][0], out[1][0])
+
+
+FULLSIZE = [n for n in ("tgv128", "cylinder") if os.path.exists(os.path.join(GOLDEN, f"{n}_full.npz"))]
+
+
+@pytest.mark.parametrize("name", FULLSIZE)
+def test_fullsize_run_matches_reference(name):
+    """The full-size runs (TGV 128^3 viscous, 200 steps; shock-cylinder
+    512x512x4 with the immersed boundary, 1000 steps) against fixtures of the
+    compiled reference's own run: the dt sequence, per-component sums and
+    sums of squares over all interior fluid nodes, and a strided subsample
+    of the field (relative L2 per variable group <= 1e-10)."""
+    print(f"full size {name}", flush=True)
+    from artracfd_amd import Case, Solver
+    g = golden(name + "_full")
+    case = Case(case_dir(name))
+    solver = Solver(case)
+    done, dts = solver.march(int(g["steps"]))
+    assert done == int(g["steps"])
+    np.testing.assert_allclose(dts, g["dt"], rtol=1e-11, atol=0)
+    U = case.interior(solver.field())
+    fluid = case.interior(case.flags()[0]) == 0
+    for c in range(5):
+        s, q = U[c][fluid].sum(), (U[c][fluid] ** 2).sum()
+        assert abs(s - g["sums"][c]) <= 1e-11 * max(abs(g["sums"][c]), np.sqrt(g["squares"][c])), c
+        assert abs(q - g["squares"][c]) <= 1e-11 * g["squares"][c], c
+    st = int(g["stride"])
+    sub = U[:, ::st, ::st, ::st]
+    err = rel_l2(sub, g["sub"])
+    print(name, "subsample rel L2", err)
+    assert max(err.values()) <= 1e-10
+    solver.close()
+    case.close()
