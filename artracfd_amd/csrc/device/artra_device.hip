@@ -485,6 +485,8 @@ struct LineHybrid {
     const float *bias;              // b1[32], b2[32], w3[32]
     float b3, threshold;
     unsigned long long *counts;     // [0] faces on the linear flux, [1] faces evaluated
+    int everyStage;                 // 1: classify at every RK stage; 0: once per directional operator
+    float smoothVariation;          // faces varying less than this (relative) skip the network; 0: none
 };
 
 // Blocks are dealt to the XCDs round robin; this bijection gives each XCD a
@@ -497,11 +499,37 @@ __device__ __forceinline__ long XcdLine(long b, long nb)
     return (x < rem) ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
 }
 
-// LDS of one line: the states [5][n], the face fluxes [5][F] and (HYB) the
-// queue of troubled faces [F] with its counter.
+// Troubled faces a line can queue: more than this and the line runs plain
+// WENO5 for the operator (nothing is gained by routing then).
+__host__ __device__ __forceinline__ int LineQueueCap(int F)
+{
+    return F / 4 > 16 ? F / 4 : 16;
+}
+
+// LDS of one line: the states [5][n] and face fluxes [5][F]; for the
+// indicator also the velocities and total enthalpy of every node [4][n],
+// the half-field WENO5 results of the queued faces [10][cap], the queue
+// [cap], each face's queue position [F], the queue length and the FP32
+// density and pressure of every node [2][n].
 __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F)
 {
-    return ((size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)(F + 2) * sizeof(int) + 15) / 16 * 16;
+    const int cap = LineQueueCap(F);
+    return ((size_t)(9 * n + 5 * F + 10 * cap) * sizeof(double) + (size_t)(cap + F + 4) * sizeof(int) +
+            (size_t)2 * n * sizeof(float) + 15) / 16 * 16;
+}
+
+// Node quantities the routed faces use: velocities and total enthalpy (FP64)
+// and, when the indicator will classify, density and pressure (FP32).
+__device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const double *st, int n, int p, double *nodeV,
+                                                   float *nodeRP, bool features)
+{
+    double u[5], nv[4];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) u[c] = st[c * n + p];
+    if (features) artra::node_rho_p(mdl.gamma, u, nodeRP[p], nodeRP[n + p]);
+    artra::node_velocity_enthalpy(mdl.gamma, u, nv);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) nodeV[c * n + p] = nv[c];
 }
 
 // All stages of the directional operator on one line, run by the T threads
@@ -509,22 +537,36 @@ __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F)
 // call it with the same loop trip counts (same n, lo, hi, stages), because
 // the stages are separated by block barriers.  With write = false the line
 // is computed but not stored (padding slots of a batched block).
+//
+// With HYB, the indicator classifies the faces once per operator (or at
+// every stage) into a queue of troubled faces; each stage then runs one
+// balanced pass over F + 10 x queued work items - the linear flux of every
+// routed face and the ten half-field WENO5 evaluations of every queued
+// face - a short pass projecting the queued faces' half-fields back, and
+// the node update, which also refreshes the node quantities of the next
+// stage.  A line with more troubled faces than the queue holds runs WENO5
+// on every face for that operator.
 template <int S, int NS, bool FAST, bool HYB, int K>
 __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &mdl, const LineHybrid &hy, long line,
                                          int tt, int T, char *mem, bool write, unsigned long long *tally,
                                          const float *bias, const artra::MlpWave &frags)
 {
     constexpr int SL = (NS == 6) ? 2 : 1;           // stencil nodes left of the face's left node
-    const int n = la.n, F = la.hi - la.lo + 1;
+    const int n = la.n, F = la.hi - la.lo + 1, cap = LineQueueCap(F);
     double *st = (double *)mem;                     // [5][n]
     double *fb = st + 5 * n;                        // [5][F]
-    int *queue = (int *)(fb + 5 * F);               // [F]
-    int *queued = queue + F;
+    double *nodeV = fb + 5 * F;                     // [4][n]
+    double *hbuf = nodeV + 4 * n;                   // [cap][10]
+    int *queue = (int *)(hbuf + 10 * cap);          // [cap]
+    int *fidx = queue + cap;                        // [F]
+    int *queued = fidx + F;
+    float *nodeRP = (float *)(queued + 4);          // [2][n]
     const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
     for (int p = tt; p < n; p += T) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) st[c * n + p] = la.U[c * la.N + base + (long)p * la.stride];
+        if (HYB) LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
     }
     bool own[K], fl[K];
 #pragma unroll
@@ -533,6 +575,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         own[k] = i < la.hi;
         fl[k] = own[k] && la.fluid[base + (long)i * la.stride];
     }
+    if (HYB && tt == 0) *queued = 0;
     __syncthreads();
     double U0[K][5];
 #pragma unroll
@@ -544,89 +587,189 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     const int M = (F + T - 1) / T;
     const double r = la.dt ? *la.dt * la.dd : la.r;
     if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
+    bool routed = false;                            // this operator routes faces (HYB, queue within cap)
+    int nq = 0;
     for (int s = 0; s < la.stages; ++s) {
-        if (HYB) {
-            if (tt == 0) *queued = 0;
-            __syncthreads();
-        }
-        // face fluxes; the loop trip count is the same for every thread
-        for (int m = 0; m < M; ++m) {
-            const int q = tt + T * m;
-            const bool valid = q < F;
-            const int f = la.lo - 1 + (valid ? q : F - 1);   // face between f and f + 1
-            double Fx[5];
-            if (HYB) {
-                // the stencil is reduced to the face pair and the two linear
-                // combinations before the network runs, to keep registers low
-                float x[artra::kFeat];
-                double UL[5], UR[5], Up[5], Um[5];
-                {
-                    double w[6][5];
-#pragma unroll
-                    for (int j = 0; j < 6; ++j)
-#pragma unroll
-                        for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
-                    artra::face_features<6>(mdl.gamma, w, x);
-                    artra::linear_averages(w, Up, Um);
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        UL[c] = w[2][c];
-                        UR[c] = w[3][c];
-                    }
+        const bool classify = HYB && (s == 0 || hy.everyStage);
+        if (classify) {
+            // the indicator runs at the first stage (or every stage); later
+            // stages reuse its queue of troubled faces.  The loop trip count
+            // is the same for every thread (whole wavefronts run the network).
+            for (int m = 0; m < M; ++m) {
+                const int q = tt + T * m;
+                const bool valid = q < F;
+                const int f = la.lo - 1 + (valid ? q : F - 1);
+                // waves whose faces are all near-uniform skip the network
+                const bool smooth = hy.smoothVariation > 0.0f &&
+                                    artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
+                bool troubled = false;
+                if (__ballot(valid && !smooth)) {
+                    float x[artra::kFeat];
+                    artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
+                    const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
+                    troubled = valid && !smooth && z > hy.threshold;
                 }
-                const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
-                const bool troubled = valid && z > hy.threshold;
-                artra::FaceState fs;
-                artra::face_state<S>(mdl, UL, UR, fs);
-                if (mdl.sw) artra::linear_flux_from<S>(fs, Up, Um, Fx);
-                else artra::linear_flux_llf<S>(mdl.gamma, fs, Up, Um, Fx);
                 const unsigned long long mask = __ballot(troubled);
                 const unsigned long long seen = __ballot(valid && write);
                 int slot = 0;
                 if (__lane_id() == 0) {
                     if (mask) slot = atomicAdd(queued, __popcll(mask));
-                    const unsigned long long routed = seen & ~mask;
-                    atomicAdd(&tally[0], (unsigned long long)__popcll(routed));
+                    atomicAdd(&tally[0], (unsigned long long)__popcll(seen & ~mask));
                     atomicAdd(&tally[1], (unsigned long long)__popcll(seen));
                 }
                 slot = __shfl(slot, 0);
-                if (troubled) {
-                    const unsigned long long below = mask & ((1ull << __lane_id()) - 1ull);
-                    queue[slot + __popcll(below)] = q;
-                }
-            } else {
-                double w[NS][5];
+                const int pos = slot + __popcll(mask & ((1ull << __lane_id()) - 1ull));
+                if (troubled && pos < cap) queue[pos] = q;
+                if (valid) fidx[q] = troubled ? pos : -1;
+            }
+            __syncthreads();
+            nq = *queued;
+            // every line of a block takes the same path, so the barriers below match
+            routed = __syncthreads_and(nq <= cap) != 0;
+            if (tt == 0) *queued = 0;       // ready for the next classification
+        }
+        if (!HYB) {
+            for (int m = 0; m < M; ++m) {
+                const int q = tt + T * m;
+                if (q >= F) break;
+                const int f = la.lo - 1 + q;   // face between f and f + 1
+                double w[NS][5], Fx[5];
 #pragma unroll
                 for (int j = 0; j < NS; ++j)
 #pragma unroll
                     for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - SL + j];
                 ConvFlux<S, NS, FAST>(mdl, w, Fx);
-            }
-            if (valid) {
 #pragma unroll
                 for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+            }
+        } else if (!routed) {
+            // too many troubled faces on this line: WENO5 on every face, in
+            // chunks of the queue's size through the same half-field items
+            for (int c0 = 0; c0 < F; c0 += cap) {
+                const int cnt = (F - c0 < cap) ? F - c0 : cap;
+                for (int e = tt; e < 10 * cnt; e += T) {
+                    const int qi = e / 10, k = e - 10 * qi;
+                    const int f = la.lo - 1 + c0 + qi;
+                    double UL[5], UR[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        UL[c] = st[c * n + f];
+                        UR[c] = st[c * n + f + 1];
+                    }
+                    artra::FaceState fs;
+                    artra::face_state<S>(mdl, UL, UR, fs);
+                    artra::face_scales(fs);
+                    hbuf[e] = artra::weno_half<S>(fs, st, n, f, k >> 1, k & 1, mdl.sw != 0);
+                }
+                __syncthreads();
+                for (int e = tt; e < cnt; e += T) {
+                    const int q = c0 + e;
+                    const int f = la.lo - 1 + q;
+                    double UL[5], UR[5], h[5], Fx[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        UL[c] = st[c * n + f];
+                        UR[c] = st[c * n + f + 1];
+                        h[c] = hbuf[10 * e + 2 * c] + hbuf[10 * e + 2 * c + 1];
+                    }
+                    artra::FaceState fs;
+                    artra::face_state<S>(mdl, UL, UR, fs);
+                    artra::project_back<S>(fs, h, Fx);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                }
+                __syncthreads();
+            }
+        } else {
+            // one pass over the linear fluxes of the routed faces and the
+            // half-field WENO5 items of the queued ones
+            const int total = F + 10 * nq;
+            for (int e = tt; e < total; e += T) {
+                if (e < F) {
+                    const int q = e;
+                    if (fidx[q] >= 0) continue;
+                    const int f = la.lo - 1 + q;
+                    double Sg[5], Dl[5], Fx[5];
+                    {
+                        double w[6][5];
+#pragma unroll
+                        for (int j = 0; j < 6; ++j)
+#pragma unroll
+                            for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
+                        artra::linear_sum_diff(w, Sg, Dl);
+                    }
+                    artra::FaceState fs;
+                    if (!mdl.roe && !mdl.sw) {
+                        double nL[4], nR[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            nL[c] = nodeV[c * n + f];
+                            nR[c] = nodeV[c * n + f + 1];
+                        }
+                        artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
+                        artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+                    } else {
+                        double UL[5], UR[5];
+#pragma unroll
+                        for (int c = 0; c < 5; ++c) {
+                            UL[c] = st[c * n + f];
+                            UR[c] = st[c * n + f + 1];
+                        }
+                        artra::face_state<S>(mdl, UL, UR, fs);
+                        if (mdl.sw) {
+                            double Up[5], Um[5];
+#pragma unroll
+                            for (int c = 0; c < 5; ++c) {
+                                Up[c] = 0.5 * (Sg[c] + Dl[c]);
+                                Um[c] = 0.5 * (Sg[c] - Dl[c]);
+                            }
+                            artra::linear_flux_from<S>(fs, Up, Um, Fx);
+                        } else {
+                            artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+                        }
+                    }
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                } else {
+                    const int it = e - F, qi = it / 10, k = it - 10 * qi;
+                    const int f = la.lo - 1 + queue[qi];
+                    double UL[5], UR[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        UL[c] = st[c * n + f];
+                        UR[c] = st[c * n + f + 1];
+                    }
+                    artra::FaceState fs;
+                    artra::face_state<S>(mdl, UL, UR, fs);
+                    artra::face_scales(fs);
+                    hbuf[it] = artra::weno_half<S>(fs, st, n, f, k >> 1, k & 1, mdl.sw != 0);
+                }
+            }
+            {
+                __syncthreads();
+                // project the queued faces' half-fields back to fluxes
+                for (int e = tt; e < nq; e += T) {
+                    const int q = queue[e];
+                    const int f = la.lo - 1 + q;
+                    double UL[5], UR[5], h[5], Fx[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        UL[c] = st[c * n + f];
+                        UR[c] = st[c * n + f + 1];
+                        h[c] = hbuf[10 * e + 2 * c] + hbuf[10 * e + 2 * c + 1];
+                    }
+                    artra::FaceState fs;
+                    artra::face_state<S>(mdl, UL, UR, fs);
+                    artra::project_back<S>(fs, h, Fx);
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                }
             }
         }
         __syncthreads();
-        if (HYB) {
-            const int nq = *queued;
-            for (int e = tt; e < nq; e += T) {
-                const int q = queue[e];
-                const int f = la.lo - 1 + q;
-                double w[6][5];
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
-                double Fx[5];
-                ConvFlux<S, 6, true>(mdl, w, Fx);
-#pragma unroll
-                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
-            }
-            __syncthreads();
-        }
         // node updates (fluid_dynamics.c:243-274); each thread reads and writes only its own nodes
         const double a = la.a[s], b = la.b[s];
+        const bool refresh = HYB && s + 1 < la.stages;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (!fl[k]) continue;
@@ -640,6 +783,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const double Uo = (s == 0) ? Un : U0[k][c];
                 st[c * n + i] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
             }
+            if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, hy.everyStage != 0);
         }
         __syncthreads();
         if (s + 1 < la.stages) {
@@ -655,6 +799,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c) st[c * n + d] = v[c];
+                if (HYB) LineNodeQuantities(mdl, st, n, d, nodeV, nodeRP, hy.everyStage != 0);
             }
             __syncthreads();
         }
@@ -671,7 +816,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 
 // One line per block.
 template <int S, int NS, bool FAST, bool HYB, int K>
-__global__ __launch_bounds__(256) void rk_line(LineArgs la, FluxModel mdl, LineHybrid hy)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : 2))) void rk_line(LineArgs la, FluxModel mdl,
+                                                                                        LineHybrid hy)
 {
     extern __shared__ double lds[];
     __shared__ unsigned long long tally[2];
@@ -703,8 +849,10 @@ struct BatchTile {
 };
 
 template <int S, int NS, bool HYB>
-__global__ __launch_bounds__(256) void rk_batch(const BatchTile *tiles, int ntiles, const LineArgs *args,
-                                                FluxModel mdl, LineHybrid hy)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void rk_batch(const BatchTile *tiles,
+                                                                                         int ntiles,
+                                                                                         const LineArgs *args,
+                                                                                         FluxModel mdl, LineHybrid hy)
 {
     extern __shared__ double lds[];
     __shared__ unsigned long long tally[2];
@@ -1022,6 +1170,8 @@ struct ArtraDevice {
     float *mlpParams = nullptr;     // FP32 biases/output vector (device)
     uint16_t *mlpW1 = nullptr, *mlpW2 = nullptr;
     float mlpThreshold = 0.0f;
+    int everyStage = 0;             // indicator at every RK stage instead of once per operator
+    float smoothVariation = 1.0e-3f; // relative stencil variation below which the network is skipped
     float mlpB3 = 0.0f;
     unsigned long long *hybCounts = nullptr;
     float *mlpBias = nullptr;       // b1, b2, w3 contiguous (96 FP32)
@@ -1313,6 +1463,8 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
         hy.b3 = dev->mlpB3;
         hy.threshold = dev->mlpThreshold;
         hy.counts = dev->hybCounts;
+        hy.everyStage = dev->everyStage;
+        hy.smoothVariation = dev->smoothVariation;
     }
     int slot;
     if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
@@ -1791,6 +1943,14 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
     return 0;
 }
 
+int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation)
+{
+    if (!(smooth_variation >= 0.0f)) return Fail("the smooth-variation gate must be non-negative");
+    dev->everyStage = (every_stage != 0);
+    dev->smoothVariation = smooth_variation;
+    return 0;
+}
+
 int artra_dev_set_fused(ArtraDevice *dev, int fused)
 {
     if (!fused && dev->hybrid) return Fail("the learned indicator needs the fused line sweeps");
@@ -2219,6 +2379,8 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
     b->hy.b3 = params[ARTRA_MLP_B3];
     b->hy.threshold = threshold;
     b->hy.counts = b->counts;
+    b->hy.everyStage = 0;
+    b->hy.smoothVariation = 1.0e-3f;
     b->hybrid = 1;
     return 0;
 }

@@ -49,21 +49,21 @@ __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
     return v.f;
 }
 
-// Face features from the FP64 stencil states (conservative).
-template <int NS>
-__host__ __device__ __forceinline__ void face_features(double gamma, const double st[NS][5], float x[kFeat])
+// FP32 density and pressure of one node (conservative state in FP64).
+__device__ __forceinline__ void node_rho_p(double gamma, const double U[5], float &rho, float &p)
 {
-    float rho[6], p[6];
     const float g1 = (float)(gamma - 1.0);
-#pragma unroll
-    for (int m = 0; m < 6; ++m) {
-        const int q = (NS == 6) ? m : (m == 0 ? 0 : (m == 5 ? NS - 1 : m - 1));
-        const float r = (float)st[q][0];
-        const float mx = (float)st[q][1], my = (float)st[q][2], mz = (float)st[q][3];
-        const float inv = 1.0f / r;
-        rho[m] = r;
-        p[m] = g1 * ((float)st[q][4] - 0.5f * (mx * mx + my * my + mz * mz) * inv);
-    }
+    const float r = (float)U[0];
+    const float mx = (float)U[1], my = (float)U[2], mz = (float)U[3];
+    const float inv = 1.0f / r;
+    rho = r;
+    p = g1 * ((float)U[4] - 0.5f * (mx * mx + my * my + mz * mz) * inv);
+}
+
+// Features of a face from the six nodes' density and pressure (the
+// normalising reciprocal is the hardware one, within 1 ulp).
+__device__ __forceinline__ void features_from(const float rho[6], const float p[6], float x[kFeat])
+{
     const float *src[2] = {rho, p};
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
@@ -72,10 +72,40 @@ __host__ __device__ __forceinline__ void face_features(double gamma, const doubl
         float dev = 0.0f;
 #pragma unroll
         for (int m = 0; m < 6; ++m) dev = fmaxf(dev, fabsf(q[m] - c));
-        const float inv = 1.0f / (dev + 0.01f * fabsf(c) + 1.0e-30f);
+        const float inv = __builtin_amdgcn_rcpf(dev + 0.01f * fabsf(c) + 1.0e-30f);
 #pragma unroll
         for (int m = 0; m < 6; ++m) x[6 * v + m] = (q[m] - c) * inv;
     }
+}
+
+// Whether the six-node density and pressure stencils of a face vary by at
+// most `variation` of their face averages; such faces can be routed to the
+// linear flux without consulting the network (at 1e-3 their features are
+// all below 0.1 in magnitude, where WENO5 and the linear flux differ by far
+// less than the routing tolerance).
+__device__ __forceinline__ bool near_uniform(const float rho[6], const float p[6], float variation)
+{
+    const float cr = 0.5f * (rho[2] + rho[3]), cp = 0.5f * (p[2] + p[3]);
+    float dr = 0.0f, dp = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        dr = fmaxf(dr, fabsf(rho[m] - cr));
+        dp = fmaxf(dp, fabsf(p[m] - cp));
+    }
+    return dr <= variation * fabsf(cr) && dp <= variation * fabsf(cp);
+}
+
+// Face features from the FP64 stencil states (conservative).
+template <int NS>
+__device__ __forceinline__ void face_features(double gamma, const double st[NS][5], float x[kFeat])
+{
+    float rho[6], p[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int q = (NS == 6) ? m : (m == 0 ? 0 : (m == 5 ? NS - 1 : m - 1));
+        node_rho_p(gamma, st[q], rho[m], p[m]);
+    }
+    features_from(rho, p, x);
 }
 
 // sum_k |omega_k - C_k| of one WENO5 evaluation (same weights as fast::weno5).
@@ -96,9 +126,18 @@ __host__ __device__ __forceinline__ double weno5_weight_deviation(double f0, dou
 
 // Face state shared by the WENO and linear fluxes (fast arithmetic).
 struct FaceState {
-    double u, v, w, hT, c, q, b, d, un;
+    double u, v, w, hT, c, q, b, d, un, g1;
     double lp[5], ln[5];
 };
+
+// b = (gamma - 1) / (2 c^2) and d = 1 / (2 c) of the eigenvectors; only the
+// characteristic projections need them.
+__host__ __device__ __forceinline__ void face_scales(FaceState &fs)
+{
+    const double rc = 1.0 / fs.c;
+    fs.d = 0.5 * rc;
+    fs.b = 0.5 * fs.g1 * rc * rc;
+}
 
 template <int S>
 __host__ __device__ __forceinline__ void face_state(const FluxModel &mdl, const double UL[5], const double UR[5], FaceState &fs)
@@ -124,9 +163,7 @@ __host__ __device__ __forceinline__ void face_state(const FluxModel &mdl, const 
     }
     fs.q = 0.5 * fma(fs.u, fs.u, fma(fs.v, fs.v, fs.w * fs.w));
     fs.c = sqrt(g1 * (fs.hT - fs.q));
-    const double rc = 1.0 / fs.c;
-    fs.d = 0.5 * rc;
-    fs.b = 0.5 * g1 * rc * rc;
+    fs.g1 = g1;
     fs.un = (S == 0) ? fs.u : (S == 1) ? fs.v : fs.w;
     const double lam[5] = {fs.un - fs.c, fs.un, fs.un, fs.un, fs.un + fs.c};
     if (mdl.sw) {
@@ -197,14 +234,54 @@ __host__ __device__ __forceinline__ void linear_averages(const double st[6][5], 
 }
 
 template <int S>
-__host__ __device__ __forceinline__ void linear_flux_from(const FaceState &fs, const double Up[5], const double Un[5], double F[5])
+__host__ __device__ __forceinline__ void linear_flux_from(FaceState fs, const double Up[5], const double Un[5], double F[5])
 {
+    face_scales(fs);
     double Wp[5], Wn[5], h[5];
     characteristic<S>(fs, Up, Wp);
     characteristic<S>(fs, Un, Wn);
 #pragma unroll
     for (int r = 0; r < 5; ++r) h[r] = fma(fs.lp[r], Wp[r], fs.ln[r] * Wn[r]);
     project_back<S>(fs, h, F);
+}
+
+// Face state of the arithmetic mean from per-node velocities and total
+// enthalpies (precomputed once per node); the sound speed uses the hardware
+// square root, which is ample for the linear flux's dissipation coefficient.
+template <int S>
+__device__ __forceinline__ void face_state_mean(double gamma, const double nL[4], const double nR[4], FaceState &fs)
+{
+    fs.u = 0.5 * (nL[0] + nR[0]);
+    fs.v = 0.5 * (nL[1] + nR[1]);
+    fs.w = 0.5 * (nL[2] + nR[2]);
+    fs.hT = 0.5 * (nL[3] + nR[3]);
+    fs.q = 0.5 * fma(fs.u, fs.u, fma(fs.v, fs.v, fs.w * fs.w));
+    fs.g1 = gamma - 1.0;
+    fs.c = __builtin_amdgcn_sqrt(fs.g1 * (fs.hT - fs.q));
+    fs.un = (S == 0) ? fs.u : (S == 1) ? fs.v : fs.w;
+}
+
+// Velocities and total enthalpy of one node (as face_state computes them per side).
+__device__ __forceinline__ void node_velocity_enthalpy(double gamma, const double U[5], double nv[4])
+{
+    const double r = 1.0 / U[0];
+    nv[0] = U[1] * r;
+    nv[1] = U[2] * r;
+    nv[2] = U[3] * r;
+    nv[3] = fma(gamma, U[4] * r, -0.5 * (gamma - 1.0) * fma(nv[0], nv[0], fma(nv[1], nv[1], nv[2] * nv[2])));
+}
+
+// Sum and difference of the two linear combinations, sum c+ U + sum c- U and
+// sum c+ U - sum c- U, from the six-node stencil in symmetric form.
+__host__ __device__ __forceinline__ void linear_sum_diff(const double st[6][5], double Sg[5], double Dl[5])
+{
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        const double a05 = st[0][c] + st[5][c], a14 = st[1][c] + st[4][c], a23 = st[2][c] + st[3][c];
+        const double d05 = st[0][c] - st[5][c], d14 = st[1][c] - st[4][c], d23 = st[2][c] - st[3][c];
+        Sg[c] = fma(2.0, a05, fma(-16.0, a14, 74.0 * a23)) * (1.0 / 60.0);
+        Dl[c] = fma(2.0, d05, fma(-10.0, d14, 20.0 * d23)) * (1.0 / 60.0);
+    }
 }
 
 // The same linear flux for the local Lax-Friedrichs split, where
@@ -216,15 +293,9 @@ __host__ __device__ __forceinline__ void linear_flux_from(const FaceState &fs, c
 //   (A V)_4 = H (V_{1+s} - u_s V_0) + u_s (P + V_4),
 //   P = (gamma - 1) (q V_0 - u V_1 - v V_2 - w V_3 + V_4).
 template <int S>
-__host__ __device__ __forceinline__ void linear_flux_llf(double gamma, const FaceState &fs, const double Up[5], const double Um[5],
-                                                double F[5])
+__host__ __device__ __forceinline__ void linear_flux_llf_sd(double gamma, const FaceState &fs, const double Sg[5],
+                                                   const double Dl[5], double F[5])
 {
-    double Sg[5], Dl[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        Sg[c] = Up[c] + Um[c];
-        Dl[c] = Up[c] - Um[c];
-    }
     const double vel[3] = {fs.u, fs.v, fs.w};
     const double un = fs.un, m = Sg[1 + S];
     const double P = (gamma - 1.0) * (fma(fs.q, Sg[0], Sg[4]) - fma(fs.u, Sg[1], fma(fs.v, Sg[2], fs.w * Sg[3])));
@@ -242,6 +313,19 @@ __host__ __device__ __forceinline__ void linear_flux_llf(double gamma, const Fac
 }
 
 template <int S>
+__host__ __device__ __forceinline__ void linear_flux_llf(double gamma, const FaceState &fs, const double Up[5], const double Um[5],
+                                                double F[5])
+{
+    double Sg[5], Dl[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        Sg[c] = Up[c] + Um[c];
+        Dl[c] = Up[c] - Um[c];
+    }
+    linear_flux_llf_sd<S>(gamma, fs, Sg, Dl, F);
+}
+
+template <int S>
 __host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const double st[6][5], double F[5])
 {
     double Up[5], Un[5];
@@ -249,11 +333,57 @@ __host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const 
     linear_flux_from<S>(fs, Up, Un, F);
 }
 
+// One half of one characteristic field of the WENO5 face flux: side 0 gives
+// WENO5(lp W_r) over the nodes f-2 .. f+2, side 1 WENO5(ln W_r) over the
+// mirrored nodes f+3 .. f-1, for a line held as [5][n] in memory; h_r is
+// their sum.  Ten independent items per face keep a short list of troubled
+// faces busy on many lanes; the operations are those of
+// fast::convective_flux, so h_r is bit-identical to it (fs needs
+// face_scales applied).
+template <int S>
+__device__ __forceinline__ double weno_half(const FaceState &fs, const double *st, int n, int f, int r, int side,
+                                            bool sw)
+{
+    const double vel[3] = {fs.u, fs.v, fs.w};
+    const double m2b = -2.0 * fs.b;
+    const bool edge = (r == 0) || (r == 4);
+    const double sgn = (r == 0) ? 1.0 : -1.0;
+    const int t = edge ? 0 : r - 1;
+    const double cK = (t == S) ? m2b : m2b * ((t == 0) ? vel[0] : (t == 1) ? vel[1] : vel[2]);
+    const int idx = (t == S) ? 0 : 1 + t;
+    const double lam = (r == 0) ? fs.un - fs.c : (r == 4) ? fs.un + fs.c : fs.un;
+    double lp, ln;
+    if (sw) {
+        const double root = sqrt(fma(lam, lam, 1.0e-6));
+        lp = 0.5 * (lam + root);
+        ln = 0.5 * (lam - root);
+    } else {
+        const double star = fabs(fs.un) + fs.c;
+        lp = 0.5 * (lam + star);
+        ln = 0.5 * (lam - star);
+    }
+    const double sc = side ? ln : lp;
+    double g[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const int node = side ? f + 3 - m : f - 2 + m;
+        const double U0 = st[node], U1 = st[n + node], U2 = st[2 * n + node], U3 = st[3 * n + node],
+                     U4 = st[4 * n + node];
+        const double K = fma(fs.q, U0, U4) - fma(fs.u, U1, fma(fs.v, U2, fs.w * U3));
+        const double M = fma(fs.un, U0, -st[(1 + S) * n + node]);
+        const double bK = fs.b * K, dM = fs.d * M;
+        const double Ui = st[idx * n + node];
+        g[m] = sc * (edge ? bK + sgn * dM : fma(cK, K, Ui));
+    }
+    return fast::weno5(g[0], g[1], g[2], g[3], g[4]);
+}
+
 // Full WENO5 flux from the same face state (matches fast::convective_flux).
 // With label != nullptr also returns the weight-deviation label.
 template <int S>
-__host__ __device__ __forceinline__ void weno_flux(const FaceState &fs, const double st[6][5], double F[5], double *label)
+__host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st[6][5], double F[5], double *label)
 {
+    face_scales(fs);
     double W[5][6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
@@ -316,44 +446,64 @@ __device__ __forceinline__ void mlp_fragments(const MlpDev &net, MlpFrags &f)
 }
 
 // Inference on one wavefront (64 faces, lane l holds face l), computed
-// transposed: H^T = W^T X^T, so the MFMA result layout of layer 1 (lane
-// (j, half) holds hidden units 8g + 4 half + (0..3), g = 0..3, of sample j)
-// is exactly the B-operand layout layer 2 needs; no LDS round trip.  The two
-// 32-sample column blocks are faces 0..31 and 32..63.
-//
-// A fragments: lane (h, half) holds W^T[h][8 ks + 4 half + t] = W[8 ks + 4 half + t][h].
+// transposed, H^T = W^T X^T, with v_mfma_f32_32x32x16_bf16 (K = 16 per
+// instruction): lane (j, half) of a 32x32 result holds the hidden units
+// 8 g + 4 half + r (g, r = 0..3) of sample j.  Layer 2 consumes them in
+// place: its K index is permuted so that K-slice ks of lane (j, half) is
+// exactly the accumulator registers 8 ks .. 8 ks + 7, i.e. hidden units
+// 16 ks + 8 (t / 4) + 4 half + t % 4 (t = 0..7); the W2 fragments use the
+// same permutation.  The two 32-sample column blocks are faces 0..31 and
+// 32..63; 6 MFMAs per 64 faces, activations never leave registers.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 struct MlpWave {
-    bf16x4 w1t[2];
-    bf16x4 w2t[4];
+    bf16x8 w1t;        // A[h][8 half + t] = W1[8 half + t][h]
+    bf16x8 w2t[2];     // A[h][K(ks, half, t)] = W2[16 ks + 8 (t / 4) + 4 half + t % 4][h]
 };
+
+__device__ __forceinline__ __bf16 bf16_of(uint16_t bits)
+{
+    return __builtin_bit_cast(__bf16, bits);
+}
 
 __device__ __forceinline__ void mlp_wave_fragments(const uint16_t *W1, const uint16_t *W2, MlpWave &f)
 {
     const int l = __lane_id();
-    const int h = l & 31, kh = 4 * (l >> 5);
+    const int h = l & 31, half = l >> 5;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) f.w1t[t] = bf16_of(W1[(8 * half + t) * kHidden + h]);
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) f.w1t[ks][t] = (short)W1[(8 * ks + kh + t) * kHidden + h];
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) f.w2t[ks][t] = (short)W2[(8 * ks + kh + t) * kHidden + h];
+        for (int t = 0; t < 8; ++t) {
+            const int unit = 16 * ks + 8 * (t >> 2) + 4 * half + (t & 3);
+            f.w2t[ks][t] = bf16_of(W2[unit * kHidden + h]);
+        }
 }
 
+// max(x, 0) as a signed-integer maximum of the bit pattern: non-negative
+// floats order like their patterns, negative ones have negative patterns.
+// One instruction, where the float maximum of an MFMA result costs two
+// (the compiler first quiets a possible signalling NaN).
+__device__ __forceinline__ float relu(float x)
+{
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int, x), 0));
+}
+
+// Two FP32 values to packed BF16 (round to nearest even, v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint32_t pack_bf16(float a, float b)
 {
-    return (uint32_t)f32_to_bf16(a) | ((uint32_t)f32_to_bf16(b) << 16);
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
 
-__device__ __forceinline__ bf16x4 frag_of(uint32_t lo, uint32_t hi)
+__device__ __forceinline__ bf16x8 frag8(uint32_t a, uint32_t b, uint32_t c, uint32_t d)
 {
-    bf16x4 v;
-    v[0] = (short)(lo & 0xffffu);
-    v[1] = (short)(lo >> 16);
-    v[2] = (short)(hi & 0xffffu);
-    v[3] = (short)(hi >> 16);
-    return v;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf16x8, v);
 }
 
 // Logit of this lane's face.  Every lane of the wavefront must call it with
@@ -363,27 +513,20 @@ __device__ __forceinline__ float mlp_logit_wave(const float x[kFeat], const MlpW
 {
     const int l = __lane_id();
     const bool upper = l >= 32;
-    // own features packed as bf16 pairs: P0 = f0..3, P1 = f4..7, P2 = f8..11
-    const uint32_t p0a = pack_bf16(x[0], x[1]), p0b = pack_bf16(x[2], x[3]);
-    const uint32_t p1a = pack_bf16(x[4], x[5]), p1b = pack_bf16(x[6], x[7]);
-    const uint32_t p2a = pack_bf16(x[8], x[9]), p2b = pack_bf16(x[10], x[11]);
-    // lower lanes need P0, P2 of their upper partner; upper lanes need P1 of their lower partner
-    const uint32_t s0 = upper ? p0a : p1a, s1 = upper ? p0b : p1b;
-    const uint32_t s2 = upper ? p2a : 0u, s3 = upper ? p2b : 0u;
+    const uint32_t p0 = pack_bf16(x[0], x[1]), p1 = pack_bf16(x[2], x[3]);
+    const uint32_t p2 = pack_bf16(x[4], x[5]), p3 = pack_bf16(x[6], x[7]);
+    const uint32_t p4 = pack_bf16(x[8], x[9]), p5 = pack_bf16(x[10], x[11]);
+    // lower lanes need features 0..7 of their upper partner, upper lanes features 8..15 of their lower partner
+    const uint32_t s0 = upper ? p0 : p4, s1 = upper ? p1 : p5, s2 = upper ? p2 : 0u, s3 = upper ? p3 : 0u;
     const uint32_t r0 = (uint32_t)__shfl_xor((int)s0, 32), r1 = (uint32_t)__shfl_xor((int)s1, 32);
     const uint32_t r2 = (uint32_t)__shfl_xor((int)s2, 32), r3 = (uint32_t)__shfl_xor((int)s3, 32);
-    bf16x4 xb[2][2];
-    const bf16x4 zero = frag_of(0u, 0u);
+    bf16x8 xb[2];
     if (!upper) {
-        xb[0][0] = frag_of(p0a, p0b);
-        xb[0][1] = frag_of(p2a, p2b);
-        xb[1][0] = frag_of(r0, r1);
-        xb[1][1] = frag_of(r2, r3);
+        xb[0] = frag8(p0, p1, p2, p3);
+        xb[1] = frag8(r0, r1, r2, r3);
     } else {
-        xb[0][0] = frag_of(r0, r1);
-        xb[0][1] = zero;
-        xb[1][0] = frag_of(p1a, p1b);
-        xb[1][1] = zero;
+        xb[0] = frag8(r0, r1, 0u, 0u);
+        xb[1] = frag8(p4, p5, 0u, 0u);
     }
     const int kh = upper ? 4 : 0;
     float part[2];
@@ -394,25 +537,26 @@ __device__ __forceinline__ float mlp_logit_wave(const float x[kFeat], const MlpW
         for (int g = 0; g < 4; ++g)
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc[4 * g + t] = bias[8 * g + kh + t];
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(f.w1t[0], xb[c][0], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(f.w1t[1], xb[c][1], acc, 0, 0, 0);
-        bf16x4 hb[4];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.w1t, xb[c], acc, 0, 0, 0);
+        bf16x8 hb[2];
 #pragma unroll
-        for (int g = 0; g < 4; ++g)
-            hb[g] = frag_of(pack_bf16(fmaxf(acc[4 * g], 0.0f), fmaxf(acc[4 * g + 1], 0.0f)),
-                            pack_bf16(fmaxf(acc[4 * g + 2], 0.0f), fmaxf(acc[4 * g + 3], 0.0f)));
+        for (int ks = 0; ks < 2; ++ks)
+            hb[ks] = frag8(pack_bf16(relu(acc[8 * ks + 0]), relu(acc[8 * ks + 1])),
+                           pack_bf16(relu(acc[8 * ks + 2]), relu(acc[8 * ks + 3])),
+                           pack_bf16(relu(acc[8 * ks + 4]), relu(acc[8 * ks + 5])),
+                           pack_bf16(relu(acc[8 * ks + 6]), relu(acc[8 * ks + 7])));
         floatx16 acc2;
 #pragma unroll
         for (int g = 0; g < 4; ++g)
 #pragma unroll
             for (int t = 0; t < 4; ++t) acc2[4 * g + t] = bias[32 + 8 * g + kh + t];
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) acc2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(f.w2t[ks], hb[ks], acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.w2t[0], hb[0], acc2, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.w2t[1], hb[1], acc2, 0, 0, 0);
         float s = 0.0f;
 #pragma unroll
         for (int g = 0; g < 4; ++g)
 #pragma unroll
-            for (int t = 0; t < 4; ++t) s += fmaxf(acc2[4 * g + t], 0.0f) * bias[64 + 8 * g + kh + t];
+            for (int t = 0; t < 4; ++t) s = fmaf(relu(acc2[4 * g + t]), bias[64 + 8 * g + kh + t], s);
         part[c] = s;
     }
     // sample j of block c: halves in lanes j and j + 32

@@ -87,6 +87,7 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
 /* faces sent to the linear flux and faces evaluated since the last reset */
 int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset);
 int artra_dev_set_fused(ArtraDevice *dev, int fused);
+int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation);
 int artra_dev_fused_axes(ArtraDevice *dev);
 /* faces along one axis of the interior lines, and their features/labels on level TO */
 long artra_dev_face_count(ArtraDevice *dev, int axis);

@@ -335,6 +335,11 @@ int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[2], int
     return artra_dev_hybrid_counts(s->dev, counts, reset) ? DevError() : 0;
 }
 
+int artra_hip_set_indicator_options(ArtraHipSolver *s, int every_stage, float smooth_variation)
+{
+    return artra_dev_set_indicator_options(s->dev, every_stage, smooth_variation) ? DevError() : 0;
+}
+
 int artra_hip_set_fused(ArtraHipSolver *s, int fused)
 {
     return artra_dev_set_fused(s->dev, fused) ? DevError() : 0;

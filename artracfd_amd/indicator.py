@@ -20,9 +20,10 @@ from . import lib
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "weights", "indicator.npz")
 
-# routing defaults chosen on the 64^3 shock-vortex case (see tests)
-TAU = 0.2
-LOGIT_THRESHOLD = -2.0
+# label threshold of the committed weights and the routing threshold chosen
+# on the 64^3 shock-vortex run
+TAU = 0.4
+LOGIT_THRESHOLD = 2.0
 
 
 def _case_text(mx, my, end, steps, init, bcs, regions):
@@ -156,7 +157,7 @@ def classification_report(params, features, labels, tau=TAU, logit_threshold=LOG
             "false_troubled": float((~truth & pred).sum() / max((~truth).sum(), 1))}
 
 
-def evaluate(case_directory, params, thresholds, max_steps=1 << 20):
+def evaluate(case_directory, params, thresholds, max_steps=1 << 20, every_stage=False):
     """Plain WENO5 and routed runs of one case to its end time: per logit
     threshold the relative L1 density difference, the linear fraction of
     the faces and both run times (s)."""
@@ -166,7 +167,7 @@ def evaluate(case_directory, params, thresholds, max_steps=1 << 20):
         case = lib.Case(case_directory)
         solver = lib.Solver(case)
         if p is not None:
-            solver.set_indicator(p, thr)
+            solver.set_indicator(p, thr, every_stage=every_stage)
         solver.hybrid_counts(reset=True)
         solver.synchronize()
         t0 = time.perf_counter()

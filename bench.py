@@ -54,13 +54,34 @@ def stage_bytes(tscheme):
     return (80.0 + 120.0 + 120.0) / 3.0 if tscheme == 1 else (80.0 + 120.0) / 2.0
 
 
-def cpu_baseline(seconds_budget=12.0):
-    """The CPU oracle (single-thread restatement of the reference's hot path,
-    FP64, -O2) on the 64^3 version of the same shock-vortex case."""
+def cpu_baseline(seconds_budget=12.0, ref_steps=10):
+    """The reference's own CPU path where its compiled harness is present
+    (oracle/_ref/artra_ref, built from the reference sources): a bounded
+    number of full time steps of the 64^3 version of the same shock-vortex
+    case, single thread, timed by the harness around its time loop.
+    Otherwise the CPU oracle (the single-thread restatement, FP64, -O2) on
+    the same case for about seconds_budget seconds."""
+    import shutil
+    import subprocess
+    import tempfile
+    ref = os.path.join(ROOT, "oracle", "_ref", "artra_ref")
+    src = os.path.join(ROOT, "cases", "shockvortex64")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as tmp:
+            work = os.path.join(tmp, "case")
+            shutil.copytree(src, work)
+            out = subprocess.run([ref, "run", str(ref_steps), "out"], cwd=work, check=True, capture_output=True,
+                                 text=True, timeout=600)
+            info = json.loads(out.stdout.strip().splitlines()[-1])
+        return {"value": info["fluid_nodes"] * 18 * info["steps"] / info["seconds"] / 1e6,
+                "unit": "Mcell-stage updates/s", "cores": 1, "kind": "reference",
+                "sample": f"shockvortex64 (63^3 interior nodes), {info['steps']} full time steps, "
+                          f"{info['seconds']:.1f} s, the reference's solver compiled from its own sources "
+                          f"(oracle/_ref), single thread (it has no threading of its own)"}
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from artracfd_amd import Case
-    case = Case(os.path.join(ROOT, "cases", "shockvortex64"))
+    case = Case(src)
     oracle.treat_boundary(case, 0)
     cells = fluid_nodes(case)
     steps, t0 = 0, time.perf_counter()
