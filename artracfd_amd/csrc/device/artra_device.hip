@@ -849,7 +849,7 @@ struct BatchTile {
 };
 
 template <int S, int NS, bool HYB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void rk_batch(const BatchTile *tiles,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void rk_batch(const BatchTile *tiles,
                                                                                          int ntiles,
                                                                                          const LineArgs *args,
                                                                                          FluxModel mdl, LineHybrid hy)
@@ -870,7 +870,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
         const bool write = slot < bt.count;
         const long line = bt.first + (write ? slot : bt.count - 1);
         char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1);
-        LineBody<S, NS, true, HYB, 2>(la, mdl, hy, line, tt, bt.T, mem, write, tally, bias, frags);
+        LineBody<S, NS, true, HYB, 1>(la, mdl, hy, line, tt, bt.T, mem, write, tally, bias, frags);
         __syncthreads();
     }
     if (HYB && threadIdx.x == 0) {
@@ -2169,8 +2169,8 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
             dev->mdl.sw != d0->mdl.sw) {
             return Fail("batched case %d: every case must share the time scheme and flux model", c);
         }
-        if (dev->g.hi[0] - dev->g.lo[0] > 512 || dev->g.hi[1] - dev->g.lo[1] > 512 || dev->g.hi[2] - dev->g.lo[2] > 512)
-            return Fail("batched case %d: lines longer than 512 interior nodes", c);
+        if (dev->g.hi[0] - dev->g.lo[0] > 256 || dev->g.hi[1] - dev->g.lo[1] > 256 || dev->g.hi[2] - dev->g.lo[2] > 256)
+            return Fail("batched case %d: lines longer than 256 interior nodes", c);
     }
     ArtraBatch *b = new ArtraBatch();
     b->cases.assign(devs, devs + count);
@@ -2193,7 +2193,12 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         int longest = 0;
         for (int p = 0; p < 3; ++p) longest = std::max(longest, g.hi[p] - g.lo[p]);
         int T = 64;
-        while (2 * T < longest) T *= 2;
+        while (T < longest) T *= 2;         // one interior node per thread
+        // fewer, wider lines per block where the LDS of 256 / T lines would not fit
+        const int ext[3] = {g.nx, g.ny, g.nz};
+        for (int p = 0; p < 3; ++p) {
+            while (T < 256 && (size_t)(256 / T) * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1) > 64 * 1024) T *= 2;
+        }
         b->threads[c] = std::min(T, 256);
     }
     // line arguments of every case and axis
@@ -2475,8 +2480,7 @@ int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_ou
         double best = 1e300;
         int bestT = 0;
         for (int T = 64; T <= 256; T *= 2) {
-            if (2 * T < L) continue;            // two nodes per lane at most
-            if (T > 64 && T / 2 >= L + 1) break; // a whole wavefront idle
+            if (T < L) continue;                // one interior node per thread
             for (int c = 0; c < b->C; ++c) {
                 int Lc = 0;
                 for (int p = 0; p < 3; ++p) Lc = std::max(Lc, b->cases[c]->g.hi[p] - b->cases[c]->g.lo[p]);

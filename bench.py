@@ -37,10 +37,12 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # Measured with rocprofv3 on this workload (profiles/rk_line_shockvortex256.md):
 # HBM bytes per cell-stage of the fused sweep kernel (FETCH_SIZE x 2.0 +
-# WRITE_SIZE, scale factors from a 1 GiB copy kernel; None until profiled) and
-# FP64 flops per cell-stage of the plain WENO5 flux and update (FMA = 2).
-TRAFFIC_B_PER_CELL_STAGE = None
+# WRITE_SIZE, scale factors from a 1 GiB copy kernel) and FP64 flops per
+# cell-stage of the plain WENO5 flux and update (FMA = 2).
+TRAFFIC_B_PER_CELL_STAGE = 27.8
 FP64_FLOP_PER_CELL_STAGE = 1124.0
+# the stage-by-stage sweeps (profiles/sweep_shockvortex256.md)
+UNFUSED_TRAFFIC_B_PER_CELL_STAGE = 129.6
 METRIC = "Mcell-stage updates/s (256³ WENO5-RK3) + learned-hybrid vs plain at ≤1e-3 L1"
 
 
@@ -163,8 +165,11 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
         b.close()
         for s in solvers:
             s.close()
-        # one case at a time, same steps, fresh state
+        for c in cases:
+            c.close()
+        # one case at a time, same steps, from the initial state
         per_case = []
+        cases = [Case(d) for d in dirs]
         for c in cases:
             solver = Solver(c)
             if params is not None:
@@ -205,12 +210,17 @@ def main():
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--unfused", action="store_true", help="stage-by-stage sweeps instead of the fused line sweeps")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-case batched workload")
+    ap.add_argument("--batched-only", action="store_true", help="run only the 64-case batched workload")
+    ap.add_argument("--batched-plain", action="store_true", help="batched workload without the indicator")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is a "
                                    "design (see DESIGN.md), so --gpus must be 1"}))
         sys.exit(2)
     from artracfd_amd import Case, Solver
+    if args.batched_only:
+        print(json.dumps({"batched": batched(use_indicator=not args.batched_plain)}))
+        return
     case_dir = os.path.join(ROOT, "cases", args.case)
     t_init = time.perf_counter()
     case = Case(case_dir)
@@ -231,8 +241,9 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9
     flops_per_launch = FP64_FLOP_PER_CELL_STAGE * cells * stages_per_launch
     traffic = None
-    if TRAFFIC_B_PER_CELL_STAGE is not None and args.case == "shockvortex256" and not args.unfused:
-        traffic = round(TRAFFIC_B_PER_CELL_STAGE * cells * stages_per_launch)
+    if args.case == "shockvortex256":
+        per = TRAFFIC_B_PER_CELL_STAGE if not args.unfused else UNFUSED_TRAFFIC_B_PER_CELL_STAGE
+        traffic = round(per * cells * stages_per_launch)
     kernel = ("rk_line (fused directional sweep, all RK stages of one line in LDS)" if fused_axes
               else "sweep_x/sweep_march (stage-by-stage directional sweeps)")
     out = {
@@ -290,7 +301,7 @@ def main():
             "logit_threshold": thr, "kernel_ms": {k: round(v[0], 3) for k, v in htm.items()},
         }
     if not args.no_batched and not args.unfused:
-        out["batched"] = batched()
+        out["batched"] = batched(use_indicator=not args.batched_plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
