@@ -159,12 +159,34 @@ def _evolve(name_or_dir, steps, params=None, threshold=0.0, gate=1e-3):
     return U, dts, counts
 
 
-def test_all_troubled_routing_is_plain_weno5():
+def _flux_model_variant(tmp_path, model):
+    """shockvortex64 with the Roe average (numerical block line 4) or the
+    Steger-Warming split (line 5) in place of the default mean state and
+    Lax-Friedrichs split."""
+    if model == "llf":
+        return case_dir("shockvortex64")
+    src = case_dir("shockvortex64")
+    dst = tmp_path / f"shockvortex64_{model}"
+    dst.mkdir()
+    lines = open(os.path.join(src, "artracfd.case")).read().split("\n")
+    at = lines.index("numerical begin") + (4 if model == "roe" else 5)
+    assert lines[at].strip() == "0"
+    lines[at] = "1"
+    (dst / "artracfd.case").write_text("\n".join(lines))
+    (dst / "artracfd.geo").write_text(open(os.path.join(src, "artracfd.geo")).read())
+    return str(dst)
+
+
+@pytest.mark.parametrize("model", ["llf", "roe", "sw"])
+def test_all_troubled_routing_is_plain_weno5(model, tmp_path):
     """With every face routed to WENO5 the hybrid sweeps reproduce the plain
-    sweeps (same face state, projection and update arithmetic)."""
-    print("hybrid all-WENO shockvortex64", flush=True)
-    U0, d0, _ = _evolve("shockvortex64", 4)
-    U1, d1, (lin, seen) = _evolve("shockvortex64", 4, _params(3), threshold=-1e30, gate=0.0)
+    sweeps (same face state, projection and update arithmetic), for the
+    Lax-Friedrichs split on the mean state and for the Roe average and the
+    Steger-Warming split, which run a kernel variant of their own."""
+    name = _flux_model_variant(tmp_path, model)
+    print("hybrid all-WENO shockvortex64", model, flush=True)
+    U0, d0, _ = _evolve(name, 4)
+    U1, d1, (lin, seen) = _evolve(name, 4, _params(3), threshold=-1e30, gate=0.0)
     assert lin == 0 and seen > 0
     np.testing.assert_array_equal(d1, d0)
     assert np.array_equal(U1, U0)
@@ -201,5 +223,24 @@ def test_trained_indicator_on_shock_vortex():
     Uh, dh, (lin, seen) = _evolve("shockvortex64", 1 << 20, params, thr)
     l1 = np.abs(Uh[0] - Up[0]).sum() / np.abs(Up[0]).sum()
     print("steps", len(dp), len(dh), "rel L1 rho", l1, "linear fraction", lin / seen)
+    assert l1 <= 1e-3
+    assert lin / seen >= 0.5
+
+
+@pytest.mark.parametrize("model", ["roe", "sw"])
+def test_trained_indicator_with_roe_and_steger_warming(model, tmp_path):
+    """The routed path of the Roe-average and Steger-Warming flux models
+    (linear flux through the characteristic projection) on the 64^3
+    shock-vortex case for 20 steps: density within 1e-3 relative L1 of
+    plain WENO5, mostly linear faces."""
+    from artracfd_amd import indicator
+    name = _flux_model_variant(tmp_path, model)
+    print("trained indicator shockvortex64", model, flush=True)
+    params, meta = indicator.load()
+    thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
+    Up, dp, _ = _evolve(name, 20)
+    Uh, dh, (lin, seen) = _evolve(name, 20, params, thr)
+    l1 = np.abs(Uh[0] - Up[0]).sum() / np.abs(Up[0]).sum()
+    print("rel L1 rho", l1, "linear fraction", lin / seen)
     assert l1 <= 1e-3
     assert lin / seen >= 0.5
