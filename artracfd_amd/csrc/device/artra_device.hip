@@ -458,6 +458,13 @@ __global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__res
 // recomputed with full WENO5 by the whole block, so WENO5 work stays dense
 // across lanes however the troubled faces are scattered.
 constexpr int kLineRules = 12;
+// The face tallies of the indicator are spread over slots on separate
+// 128-byte lines (block b adds into slot b mod kCountSlots): one counter
+// pair taking an atomic from every block serialises tens of thousands of
+// atomics per launch on one L2 line.
+constexpr int kCountSlots = 256;
+constexpr int kCountStride = 16;                 // unsigned long long per slot
+constexpr size_t kCountBytes = (size_t)kCountSlots * kCountStride * sizeof(unsigned long long);
 constexpr int kLineMaxK = 2;        // interior nodes per thread (K * blockDim >= interior)
 
 struct LineArgs {
@@ -484,7 +491,7 @@ struct LineHybrid {
     const uint16_t *W1, *W2;        // bf16 weights
     const float *bias;              // b1[32], b2[32], w3[32]
     float b3, threshold;
-    unsigned long long *counts;     // [0] faces on the linear flux, [1] faces evaluated
+    unsigned long long *counts;     // kCountSlots slots of kCountStride: [0] faces on the linear flux, [1] faces evaluated
     int everyStage;                 // 1: classify at every RK stage; 0: once per directional operator
     float smoothVariation;          // faces varying less than this (relative) skip the network; 0: none
 };
@@ -509,13 +516,18 @@ __host__ __device__ __forceinline__ int LineQueueCap(int F)
 // LDS of one line: the states [5][n] and face fluxes [5][F]; for the
 // indicator also the velocities and total enthalpy of every node [4][n],
 // the half-field WENO5 results of the queued faces [10][cap], the queue
-// [cap], each face's queue position [F], the queue length and the FP32
-// density and pressure of every node [2][n].
-__host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F)
+// [cap], each face's queue position [F], the queue length, the line's fill
+// rules [2][kLineRules] and the FP32 density and pressure of every node
+// [2][n]; with u0 (the indicator's
+// sweeps) then the operator's initial state of the interior nodes [5][F - 1],
+// which the plain sweeps keep in registers.
+__host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool u0)
 {
     const int cap = LineQueueCap(F);
-    return ((size_t)(9 * n + 5 * F + 10 * cap) * sizeof(double) + (size_t)(cap + F + 4) * sizeof(int) +
-            (size_t)2 * n * sizeof(float) + 15) / 16 * 16;
+    const size_t base = ((size_t)(9 * n + 5 * F + 10 * cap) * sizeof(double) +
+                         (size_t)(cap + F + 4 + 2 * kLineRules) * sizeof(int) +
+                         (size_t)2 * n * sizeof(float) + 15) / 16 * 16;
+    return base + (u0 ? (size_t)5 * (F - 1) * sizeof(double) : 0);
 }
 
 // Node quantities the routed faces use: velocities and total enthalpy (FP64)
@@ -560,13 +572,28 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     int *queue = (int *)(hbuf + 10 * cap);          // [cap]
     int *fidx = queue + cap;                        // [F]
     int *queued = fidx + F;
-    float *nodeRP = (float *)(queued + 4);          // [2][n]
+    int *rules = queued + 4;                        // [2][kLineRules]: destinations, sources
+    float *nodeRP = (float *)(rules + 2 * kLineRules); // [2][n]
+    double *u0 = (double *)(mem + LineLdsBytes(n, F, false));  // [5][F - 1] (HYB)
     const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
     for (int p = tt; p < n; p += T) {
+        double v[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) st[c * n + p] = la.U[c * la.N + base + (long)p * la.stride];
-        if (HYB) LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
+        for (int c = 0; c < 5; ++c) st[c * n + p] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
+        if (HYB) {
+            LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
+            if (p >= la.lo && p < la.hi) {
+#pragma unroll
+                for (int c = 0; c < 5; ++c) u0[c * (F - 1) + p - la.lo] = v[c];
+            }
+        }
+    }
+    // the line's fill rules into LDS: read per lane from the argument
+    // block once here, where the field loads hide the latency
+    for (int e = tt; e < la.nrules; e += T) {
+        rules[e] = la.rdst[e];
+        rules[kLineRules + e] = la.rsrc[e];
     }
     bool own[K], fl[K];
 #pragma unroll
@@ -577,12 +604,16 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     }
     if (HYB && tt == 0) *queued = 0;
     __syncthreads();
+    // the operator's initial state: in registers, or (HYB, whose flux
+    // passes leave no registers for it) in LDS
     double U0[K][5];
+    if (!HYB) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int i = own[k] ? la.lo + tt + T * k : la.lo;
+        for (int k = 0; k < K; ++k) {
+            const int i = own[k] ? la.lo + tt + T * k : la.lo;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) U0[k][c] = st[c * n + i];
+            for (int c = 0; c < 5; ++c) U0[k][c] = st[c * n + i];
+        }
     }
     const int M = (F + T - 1) / T;
     const double r = la.dt ? *la.dt * la.dd : la.r;
@@ -780,7 +811,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const double FL = fb[c * F + q], FR = fb[c * F + q + 1];
                 const double phi = FAST ? FL - FR : ((FL - FR) + 0.0) - 0.0;
                 const double Un = st[c * n + i];
-                const double Uo = (s == 0) ? Un : U0[k][c];
+                const double Uo = (s == 0) ? Un : HYB ? u0[c * (F - 1) + q] : U0[k][c];
                 st[c * n + i] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
             }
             if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, hy.everyStage != 0);
@@ -789,7 +820,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         if (s + 1 < la.stages) {
             // refill the line's ghost ends from the stage's interior
             for (int e = tt; e < la.nrules; e += T) {
-                const int d = la.rdst[e], q = la.rsrc[e];
+                const int d = rules[e], q = rules[kLineRules + e];
                 double v[5];
                 if (q >= 0) {
 #pragma unroll
@@ -831,8 +862,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : 2
     const long line = XcdLine(blockIdx.x, gridDim.x);
     LineBody<S, NS, FAST, HYB, K>(la, mdl, hy, line, threadIdx.x, blockDim.x, (char *)lds, true, tally, bias, frags);
     if (HYB && threadIdx.x == 0) {
-        atomicAdd(hy.counts, tally[0]);
-        atomicAdd(hy.counts + 1, tally[1]);
+        unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
+        atomicAdd(slot, tally[0]);
+        atomicAdd(slot + 1, tally[1]);
     }
 }
 
@@ -869,13 +901,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         const int slot = threadIdx.x / bt.T, tt = threadIdx.x % bt.T;
         const bool write = slot < bt.count;
         const long line = bt.first + (write ? slot : bt.count - 1);
-        char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1);
+        char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1, true);
         LineBody<S, NS, true, HYB, 1>(la, mdl, hy, line, tt, bt.T, mem, write, tally, bias, frags);
         __syncthreads();
     }
     if (HYB && threadIdx.x == 0) {
-        atomicAdd(hy.counts, tally[0]);
-        atomicAdd(hy.counts + 1, tally[1]);
+        unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
+        atomicAdd(slot, tally[0]);
+        atomicAdd(slot + 1, tally[1]);
     }
 }
 
@@ -1357,7 +1390,7 @@ struct LineShape {
     long lines = 0;
 };
 
-LineShape LineShapeOf(const ArtraDevice *dev, int p)
+LineShape LineShapeOf(const ArtraDevice *dev, int p, bool hyb)
 {
     const Grid &g = dev->g;
     const int ext[3] = {g.nx, g.ny, g.nz};
@@ -1370,19 +1403,19 @@ LineShape LineShapeOf(const ArtraDevice *dev, int p)
     sh.T = ((per + 63) / 64) * 64;
     if (sh.T > 256) return LineShape{};
     const int F = interior + 1;
-    sh.lds = LineLdsBytes(ext[p], F);
+    sh.lds = LineLdsBytes(ext[p], F, hyb);
     sh.lines = (long)(g.hi[t1] - g.lo[t1]) * (g.hi[t2] - g.lo[t2]);
     return sh;
 }
 
 // Whether one directional operator along p can run as one fused rk_line launch.
-bool LineEligible(const ArtraDevice *dev, int p)
+bool LineEligible(const ArtraDevice *dev, int p, bool hyb)
 {
     if (!dev->fused || !dev->lines[p].ok || !dev->allFluid) return false;
     if (dev->layerStart.back() != 0) return false;               // immersed-boundary ghosts present
     if (dev->mdl.refMu > 0.0 || dev->par.multidim != 0) return false;
     if (!dev->gatherResolved || !dev->useGather || !dev->partialFill) return false;
-    const LineShape sh = LineShapeOf(dev, p);
+    const LineShape sh = LineShapeOf(dev, p, hyb);
     return sh.T > 0 && sh.lines > 0 && sh.lds <= 64 * 1024;
 }
 
@@ -1417,7 +1450,7 @@ void LaunchLineS(ArtraDevice *dev, const LineShape &sh, const LineArgs &la, cons
 int LineSweep(ArtraDevice *dev, double dt, int p)
 {
     const Grid &g = dev->g;
-    const LineShape sh = LineShapeOf(dev, p);
+    const LineShape sh = LineShapeOf(dev, p, dev->hybrid != 0);
     const int ext[3] = {g.nx, g.ny, g.nz};
     const int stride[3] = {1, g.nx, g.nx * g.ny};
     const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
@@ -1557,7 +1590,7 @@ void DeriveLineRules(ArtraDevice *dev, const ArtraBcGather *gt)
 // exactly as TreatBoundary does after every stage.
 int Stages(ArtraDevice *dev, double dt, int p)
 {
-    if (p < 3 && LineEligible(dev, p)) return LineSweep(dev, dt, p) || TreatBoundary(dev, 0);
+    if (p < 3 && LineEligible(dev, p, dev->hybrid != 0)) return LineSweep(dev, dt, p) || TreatBoundary(dev, 0);
     if (dev->hybrid) return Fail("the learned indicator is on but axis %d cannot run fused", p);
     const int mid = (p < 3 && !(dev->mdl.refMu > 0.0) && dev->partialFill) ? 1 + p : 0;
     if (dev->par.tScheme == 0) {
@@ -1910,7 +1943,7 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
                                     {-1, -1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1}, {0, 0, -1, -1, -1, -1}};
     const int c = dev->par.collapse;
     for (int q = 0; c >= 0 && c <= 8 && q < 6 && order[c][q] >= 0; ++q) {
-        if (!LineEligible(dev, order[c][q])) {
+        if (!LineEligible(dev, order[c][q], true)) {
             return Fail("the learned indicator runs inside the fused line sweeps: it needs an inviscid split case "
                         "without immersed bodies whose line fill copies along lines and whose lines fit in LDS "
                         "(axis %d does not qualify)", order[c][q]);
@@ -1921,7 +1954,7 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
         HIP_TRY(hipMalloc(&dev->mlpW1, 512 * sizeof(uint16_t)));
         HIP_TRY(hipMalloc(&dev->mlpW2, 1024 * sizeof(uint16_t)));
         HIP_TRY(hipMalloc(&dev->mlpBias, 96 * sizeof(float)));
-        HIP_TRY(hipMalloc(&dev->hybCounts, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&dev->hybCounts, kCountBytes));
     }
     std::vector<uint16_t> w1(512), w2(1024);
     for (int q = 0; q < 512; ++q) w1[q] = artra::f32_to_bf16(params[ARTRA_MLP_W1 + q]);
@@ -1936,7 +1969,7 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
     HIP_TRY(hipMemcpy(dev->mlpW1, w1.data(), 512 * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dev->mlpW2, w2.data(), 1024 * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dev->mlpBias, bias.data(), 96 * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(dev->hybCounts, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(dev->hybCounts, 0, kCountBytes));
     dev->mlpB3 = params[ARTRA_MLP_B3];
     dev->mlpThreshold = threshold;
     dev->hybrid = 1;
@@ -1962,8 +1995,22 @@ int artra_dev_fused_axes(ArtraDevice *dev)
 {
     int mask = 0;
     for (int p = 0; p < 3; ++p)
-        if (LineEligible(dev, p)) mask |= 1 << p;
+        if (LineEligible(dev, p, dev->hybrid != 0)) mask |= 1 << p;
     return mask;
+}
+
+// Sum of the tally slots (and their reset).
+static int SumCounts(unsigned long long *slots, unsigned long long counts[2], int reset)
+{
+    std::vector<unsigned long long> h((size_t)kCountSlots * kCountStride);
+    HIP_TRY(hipMemcpy(h.data(), slots, kCountBytes, hipMemcpyDeviceToHost));
+    counts[0] = counts[1] = 0;
+    for (int q = 0; q < kCountSlots; ++q) {
+        counts[0] += h[(size_t)q * kCountStride];
+        counts[1] += h[(size_t)q * kCountStride + 1];
+    }
+    if (reset) HIP_TRY(hipMemset(slots, 0, kCountBytes));
+    return 0;
 }
 
 int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset)
@@ -1973,9 +2020,7 @@ int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int 
         return 0;
     }
     HIP_TRY(hipStreamSynchronize(dev->stream));
-    HIP_TRY(hipMemcpy(counts, dev->hybCounts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    if (reset) HIP_TRY(hipMemset(dev->hybCounts, 0, 2 * sizeof(unsigned long long)));
-    return 0;
+    return SumCounts(dev->hybCounts, counts, reset);
 }
 
 long artra_dev_face_count(ArtraDevice *dev, int axis)
@@ -2084,7 +2129,7 @@ int BatchBuildTiles(ArtraBatch *b, int p)
         const long lines = (long)(g.hi[t1] - g.lo[t1]) * (g.hi[t2] - g.lo[t2]);
         const int T = b->threads[c], P = 256 / T;
         const int ext[3] = {g.nx, g.ny, g.nz};
-        lds = std::max(lds, (size_t)P * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1));
+        lds = std::max(lds, (size_t)P * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1, true));
         for (long first = 0; first < lines; first += P) {
             BatchTile bt{};
             bt.c = c;
@@ -2159,7 +2204,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         const ArtraDevice *dev = devs[c];
         if (dev->par.collapse != 0) return Fail("batched case %d: only three-dimensional cases are batched", c);
         for (int p = 0; p < 3; ++p) {
-            if (!LineEligible(dev, p)) {
+            if (!LineEligible(dev, p, true)) {
                 return Fail("batched case %d: axis %d cannot run as fused line sweeps (inviscid, split, no "
                             "immersed body, resolvable fills and lines that fit in LDS are required)", c, p);
             }
@@ -2197,7 +2242,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         // fewer, wider lines per block where the LDS of 256 / T lines would not fit
         const int ext[3] = {g.nx, g.ny, g.nz};
         for (int p = 0; p < 3; ++p) {
-            while (T < 256 && (size_t)(256 / T) * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1) > 64 * 1024) T *= 2;
+            while (T < 256 && (size_t)(256 / T) * LineLdsBytes(ext[p], g.hi[p] - g.lo[p] + 1, true) > 64 * 1024) T *= 2;
         }
         b->threads[c] = std::min(T, 256);
     }
@@ -2363,7 +2408,7 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
         HIP_TRY(hipMalloc(&b->mlpBias, 96 * sizeof(float)));
         HIP_TRY(hipMalloc(&b->mlpW1, 512 * sizeof(uint16_t)));
         HIP_TRY(hipMalloc(&b->mlpW2, 1024 * sizeof(uint16_t)));
-        HIP_TRY(hipMalloc(&b->counts, 2 * sizeof(unsigned long long)));
+        HIP_TRY(hipMalloc(&b->counts, kCountBytes));
     }
     std::vector<uint16_t> w1(512), w2(1024);
     for (int q = 0; q < 512; ++q) w1[q] = artra::f32_to_bf16(params[ARTRA_MLP_W1 + q]);
@@ -2377,7 +2422,7 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
     HIP_TRY(hipMemcpy(b->mlpW1, w1.data(), 512 * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->mlpW2, w2.data(), 1024 * sizeof(uint16_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->mlpBias, bias.data(), 96 * sizeof(float), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(b->counts, 0, kCountBytes));
     b->hy.W1 = b->mlpW1;
     b->hy.W2 = b->mlpW2;
     b->hy.bias = b->mlpBias;
@@ -2395,9 +2440,7 @@ int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int rese
     counts[0] = counts[1] = 0;
     if (!b->counts) return 0;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(hipMemcpy(counts, b->counts, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    if (reset) HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
-    return 0;
+    return SumCounts(b->counts, counts, reset);
 }
 
 // steps time steps of every case; with use_graph the step is captured once
@@ -2537,7 +2580,7 @@ int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_ou
     HIP_TRY(hipMemcpyAsync(b->clock, clockSave.data(), clockSave.size() * sizeof(double), hipMemcpyHostToDevice,
                            b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    if (b->counts) HIP_TRY(hipMemset(b->counts, 0, 2 * sizeof(unsigned long long)));
+    if (b->counts) HIP_TRY(hipMemset(b->counts, 0, kCountBytes));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     for (int c = 0; c < b->C; ++c)
