@@ -131,6 +131,18 @@ def run_to_end(case_dir, params=None, threshold=0.0):
     return rho, steps, el
 
 
+def batched_both(with_indicator=True):
+    """The batched workload with plain WENO5, and (unless told not to) again
+    with the learned indicator, reported as a sub-object."""
+    plain = batched(use_indicator=False)
+    if with_indicator:
+        hyb = batched(use_indicator=True)
+        plain["hybrid"] = {k: hyb[k] for k in ("value", "serial_value", "wall_ms", "linear_face_fraction", "tuned",
+                                                "serial_ms_per_case")}
+        plain["hybrid"]["vs_plain"] = round(hyb["value"] / plain["value"], 3)
+    return plain
+
+
 def batched(seed=2026, count=64, steps=100, use_indicator=True):
     """The batched workload: `count` oblique-shock cases of uneven cubic
     sizes (artracfd_amd.batch), tuned, then `steps` lockstep steps replayed
@@ -211,7 +223,7 @@ def main():
     ap.add_argument("--unfused", action="store_true", help="stage-by-stage sweeps instead of the fused line sweeps")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-case batched workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the 64-case batched workload")
-    ap.add_argument("--batched-plain", action="store_true", help="batched workload without the indicator")
+    ap.add_argument("--batched-plain", action="store_true", help="batched workload without the indicator run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is a "
@@ -219,7 +231,7 @@ def main():
         sys.exit(2)
     from artracfd_amd import Case, Solver
     if args.batched_only:
-        print(json.dumps({"batched": batched(use_indicator=not args.batched_plain)}))
+        print(json.dumps({"batched": batched_both(not args.batched_plain)}))
         return
     case_dir = os.path.join(ROOT, "cases", args.case)
     t_init = time.perf_counter()
@@ -301,7 +313,7 @@ def main():
             "logit_threshold": thr, "kernel_ms": {k: round(v[0], 3) for k, v in htm.items()},
         }
     if not args.no_batched and not args.unfused:
-        out["batched"] = batched(use_indicator=not args.batched_plain)
+        out["batched"] = batched_both(not args.batched_plain)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
