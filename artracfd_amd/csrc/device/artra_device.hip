@@ -492,7 +492,6 @@ struct LineHybrid {
     const float *bias;              // b1[32], b2[32], w3[32]
     float b3, threshold;
     unsigned long long *counts;     // kCountSlots slots of kCountStride: [0] faces on the linear flux, [1] faces evaluated
-    int everyStage;                 // 1: classify at every RK stage; 0: once per directional operator
     float smoothVariation;          // faces varying less than this (relative) skip the network; 0: none
 };
 
@@ -550,14 +549,14 @@ __device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const d
 // the stages are separated by block barriers.  With write = false the line
 // is computed but not stored (padding slots of a batched block).
 //
-// With HYB, the indicator classifies the faces once per operator (or at
-// every stage) into a queue of troubled faces; each stage then runs one
+// With HYB, the indicator classifies the faces once per operator into a
+// queue of troubled faces; each stage then runs one
 // balanced pass over F + 10 x queued work items - the linear flux of every
 // routed face and the ten half-field WENO5 evaluations of every queued
 // face - a short pass projecting the queued faces' half-fields back, and
 // the node update, which also refreshes the node quantities of the next
-// stage.  A line with more troubled faces than the queue holds runs WENO5
-// on every face for that operator.
+// stage.  A line with more troubled faces than the queue holds runs the
+// plain WENO5 flux on every face for that operator.
 template <int S, int NS, bool FAST, bool HYB, int K>
 __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &mdl, const LineHybrid &hy, long line,
                                          int tt, int T, char *mem, bool write, unsigned long long *tally,
@@ -620,45 +619,48 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
     bool routed = false;                            // this operator routes faces (HYB, queue within cap)
     int nq = 0;
-    for (int s = 0; s < la.stages; ++s) {
-        const bool classify = HYB && (s == 0 || hy.everyStage);
-        if (classify) {
-            // the indicator runs at the first stage (or every stage); later
-            // stages reuse its queue of troubled faces.  The loop trip count
-            // is the same for every thread (whole wavefronts run the network).
-            for (int m = 0; m < M; ++m) {
-                const int q = tt + T * m;
-                const bool valid = q < F;
-                const int f = la.lo - 1 + (valid ? q : F - 1);
-                // waves whose faces are all near-uniform skip the network
-                const bool smooth = hy.smoothVariation > 0.0f &&
-                                    artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
-                bool troubled = false;
-                if (__ballot(valid && !smooth)) {
-                    float x[artra::kFeat];
-                    artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
-                    const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
-                    troubled = valid && !smooth && z > hy.threshold;
-                }
-                const unsigned long long mask = __ballot(troubled);
-                const unsigned long long seen = __ballot(valid && write);
-                int slot = 0;
-                if (__lane_id() == 0) {
-                    if (mask) slot = atomicAdd(queued, __popcll(mask));
-                    atomicAdd(&tally[0], (unsigned long long)__popcll(seen & ~mask));
-                    atomicAdd(&tally[1], (unsigned long long)__popcll(seen));
-                }
-                slot = __shfl(slot, 0);
-                const int pos = slot + __popcll(mask & ((1ull << __lane_id()) - 1ull));
-                if (troubled && pos < cap) queue[pos] = q;
-                if (valid) fidx[q] = troubled ? pos : -1;
+    // The indicator's pass over the faces of the line, once per directional
+    // operator before its stages (the solution moves less than a cell within
+    // one operator): the queue of troubled faces, each face's position in
+    // it, and whether the line routes (the queue within its capacity).
+    // Every thread runs the same trip count (whole wavefronts run the
+    // network).  The weight fragments are dead after it, so they hold no
+    // registers in the flux passes.
+    auto classifyLine = [&](const artra::MlpWave &frags) {
+        for (int m = 0; m < M; ++m) {
+            const int q = tt + T * m;
+            const bool valid = q < F;
+            const int f = la.lo - 1 + (valid ? q : F - 1);
+            // waves whose faces are all near-uniform skip the network
+            const bool smooth = hy.smoothVariation > 0.0f &&
+                                artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
+            bool troubled = false;
+            if (__ballot(valid && !smooth)) {
+                float x[artra::kFeat];
+                artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
+                const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
+                troubled = valid && !smooth && z > hy.threshold;
             }
-            __syncthreads();
-            nq = *queued;
-            // every line of a block takes the same path, so the barriers below match
-            routed = __syncthreads_and(nq <= cap) != 0;
-            if (tt == 0) *queued = 0;       // ready for the next classification
+            const unsigned long long mask = __ballot(troubled);
+            const unsigned long long seen = __ballot(valid && write);
+            int slot = 0;
+            if (__lane_id() == 0) {
+                if (mask) slot = atomicAdd(queued, __popcll(mask));
+                atomicAdd(&tally[0], (unsigned long long)__popcll(seen & ~mask));
+                atomicAdd(&tally[1], (unsigned long long)__popcll(seen));
+            }
+            slot = __shfl(slot, 0);
+            const int pos = slot + __popcll(mask & ((1ull << __lane_id()) - 1ull));
+            if (troubled && pos < cap) queue[pos] = q;
+            if (valid) fidx[q] = troubled ? pos : -1;
         }
+        __syncthreads();
+        nq = *queued;
+        // every line of a block takes the same path, so the barriers below match
+        routed = __syncthreads_and(nq <= cap) != 0;
+    };
+    if (HYB) classifyLine(frags);
+    for (int s = 0; s < la.stages; ++s) {
         if (!HYB) {
             for (int m = 0; m < M; ++m) {
                 const int q = tt + T * m;
@@ -674,42 +676,20 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
             }
         } else if (!routed) {
-            // too many troubled faces on this line: WENO5 on every face, in
-            // chunks of the queue's size through the same half-field items
-            for (int c0 = 0; c0 < F; c0 += cap) {
-                const int cnt = (F - c0 < cap) ? F - c0 : cap;
-                for (int e = tt; e < 10 * cnt; e += T) {
-                    const int qi = e / 10, k = e - 10 * qi;
-                    const int f = la.lo - 1 + c0 + qi;
-                    double UL[5], UR[5];
+            // too many troubled faces on this line: WENO5 on every face, as
+            // the plain sweep computes it
+            for (int m = 0; m < M; ++m) {
+                const int q = tt + T * m;
+                if (q >= F) break;
+                const int f = la.lo - 1 + q;
+                double w[NS][5], Fx[5];
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        UL[c] = st[c * n + f];
-                        UR[c] = st[c * n + f + 1];
-                    }
-                    artra::FaceState fs;
-                    artra::face_state<S>(mdl, UL, UR, fs);
-                    artra::face_scales(fs);
-                    hbuf[e] = artra::weno_half<S>(fs, st, n, f, k >> 1, k & 1, mdl.sw != 0);
-                }
-                __syncthreads();
-                for (int e = tt; e < cnt; e += T) {
-                    const int q = c0 + e;
-                    const int f = la.lo - 1 + q;
-                    double UL[5], UR[5], h[5], Fx[5];
+                for (int j = 0; j < NS; ++j)
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        UL[c] = st[c * n + f];
-                        UR[c] = st[c * n + f + 1];
-                        h[c] = hbuf[10 * e + 2 * c] + hbuf[10 * e + 2 * c + 1];
-                    }
-                    artra::FaceState fs;
-                    artra::face_state<S>(mdl, UL, UR, fs);
-                    artra::project_back<S>(fs, h, Fx);
+                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - SL + j];
+                ConvFlux<S, NS, FAST>(mdl, w, Fx);
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
-                }
-                __syncthreads();
+                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
             }
         } else {
             // one pass over the linear fluxes of the routed faces and the
@@ -814,7 +794,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const double Uo = (s == 0) ? Un : HYB ? u0[c * (F - 1) + q] : U0[k][c];
                 st[c * n + i] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
             }
-            if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, hy.everyStage != 0);
+            if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, false);
         }
         __syncthreads();
         if (s + 1 < la.stages) {
@@ -830,7 +810,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c) st[c * n + d] = v[c];
-                if (HYB) LineNodeQuantities(mdl, st, n, d, nodeV, nodeRP, hy.everyStage != 0);
+                if (HYB) LineNodeQuantities(mdl, st, n, d, nodeV, nodeRP, false);
             }
             __syncthreads();
         }
@@ -889,15 +869,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
     extern __shared__ double lds[];
     __shared__ unsigned long long tally[2];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
-    artra::MlpWave frags;
     if (HYB) {
         for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
         if (threadIdx.x < 2) tally[threadIdx.x] = 0ull;
-        artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
     }
     for (long w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile bt = tiles[w];
         const LineArgs &la = args[bt.c];
+        // loaded per tile (L2 hits, issued ahead of the field loads) so that
+        // they hold no registers past the tile's classification
+        artra::MlpWave frags;
+        if (HYB) artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
         const int slot = threadIdx.x / bt.T, tt = threadIdx.x % bt.T;
         const bool write = slot < bt.count;
         const long line = bt.first + (write ? slot : bt.count - 1);
@@ -1203,7 +1185,6 @@ struct ArtraDevice {
     float *mlpParams = nullptr;     // FP32 biases/output vector (device)
     uint16_t *mlpW1 = nullptr, *mlpW2 = nullptr;
     float mlpThreshold = 0.0f;
-    int everyStage = 0;             // indicator at every RK stage instead of once per operator
     float smoothVariation = 1.0e-3f; // relative stencil variation below which the network is skipped
     float mlpB3 = 0.0f;
     unsigned long long *hybCounts = nullptr;
@@ -1496,7 +1477,6 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
         hy.b3 = dev->mlpB3;
         hy.threshold = dev->mlpThreshold;
         hy.counts = dev->hybCounts;
-        hy.everyStage = dev->everyStage;
         hy.smoothVariation = dev->smoothVariation;
     }
     int slot;
@@ -1978,8 +1958,9 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
 
 int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation)
 {
+    if (every_stage != 0)
+        return Fail("the indicator classifies once per directional operator; per-stage classification is not provided");
     if (!(smooth_variation >= 0.0f)) return Fail("the smooth-variation gate must be non-negative");
-    dev->everyStage = (every_stage != 0);
     dev->smoothVariation = smooth_variation;
     return 0;
 }
@@ -2429,7 +2410,6 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
     b->hy.b3 = params[ARTRA_MLP_B3];
     b->hy.threshold = threshold;
     b->hy.counts = b->counts;
-    b->hy.everyStage = 0;
     b->hy.smoothVariation = 1.0e-3f;
     b->hybrid = 1;
     return 0;

@@ -157,7 +157,7 @@ def classification_report(params, features, labels, tau=TAU, logit_threshold=LOG
             "false_troubled": float((~truth & pred).sum() / max((~truth).sum(), 1))}
 
 
-def evaluate(case_directory, params, thresholds, max_steps=1 << 20, every_stage=False):
+def evaluate(case_directory, params, thresholds, max_steps=1 << 20):
     """Plain WENO5 and routed runs of one case to its end time: per logit
     threshold the relative L1 density difference, the linear fraction of
     the faces and both run times (s)."""
@@ -167,7 +167,7 @@ def evaluate(case_directory, params, thresholds, max_steps=1 << 20, every_stage=
         case = lib.Case(case_directory)
         solver = lib.Solver(case)
         if p is not None:
-            solver.set_indicator(p, thr, every_stage=every_stage)
+            solver.set_indicator(p, thr)
         solver.hybrid_counts(reset=True)
         solver.synchronize()
         t0 = time.perf_counter()

@@ -323,12 +323,12 @@ class Solver:
         _check(library().artra_hip_sample_faces(self._h, axis, _fptr(feat), _fptr(lab)))
         return feat, lab
 
-    def set_indicator(self, params, logit_threshold=0.0, every_stage=False, smooth_variation=1e-3):
-        """Route WENO5 faces with the learned indicator (None: plain WENO5);
-        every_stage classifies at every RK stage instead of once per
-        directional operator; faces varying less than smooth_variation
-        (relative) take the linear flux without the network (0: never)."""
-        _check(library().artra_hip_set_indicator_options(self._h, 1 if every_stage else 0, float(smooth_variation)))
+    def set_indicator(self, params, logit_threshold=0.0, smooth_variation=1e-3):
+        """Route WENO5 faces with the learned indicator (None: plain WENO5),
+        classified once per directional operator; faces varying less than
+        smooth_variation (relative) take the linear flux without the network
+        (0: never)."""
+        _check(library().artra_hip_set_indicator_options(self._h, 0, float(smooth_variation)))
         if params is None:
             _check(library().artra_hip_set_indicator(self._h, None, 0.0))
             return

@@ -189,11 +189,12 @@ int artra_hip_sample_faces(ArtraHipSolver *solver, int axis, float *features, fl
  * restores plain WENO5.  The indicator runs inside the fused sweeps, so
  * every swept axis must qualify for them (see artra_hip_set_fused). */
 int artra_hip_set_indicator(ArtraHipSolver *solver, const float *params, float logit_threshold);
-/* When and where the indicator runs.  every_stage = 0 (default) classifies
- * the faces at the first RK stage of each directional operator and keeps
- * that set of troubled faces for its later stages (the solution moves less
- * than a cell within one operator); every_stage = 1 classifies at every
- * stage.  Faces whose density and pressure stencils vary by at most
+/* When and where the indicator runs.  It classifies the faces at the first
+ * RK stage of each directional operator and keeps that set of troubled
+ * faces for its later stages (the solution moves less than a cell within
+ * one operator); every_stage must be 0 (per-stage classification is not
+ * provided: it would hold the network's registers through the flux
+ * passes).  Faces whose density and pressure stencils vary by at most
  * smooth_variation (relative to the face average; default 1e-3, 0 turns
  * the gate off) take the linear flux without the network, and a wavefront
  * of such faces skips the network altogether. */
