@@ -244,3 +244,32 @@ def test_trained_indicator_with_roe_and_steger_warming(model, tmp_path):
     print("rel L1 rho", l1, "linear fraction", lin / seen)
     assert l1 <= 1e-3
     assert lin / seen >= 0.5
+
+
+def test_trained_network_gains_over_routing_without_it():
+    """What the trained network adds, on the 64^3 shock-vortex run to
+    t = 0.6: without a network the choice is between the smoothness gate
+    alone (near-uniform faces linear, every other face WENO5) and the linear
+    flux everywhere.  The network sends far more faces to the linear flux
+    than the gate while staying within 1e-3 relative L1 of plain WENO5; the
+    linear flux everywhere does not stay within it."""
+    from artracfd_amd import indicator
+    params, meta = indicator.load()
+    thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
+
+    def l1(U, Up):
+        if not np.all(np.isfinite(U[0])):
+            return np.inf
+        return float(np.abs(U[0] - Up[0]).sum() / np.abs(Up[0]).sum())
+
+    print("gain over routing without the network, shockvortex64", flush=True)
+    Up, _, _ = _evolve("shockvortex64", 1 << 20)
+    Un, _, (ln, sn) = _evolve("shockvortex64", 1 << 20, params, thr)
+    Ug, _, (lg, sg) = _evolve("shockvortex64", 1 << 20, params, -1e30)      # gate only
+    Ua, _, (la, sa) = _evolve("shockvortex64", 1 << 20, params, 1e30, gate=0.0)  # all linear
+    net, gate, lin = (l1(Un, Up), ln / sn), (l1(Ug, Up), lg / sg), (l1(Ua, Up), la / sa)
+    print("network", net, "gate only", gate, "all linear", lin)
+    assert net[0] <= 1e-3
+    assert lin[1] == 1.0 and lin[0] > 1e-3
+    assert gate[0] <= 1e-3
+    assert net[1] >= gate[1] + 0.1
