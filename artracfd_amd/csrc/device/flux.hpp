@@ -294,7 +294,10 @@ __host__ __device__ __forceinline__ void viscous_flux(const FluxModel &mdl, cons
 namespace artra {
 namespace fast {
 
-__host__ __device__ __forceinline__ double weno5(double f0, double f1, double f2, double f3, double f4)
+// WENO5 of one split side as the numerator and denominator of
+// (t0 Q0 + t1 Q1 + t2 Q2) / (6 (t0 + t1 + t2)), t_k = C_k prod_{j != k} (eps + IS_j)^2.
+__host__ __device__ __forceinline__ void weno5_parts(double f0, double f1, double f2, double f3, double f4, double &num,
+                                                     double &den)
 {
     const double eps = 1.0e-6;
     const double a0 = fma(-2.0, f1, f0 + f2), b0 = fma(3.0, f2, fma(-4.0, f1, f0));
@@ -309,8 +312,29 @@ __host__ __device__ __forceinline__ double weno5(double f0, double f1, double f2
     const double Q0 = fma(2.0, f0, fma(-7.0, f1, 11.0 * f2));
     const double Q1 = fma(5.0, f2, fma(2.0, f3, -f1));
     const double Q2 = fma(2.0, f2, fma(5.0, f3, -f4));
-    const double num = fma(t0, Q0, fma(t1, Q1, t2 * Q2));
-    return num / (6.0 * (t0 + t1 + t2));
+    num = fma(t0, Q0, fma(t1, Q1, t2 * Q2));
+    den = 6.0 * (t0 + t1 + t2);
+}
+
+__host__ __device__ __forceinline__ double weno5(double f0, double f1, double f2, double f3, double f4)
+{
+    double num, den;
+    weno5_parts(f0, f1, f2, f3, f4, num, den);
+    return num / den;
+}
+
+// The sum of the two split sides of one characteristic field over one
+// common denominator, n+/d+ + n-/d- = (n+ d- + n- d+) / (d+ d-): one
+// division instead of two.  d+ and d- are at least 6e-25 (every eps + IS is
+// at least 1e-6), so the product cannot underflow; it would overflow only
+// for smoothness indicators above about 1e38.
+__host__ __device__ __forceinline__ double weno5_sum(double p0, double p1, double p2, double p3, double p4, double m0,
+                                                     double m1, double m2, double m3, double m4)
+{
+    double np, dp, nm, dm;
+    weno5_parts(p0, p1, p2, p3, p4, np, dp);
+    weno5_parts(m0, m1, m2, m3, m4, nm, dm);
+    return fma(np, dm, nm * dp) / (dp * dm);
 }
 
 __host__ __device__ __forceinline__ double weno3(double f0, double f1, double f2)
@@ -395,13 +419,13 @@ __host__ __device__ __forceinline__ void convective_flux(const FluxModel &mdl, c
     for (int r = 0; r < 5; ++r) {
         double hp, hn;
         if (NS == 6) {
-            hp = weno5(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2], lp[r] * W[r][3], lp[r] * W[r][4]);
-            hn = weno5(ln[r] * W[r][5], ln[r] * W[r][4], ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
+            h[r] = weno5_sum(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2], lp[r] * W[r][3], lp[r] * W[r][4],
+                             ln[r] * W[r][5], ln[r] * W[r][4], ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
         } else {
             hp = weno3(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2]);
             hn = weno3(ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
+            h[r] = hp + hn;
         }
-        h[r] = hp + hn;
     }
     // F = R h with the right eigenvectors along S
     const double ha = h[0] + h[4], hd = h[4] - h[0];

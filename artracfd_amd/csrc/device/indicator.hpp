@@ -333,16 +333,14 @@ __host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const 
     linear_flux_from<S>(fs, Up, Un, F);
 }
 
-// One half of one characteristic field of the WENO5 face flux: side 0 gives
-// WENO5(lp W_r) over the nodes f-2 .. f+2, side 1 WENO5(ln W_r) over the
-// mirrored nodes f+3 .. f-1, for a line held as [5][n] in memory; h_r is
-// their sum.  Ten independent items per face keep a short list of troubled
-// faces busy on many lanes; the operations are those of
-// fast::convective_flux, so h_r is bit-identical to it (fs needs
-// face_scales applied).
+// One characteristic field of the WENO5 face flux: h_r = WENO5(lp W_r) over
+// the nodes f-2 .. f+2 plus WENO5(ln W_r) over the mirrored nodes
+// f+3 .. f-1, for a line held as [5][n] in memory.  Five independent items
+// per face keep a short list of troubled faces busy on many lanes; the
+// operations are those of fast::convective_flux, so h_r is bit-identical
+// to it (fs needs face_scales applied).
 template <int S>
-__device__ __forceinline__ double weno_half(const FaceState &fs, const double *st, int n, int f, int r, int side,
-                                            bool sw)
+__device__ __forceinline__ double weno_field(const FaceState &fs, const double *st, int n, int f, int r, bool sw)
 {
     const double vel[3] = {fs.u, fs.v, fs.w};
     const double m2b = -2.0 * fs.b;
@@ -362,20 +360,20 @@ __device__ __forceinline__ double weno_half(const FaceState &fs, const double *s
         lp = 0.5 * (lam + star);
         ln = 0.5 * (lam - star);
     }
-    const double sc = side ? ln : lp;
-    double g[5];
+    double w[6];                                    // the field's value at nodes f-2 .. f+3
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        const int node = side ? f + 3 - m : f - 2 + m;
+    for (int m = 0; m < 6; ++m) {
+        const int node = f - 2 + m;
         const double U0 = st[node], U1 = st[n + node], U2 = st[2 * n + node], U3 = st[3 * n + node],
                      U4 = st[4 * n + node];
         const double K = fma(fs.q, U0, U4) - fma(fs.u, U1, fma(fs.v, U2, fs.w * U3));
         const double M = fma(fs.un, U0, -st[(1 + S) * n + node]);
         const double bK = fs.b * K, dM = fs.d * M;
         const double Ui = st[idx * n + node];
-        g[m] = sc * (edge ? bK + sgn * dM : fma(cK, K, Ui));
+        w[m] = edge ? bK + sgn * dM : fma(cK, K, Ui);
     }
-    return fast::weno5(g[0], g[1], g[2], g[3], g[4]);
+    return fast::weno5_sum(lp * w[0], lp * w[1], lp * w[2], lp * w[3], lp * w[4], ln * w[5], ln * w[4], ln * w[3],
+                           ln * w[2], ln * w[1]);
 }
 
 // Full WENO5 flux from the same face state (matches fast::convective_flux).
@@ -397,8 +395,8 @@ __host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
         const double lp = fs.lp[r], ln = fs.ln[r];
-        h[r] = fast::weno5(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3], lp * W[r][4]) +
-               fast::weno5(ln * W[r][5], ln * W[r][4], ln * W[r][3], ln * W[r][2], ln * W[r][1]);
+        h[r] = fast::weno5_sum(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3], lp * W[r][4], ln * W[r][5],
+                               ln * W[r][4], ln * W[r][3], ln * W[r][2], ln * W[r][1]);
         if (label) {
             dev = fmax(dev, weno5_weight_deviation(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3],
                                                    lp * W[r][4]));
