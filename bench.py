@@ -51,6 +51,31 @@ def fluid_nodes(case):
     return int((case.interior(flags[0]) == 0).sum())
 
 
+WORKLOADS = {
+    "shockvortex256": "shock-vortex interaction",
+    "shockvortex128": "shock-vortex interaction",
+    "shockvortex64": "shock-vortex interaction",
+    "tgv128": "viscous Taylor-Green vortex",
+    "cylinder": "shock diffraction over a cylinder (immersed boundary)",
+    "sod": "Sod shock tube",
+}
+
+
+def describe(case_dir, name, cells):
+    """Workload line: the case, its size and its scheme as its case file states it."""
+    lines = [ln.split("#")[0].strip() for ln in open(os.path.join(case_dir, "artracfd.case"))]
+    lines = [ln for ln in lines if ln]
+    num = lines[lines.index("numerical begin") + 1:]
+    mat = lines[lines.index("material begin") + 1:]
+    rk = "RK3" if num[0] == "1" else "RK2"
+    weno = "WENO5" if num[1] == "1" else "WENO3"
+    split = "split" if num[2] == "0" else "unsplit"
+    flux = ("Roe-" if num[3] == "1" else "") + ("SW" if num[4] == "1" else "LLF")
+    visc = "viscous" if float(mat[1]) != 0.0 else "inviscid"
+    what = WORKLOADS.get(name, "case")
+    return f"{name}: {what}, {cells} interior fluid nodes, {weno}-{flux}-{rk} {split}, {visc}, plain FP64 path"
+
+
 def stage_bytes(tscheme):
     # RK3: stage 1 reads U_tn and writes U_tm; stages 2 and 3 also read U_to
     return (80.0 + 120.0 + 120.0) / 3.0 if tscheme == 1 else (80.0 + 120.0) / 2.0
@@ -271,8 +296,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.case}: shock-vortex interaction, {cells} interior fluid nodes, WENO5-LLF-RK3 "
-                               f"split, inviscid, plain FP64 path", "parallelism": "single GPU",
+        "config": {"workload": describe(case_dir, args.case, cells), "parallelism": "single GPU",
                    "fused_axes": fused_axes},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
