@@ -192,6 +192,21 @@ def test_all_troubled_routing_is_plain_weno5(model, tmp_path):
     assert np.array_equal(U1, U0)
 
 
+def test_queued_faces_are_plain_weno5(tmp_path):
+    """Lines of 12 interior nodes have 13 faces, within the 16-face queue,
+    so with every face troubled each face goes through the queue's per-field
+    WENO5 items (no line overflows to the plain flux): a 2-D Riemann problem
+    must then match the plain sweeps bit for bit."""
+    from artracfd_amd import indicator
+    d = indicator.riemann_case(str(tmp_path / "riemann12"), np.random.default_rng(7), n=12, steps=30)
+    print("hybrid all-WENO through the queue, riemann 12x12", flush=True)
+    U0, d0, _ = _evolve(d, 30)
+    U1, d1, (lin, seen) = _evolve(d, 30, _params(3), threshold=-1e30, gate=0.0)
+    assert lin == 0 and seen > 0
+    np.testing.assert_array_equal(d1, d0)
+    assert np.array_equal(U1, U0)
+
+
 def test_all_linear_on_smooth_data_is_close_to_weno5(tmp_path):
     """A small-amplitude density wave: every face routed to the linear
     fifth-order flux must agree with WENO5 (whose weights are then nearly
