@@ -551,9 +551,9 @@ __device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const d
 //
 // With HYB, the indicator classifies the faces once per operator into a
 // queue of troubled faces; each stage then runs one
-// balanced pass over F + 5 x queued work items - the linear flux of every
-// routed face and the five per-field WENO5 evaluations of every queued
-// face - a short pass projecting the queued faces' fields back, and
+// balanced pass over F + 3 x queued work items - the linear flux of every
+// routed face and the three WENO5 items (fields 0-1, 2-3, 4) of every
+// queued face - a short pass projecting the queued faces' fields back, and
 // the node update, which also refreshes the node quantities of the next
 // stage.  A line with more troubled faces than the queue holds runs the
 // plain WENO5 flux on every face for that operator.
@@ -693,8 +693,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             }
         } else {
             // one pass over the linear fluxes of the routed faces and the
-            // per-field WENO5 items of the queued ones
-            const int total = F + 5 * nq;
+            // WENO5 items (fields 0-1, 2-3, 4) of the queued ones
+            const int total = F + 3 * nq;
             for (int e = tt; e < total; e += T) {
                 if (e < F) {
                     const int q = e;
@@ -742,7 +742,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 #pragma unroll
                     for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
                 } else {
-                    const int it = e - F, qi = it / 5, k = it - 5 * qi;
+                    const int it = e - F, qi = it / 3, k = it - 3 * qi;
                     const int f = la.lo - 1 + queue[qi];
                     double UL[5], UR[5];
 #pragma unroll
@@ -753,7 +753,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                     artra::FaceState fs;
                     artra::face_state<S>(mdl, UL, UR, fs);
                     artra::face_scales(fs);
-                    hbuf[it] = artra::weno_field<S>(fs, st, n, f, k, mdl.sw != 0);
+                    artra::weno_fields<S>(fs, st, n, f, k, mdl.sw != 0, hbuf + 5 * qi);
                 }
             }
             {

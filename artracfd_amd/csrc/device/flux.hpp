@@ -328,13 +328,31 @@ __host__ __device__ __forceinline__ double weno5(double f0, double f1, double f2
 // division instead of two.  d+ and d- are at least 6e-25 (every eps + IS is
 // at least 1e-6), so the product cannot underflow; it would overflow only
 // for smoothness indicators above about 1e38.
-__host__ __device__ __forceinline__ double weno5_sum(double p0, double p1, double p2, double p3, double p4, double m0,
-                                                     double m1, double m2, double m3, double m4)
+__host__ __device__ __forceinline__ void weno5_sum_parts(double p0, double p1, double p2, double p3, double p4,
+                                                         double m0, double m1, double m2, double m3, double m4,
+                                                         double &num, double &den)
 {
     double np, dp, nm, dm;
     weno5_parts(p0, p1, p2, p3, p4, np, dp);
     weno5_parts(m0, m1, m2, m3, m4, nm, dm);
-    return fma(np, dm, nm * dp) / (dp * dm);
+    num = fma(np, dm, nm * dp);
+    den = dp * dm;
+}
+
+__host__ __device__ __forceinline__ double weno5_sum(double p0, double p1, double p2, double p3, double p4, double m0,
+                                                     double m1, double m2, double m3, double m4)
+{
+    double num, den;
+    weno5_sum_parts(p0, p1, p2, p3, p4, m0, m1, m2, m3, m4, num, den);
+    return num / den;
+}
+
+// Two quotients from one division: a/b and c/d as a (d q) and c (b q), q = 1/(b d).
+__host__ __device__ __forceinline__ void divide_pair(double a, double b, double c, double d, double &x, double &y)
+{
+    const double q = 1.0 / (b * d);
+    x = a * (d * q);
+    y = c * (b * q);
 }
 
 __host__ __device__ __forceinline__ double weno3(double f0, double f1, double f2)
@@ -354,7 +372,8 @@ __host__ __device__ __forceinline__ void convective_flux(const FluxModel &mdl, c
     const double g1 = gamma - 1.0;
     const double *UL = st[K0];
     const double *UR = st[K0 + 1];
-    const double rL = 1.0 / UL[0], rR = 1.0 / UR[0];
+    double rL, rR;
+    divide_pair(1.0, UL[0], 1.0, UR[0], rL, rR);
     const double uL = UL[1] * rL, vL = UL[2] * rL, wL = UL[3] * rL;
     const double uR = UR[1] * rR, vR = UR[2] * rR, wR = UR[3] * rR;
     const double hTL = fma(gamma, UL[4] * rL, -0.5 * g1 * fma(uL, uL, fma(vL, vL, wL * wL)));
@@ -415,15 +434,21 @@ __host__ __device__ __forceinline__ void convective_flux(const FluxModel &mdl, c
         for (int t = 0; t < 3; ++t) W[1 + t][m] = (t == S) ? fma(m2b, K, U[0]) : fma(sh[t], K, U[1 + t]);
     }
     double h[5];
+    if (NS == 6) {
+        double num[5], den[5];
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
-        double hp, hn;
-        if (NS == 6) {
-            h[r] = weno5_sum(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2], lp[r] * W[r][3], lp[r] * W[r][4],
-                             ln[r] * W[r][5], ln[r] * W[r][4], ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
-        } else {
-            hp = weno3(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2]);
-            hn = weno3(ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
+        for (int r = 0; r < 5; ++r)
+            weno5_sum_parts(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2], lp[r] * W[r][3], lp[r] * W[r][4],
+                            ln[r] * W[r][5], ln[r] * W[r][4], ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1],
+                            num[r], den[r]);
+        divide_pair(num[0], den[0], num[1], den[1], h[0], h[1]);
+        divide_pair(num[2], den[2], num[3], den[3], h[2], h[3]);
+        h[4] = num[4] / den[4];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const double hp = weno3(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2]);
+            const double hn = weno3(ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1]);
             h[r] = hp + hn;
         }
     }

@@ -143,7 +143,8 @@ template <int S>
 __host__ __device__ __forceinline__ void face_state(const FluxModel &mdl, const double UL[5], const double UR[5], FaceState &fs)
 {
     const double gamma = mdl.gamma, g1 = gamma - 1.0;
-    const double rL = 1.0 / UL[0], rR = 1.0 / UR[0];
+    double rL, rR;
+    fast::divide_pair(1.0, UL[0], 1.0, UR[0], rL, rR);
     const double uL = UL[1] * rL, vL = UL[2] * rL, wL = UL[3] * rL;
     const double uR = UR[1] * rR, vR = UR[2] * rR, wR = UR[3] * rR;
     const double hTL = fma(gamma, UL[4] * rL, -0.5 * g1 * fma(uL, uL, fma(vL, vL, wL * wL)));
@@ -333,14 +334,13 @@ __host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const 
     linear_flux_from<S>(fs, Up, Un, F);
 }
 
-// One characteristic field of the WENO5 face flux: h_r = WENO5(lp W_r) over
-// the nodes f-2 .. f+2 plus WENO5(ln W_r) over the mirrored nodes
-// f+3 .. f-1, for a line held as [5][n] in memory.  Five independent items
-// per face keep a short list of troubled faces busy on many lanes; the
-// operations are those of fast::convective_flux, so h_r is bit-identical
-// to it (fs needs face_scales applied).
+// The WENO5 sum of one characteristic field of the face flux, h_r =
+// WENO5(lp W_r) over the nodes f-2 .. f+2 plus WENO5(ln W_r) over the
+// mirrored nodes f+3 .. f-1 (for a line held as [5][n] in memory), as a
+// numerator and denominator (fs needs face_scales applied).
 template <int S>
-__device__ __forceinline__ double weno_field(const FaceState &fs, const double *st, int n, int f, int r, bool sw)
+__device__ __forceinline__ void weno_field_parts(const FaceState &fs, const double *st, int n, int f, int r, bool sw,
+                                                 double &num, double &den)
 {
     const double vel[3] = {fs.u, fs.v, fs.w};
     const double m2b = -2.0 * fs.b;
@@ -372,8 +372,27 @@ __device__ __forceinline__ double weno_field(const FaceState &fs, const double *
         const double Ui = st[idx * n + node];
         w[m] = edge ? bK + sgn * dM : fma(cK, K, Ui);
     }
-    return fast::weno5_sum(lp * w[0], lp * w[1], lp * w[2], lp * w[3], lp * w[4], ln * w[5], ln * w[4], ln * w[3],
-                           ln * w[2], ln * w[1]);
+    fast::weno5_sum_parts(lp * w[0], lp * w[1], lp * w[2], lp * w[3], lp * w[4], ln * w[5], ln * w[4], ln * w[3],
+                          ln * w[2], ln * w[1], num, den);
+}
+
+// Work item g (0, 1, 2) of a queued face: the fields 0-1, 2-3 or 4, with
+// the divisions fast::convective_flux makes for them (fields 0 and 1, and 2
+// and 3, share one), so that h is bit-identical to it.  Three independent
+// items per face keep a short list of troubled faces busy on many lanes.
+template <int S>
+__device__ __forceinline__ void weno_fields(const FaceState &fs, const double *st, int n, int f, int g, bool sw,
+                                            double *h)
+{
+    double n0, d0;
+    weno_field_parts<S>(fs, st, n, f, 2 * g, sw, n0, d0);
+    if (g == 2) {
+        h[4] = n0 / d0;
+        return;
+    }
+    double n1, d1;
+    weno_field_parts<S>(fs, st, n, f, 2 * g + 1, sw, n1, d1);
+    fast::divide_pair(n0, d0, n1, d1, h[2 * g], h[2 * g + 1]);
 }
 
 // Full WENO5 flux from the same face state (matches fast::convective_flux).
@@ -390,13 +409,13 @@ __host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st
 #pragma unroll
         for (int r = 0; r < 5; ++r) W[r][m] = t[r];
     }
-    double h[5];
+    double h[5], num[5], den[5];
     double dev = 0.0;
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
         const double lp = fs.lp[r], ln = fs.ln[r];
-        h[r] = fast::weno5_sum(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3], lp * W[r][4], ln * W[r][5],
-                               ln * W[r][4], ln * W[r][3], ln * W[r][2], ln * W[r][1]);
+        fast::weno5_sum_parts(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3], lp * W[r][4], ln * W[r][5],
+                              ln * W[r][4], ln * W[r][3], ln * W[r][2], ln * W[r][1], num[r], den[r]);
         if (label) {
             dev = fmax(dev, weno5_weight_deviation(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3],
                                                    lp * W[r][4]));
@@ -405,6 +424,9 @@ __host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st
         }
     }
     if (label) *label = dev;
+    fast::divide_pair(num[0], den[0], num[1], den[1], h[0], h[1]);
+    fast::divide_pair(num[2], den[2], num[3], den[3], h[2], h[3]);
+    h[4] = num[4] / den[4];
     project_back<S>(fs, h, F);
 }
 
