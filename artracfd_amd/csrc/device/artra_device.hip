@@ -514,16 +514,16 @@ __host__ __device__ __forceinline__ int LineQueueCap(int F)
 
 // LDS of one line: the states [5][n] and face fluxes [5][F]; for the
 // indicator also the velocities and total enthalpy of every node [4][n],
-// the per-field WENO5 results of the queued faces [cap][5], the queue
-// [cap], each face's queue position [F], the queue length, the line's fill
-// rules [2][kLineRules] and the FP32 density and pressure of every node
-// [2][n]; with u0 (the indicator's
-// sweeps) then the operator's initial state of the interior nodes [5][F - 1],
-// which the plain sweeps keep in registers.
+// the per-field WENO5 numerators and denominators of the queued faces
+// [cap][5][2], the queue [cap], each face's queue position [F], the queue
+// length, the line's fill rules [2][kLineRules] and the FP32 density and
+// pressure of every node [2][n]; with u0 (the indicator's sweeps) then the
+// operator's initial state of the interior nodes [5][F - 1], which the
+// plain sweeps keep in registers.
 __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool u0)
 {
     const int cap = LineQueueCap(F);
-    const size_t base = ((size_t)(9 * n + 5 * F + 5 * cap) * sizeof(double) +
+    const size_t base = ((size_t)(9 * n + 5 * F + 10 * cap) * sizeof(double) +
                          (size_t)(cap + F + 4 + 2 * kLineRules) * sizeof(int) +
                          (size_t)2 * n * sizeof(float) + 15) / 16 * 16;
     return base + (u0 ? (size_t)5 * (F - 1) * sizeof(double) : 0);
@@ -551,9 +551,10 @@ __device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const d
 //
 // With HYB, the indicator classifies the faces once per operator into a
 // queue of troubled faces; each stage then runs one
-// balanced pass over F + 3 x queued work items - the linear flux of every
-// routed face and the three WENO5 items (fields 0-1, 2-3, 4) of every
-// queued face - a short pass projecting the queued faces' fields back, and
+// balanced pass over F + 5 x queued work items - the linear flux of every
+// routed face and the five per-field WENO5 items (numerator and denominator
+// of each field's sum) of every queued face - a short pass dividing and
+// projecting the queued faces' fields back, and
 // the node update, which also refreshes the node quantities of the next
 // stage.  A line with more troubled faces than the queue holds runs the
 // plain WENO5 flux on every face for that operator.
@@ -567,8 +568,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     double *st = (double *)mem;                     // [5][n]
     double *fb = st + 5 * n;                        // [5][F]
     double *nodeV = fb + 5 * F;                     // [4][n]
-    double *hbuf = nodeV + 4 * n;                   // [cap][5]
-    int *queue = (int *)(hbuf + 5 * cap);           // [cap]
+    double *hbuf = nodeV + 4 * n;                   // [cap][5 fields][numerator, denominator]
+    int *queue = (int *)(hbuf + 10 * cap);          // [cap]
     int *fidx = queue + cap;                        // [F]
     int *queued = fidx + F;
     int *rules = queued + 4;                        // [2][kLineRules]: destinations, sources
@@ -693,8 +694,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             }
         } else {
             // one pass over the linear fluxes of the routed faces and the
-            // WENO5 items (fields 0-1, 2-3, 4) of the queued ones
-            const int total = F + 3 * nq;
+            // per-field WENO5 items of the queued ones
+            const int total = F + 5 * nq;
             for (int e = tt; e < total; e += T) {
                 if (e < F) {
                     const int q = e;
@@ -742,7 +743,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 #pragma unroll
                     for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
                 } else {
-                    const int it = e - F, qi = it / 3, k = it - 3 * qi;
+                    const int it = e - F, qi = it / 5, k = it - 5 * qi;
                     const int f = la.lo - 1 + queue[qi];
                     double UL[5], UR[5];
 #pragma unroll
@@ -753,7 +754,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                     artra::FaceState fs;
                     artra::face_state<S>(mdl, UL, UR, fs);
                     artra::face_scales(fs);
-                    artra::weno_fields<S>(fs, st, n, f, k, mdl.sw != 0, hbuf + 5 * qi);
+                    artra::weno_field_parts<S>(fs, st, n, f, k, mdl.sw != 0, hbuf[10 * qi + 2 * k],
+                                               hbuf[10 * qi + 2 * k + 1]);
                 }
             }
             {
@@ -767,8 +769,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                     for (int c = 0; c < 5; ++c) {
                         UL[c] = st[c * n + f];
                         UR[c] = st[c * n + f + 1];
-                        h[c] = hbuf[5 * e + c];
                     }
+                    artra::fields_from_parts(hbuf + 10 * e, h);
                     artra::FaceState fs;
                     artra::face_state<S>(mdl, UL, UR, fs);
                     artra::project_back<S>(fs, h, Fx);

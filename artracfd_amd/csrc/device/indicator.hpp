@@ -376,23 +376,15 @@ __device__ __forceinline__ void weno_field_parts(const FaceState &fs, const doub
                           ln * w[2], ln * w[1], num, den);
 }
 
-// Work item g (0, 1, 2) of a queued face: the fields 0-1, 2-3 or 4, with
-// the divisions fast::convective_flux makes for them (fields 0 and 1, and 2
-// and 3, share one), so that h is bit-identical to it.  Three independent
-// items per face keep a short list of troubled faces busy on many lanes.
-template <int S>
-__device__ __forceinline__ void weno_fields(const FaceState &fs, const double *st, int n, int f, int g, bool sw,
-                                            double *h)
+// The field sums of a queued face from their numerators and denominators
+// num/den[r] = parts[2 r], parts[2 r + 1], with the divisions
+// fast::convective_flux makes (fields 0 and 1, and 2 and 3, share one), so
+// that h is bit-identical to it.
+__device__ __forceinline__ void fields_from_parts(const double *parts, double h[5])
 {
-    double n0, d0;
-    weno_field_parts<S>(fs, st, n, f, 2 * g, sw, n0, d0);
-    if (g == 2) {
-        h[4] = n0 / d0;
-        return;
-    }
-    double n1, d1;
-    weno_field_parts<S>(fs, st, n, f, 2 * g + 1, sw, n1, d1);
-    fast::divide_pair(n0, d0, n1, d1, h[2 * g], h[2 * g + 1]);
+    fast::divide_pair(parts[0], parts[1], parts[2], parts[3], h[0], h[1]);
+    fast::divide_pair(parts[4], parts[5], parts[6], parts[7], h[2], h[3]);
+    h[4] = parts[8] / parts[9];
 }
 
 // Full WENO5 flux from the same face state (matches fast::convective_flux).

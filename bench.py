@@ -40,7 +40,7 @@ FP64_PEAK_TFLOPS = 78.6
 # WRITE_SIZE, scale factors from a 1 GiB copy kernel) and FP64 flops per
 # cell-stage of the plain WENO5 flux and update (FMA = 2).
 TRAFFIC_B_PER_CELL_STAGE = 27.8
-FP64_FLOP_PER_CELL_STAGE = 1072.0
+FP64_FLOP_PER_CELL_STAGE = 1044.0
 # the stage-by-stage sweeps (profiles/sweep_shockvortex256.md)
 UNFUSED_TRAFFIC_B_PER_CELL_STAGE = 129.6
 METRIC = "Mcell-stage updates/s (256³ WENO5-RK3) + learned-hybrid vs plain at ≤1e-3 L1"
