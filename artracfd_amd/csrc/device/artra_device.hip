@@ -465,7 +465,6 @@ constexpr int kLineRules = 12;
 constexpr int kCountSlots = 256;
 constexpr int kCountStride = 16;                 // unsigned long long per slot
 constexpr size_t kCountBytes = (size_t)kCountSlots * kCountStride * sizeof(unsigned long long);
-constexpr int kLineMaxK = 2;        // interior nodes per thread (K * blockDim >= interior)
 
 struct LineArgs {
     double *U;                      // level TO: read at the start, written at the end
