@@ -252,7 +252,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU slab decomposition is a "
-                                   "design (see DESIGN.md), so --gpus must be 1"}))
+                                   "design that is not built, so --gpus must be 1"}))
         sys.exit(2)
     from artracfd_amd import Case, Solver
     if args.batched_only:
