@@ -4,7 +4,7 @@
  * This file is part of the parity oracle, never of the product.  It is
  * linked against objects compiled from the reference sources where they lie
  * (see oracle/Makefile), and only in a container that holds those sources.
- * It replaces the reference's main() (main.c:24-39) with a driver that
+ * It replaces the reference's main() (main.c:24-39) with a main program that
  *   - runs Preprocess (preprocess.c:28-42) and InitializeComputeDomain
  *     (initialization.c:38-56) on the case in the working directory,
  *   - advances with the reference's own ComputeTimeStep (solve.c:108-153) and
