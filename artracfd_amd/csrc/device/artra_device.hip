@@ -480,8 +480,7 @@ struct LineArgs {
     double r;                       // dt / ds
     const double *dt;               // when set, the step's dt on the device (r = *dt * dd)
     double dd;                      // 1 / ds
-    double gamma, gasR;
-    unsigned long long *cfl;        // when set, per-axis maximum speeds of the result into slot b mod kCountSlots
+    double gamma;
     int nrules;
     int rdst[kLineRules], rsrc[kLineRules];   // src >= 0: line position, src < 0: constant state -1 - src
     double given[2][6];             // primitive states of constant (inflow) rules
@@ -818,40 +817,6 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         }
     }
     if (!write) return;
-    if (la.cfl) {
-        // the next step's CFL maxima from the operator's result, exactly as
-        // cfl_max computes them (solve.c:108-153), one atomic per wavefront
-        double m[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (!fl[k]) continue;
-            const int i = la.lo + tt + T * k;
-            double Un[5], Uo[6];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) Un[c] = st[c * n + i];
-            Primitive(la.gamma, la.gasR, Un, Uo);
-            const double c = sqrt(la.gamma * la.gasR * Uo[5]);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const double V = fabs(Uo[a + 1]) + c;
-                if (m[a] < V) m[a] = V;
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                const double w = __shfl_xor(m[a], o);
-                if (m[a] < w) m[a] = w;
-            }
-        }
-        if (__lane_id() == 0) {
-            unsigned long long *slot = la.cfl + (blockIdx.x % kCountSlots) * kCountStride;
-            // speeds are >= 0, so their bit patterns order like the values
-#pragma unroll
-            for (int a = 0; a < 3; ++a) atomicMax(slot + a, (unsigned long long)__double_as_longlong(m[a]));
-        }
-    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         if (!fl[k]) continue;
@@ -1224,10 +1189,6 @@ struct ArtraDevice {
     float smoothVariation = 1.0e-3f; // relative stencil variation below which the network is skipped
     float mlpB3 = 0.0f;
     unsigned long long *hybCounts = nullptr;
-    // CFL maxima from the last fused operator of a step (kCountSlots slots),
-    // valid for the next max-speed query when cflReady
-    unsigned long long *cflSlots = nullptr;
-    int cflPending = 0, cflReady = 0;
     float *mlpBias = nullptr;       // b1, b2, w3 contiguous (96 FP32)
     // fused directional sweeps (rk_line)
     int fused = 1;                  // allowed by the caller
@@ -1502,8 +1463,6 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
     la.dt = nullptr;
     la.dd = dev->par.dd[p];
     la.gamma = dev->par.gamma;
-    la.gasR = dev->par.gasR;
-    la.cfl = dev->cflPending ? dev->cflSlots : nullptr;
     const ArtraDevice::LineRules &R = dev->lines[p];
     la.nrules = R.count;
     for (int e = 0; e < R.count; ++e) {
@@ -1662,7 +1621,6 @@ int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevic
     if (e == hipSuccess) e = hipMalloc(&dev->U, fieldBytes);
     if (e == hipSuccess) e = hipMalloc(&dev->fluid, (size_t)g.N);
     if (e == hipSuccess) e = hipMalloc(&dev->vmax, 4 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&dev->cflSlots, kCountBytes);
     if (e == hipSuccess) e = hipMemcpy(dev->fluid, fluid, (size_t)g.N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         artra_dev_destroy(dev);
@@ -1711,7 +1669,6 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->mlpW1);
     hipFree(dev->mlpW2);
     hipFree(dev->hybCounts);
-    hipFree(dev->cflSlots);
     hipFree(dev->mlpBias);
     if (dev->stream) hipStreamDestroy(dev->stream);
     delete dev;
@@ -1825,7 +1782,6 @@ int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *t)
 
 int artra_dev_upload(ArtraDevice *dev, int level, const double *soa)
 {
-    dev->cflReady = 0;
     HIP_TRY(hipMemcpyAsync(Level(dev, level), soa, (size_t)5 * dev->g.N * sizeof(double), hipMemcpyHostToDevice,
                            dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
@@ -1842,7 +1798,6 @@ int artra_dev_download(ArtraDevice *dev, int level, double *soa)
 
 int artra_dev_poison(ArtraDevice *dev)
 {
-    dev->cflReady = 0;
     HIP_TRY(hipMemsetAsync(dev->U, 1, (size_t)3 * 5 * dev->g.N * sizeof(double), dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     return 0;
@@ -1856,23 +1811,6 @@ int artra_dev_treat_boundary(ArtraDevice *dev, int level)
 
 int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 {
-    if (dev->cflReady) {
-        // maxima computed by the last fused operator of the step just taken
-        dev->cflReady = 0;
-        std::vector<unsigned long long> h((size_t)kCountSlots * kCountStride);
-        HIP_TRY(hipMemcpyAsync(h.data(), dev->cflSlots, kCountBytes, hipMemcpyDeviceToHost, dev->stream));
-        HIP_TRY(hipStreamSynchronize(dev->stream));
-        for (int s = 0; s < 3; ++s) {
-            double v = (floor[s] > 0.0) ? floor[s] : 0.0;
-            for (int q = 0; q < kCountSlots; ++q) {
-                double w;
-                memcpy(&w, &h[(size_t)q * kCountStride + s], sizeof w);
-                if (v < w) v = w;
-            }
-            vmax[s] = v;
-        }
-        return Harvest(dev);
-    }
     unsigned long long init[3];
     for (int s = 0; s < 3; ++s) {
         const double f = (floor[s] > 0.0) ? floor[s] : 0.0;
@@ -1911,23 +1849,8 @@ int artra_dev_evolve(ArtraDevice *dev, double dt)
                                     {-1, -1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1}, {0, 0, -1, -1, -1, -1}};
     const int c = dev->par.collapse;
     if (c < 0 || c > 8 || order[c][0] < 0) return Fail("unsupported collapse flag %d", c);
-    dev->cflReady = 0;
-    int last = 0;
-    while (last + 1 < 6 && order[c][last + 1] >= 0) ++last;
-    for (int q = 0; q <= last; ++q) {
-        const int p = order[c][q];
-        // the last operator, when fused, also yields the next step's CFL maxima
-        const bool cfl = q == last && LineEligible(dev, p, dev->hybrid != 0);
-        if (cfl) {
-            hipLaunchKernelGGL(fill_u64, dim3((kCountSlots * kCountStride + 255) / 256), dim3(256), 0, dev->stream,
-                               dev->cflSlots, kCountSlots * kCountStride, 0ull);
-            HIP_TRY(hipGetLastError());
-        }
-        dev->cflPending = cfl;
-        const int rc = Stages(dev, h, p);
-        dev->cflPending = 0;
-        if (rc) return 1;
-        dev->cflReady = cfl;
+    for (int q = 0; q < 6 && order[c][q] >= 0; ++q) {
+        if (Stages(dev, h, order[c][q])) return 1;
     }
     if (dev->timing && dev->evUsed > 4096) return Harvest(dev);
     return 0;
@@ -2506,7 +2429,6 @@ int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int rese
 // steps measured with events on the batch stream.
 int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms)
 {
-    for (int c = 0; c < b->C; ++c) b->cases[c]->cflReady = 0;     // their fields change here
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -2556,7 +2478,6 @@ int artra_dev_batch_clock(ArtraBatch *b, double *clock)
 int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_out)
 {
     if (reps < 1) reps = 1;
-    for (int c = 0; c < b->C; ++c) b->cases[c]->cflReady = 0;
     // save TO of every case into TN, restore at the end
     for (ArtraDevice *dv : b->cases) {
         HIP_TRY(hipMemcpyAsync(Level(dv, 1), Level(dv, 0), 5 * dv->g.N * sizeof(double), hipMemcpyDeviceToDevice,
