@@ -508,7 +508,7 @@ __device__ __forceinline__ long XcdLine(long b, long nb)
 // WENO5 for the operator (nothing is gained by routing then).
 __host__ __device__ __forceinline__ int LineQueueCap(int F)
 {
-    return F / 4 > 16 ? F / 4 : 16;
+    return F / 8 > 16 ? F / 8 : 16;
 }
 
 // LDS of one line: the states [5][n] and face fluxes [5][F]; for the
