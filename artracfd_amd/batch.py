@@ -30,6 +30,7 @@ def _bind():
                                                   ctypes.POINTER(vp)]),
         "artra_hip_batch_destroy": (ctypes.c_int, [vp]),
         "artra_hip_batch_tune": (ctypes.c_int, [vp, ctypes.c_int, ip, ip]),
+        "artra_hip_batch_set_threads": (ctypes.c_int, [vp, ip]),
         "artra_hip_batch_set_indicator": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float), ctypes.c_float]),
         "artra_hip_batch_counts": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_int]),
         "artra_hip_batch_march": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp]),
@@ -63,6 +64,12 @@ class Batch:
         _check(_bind().artra_hip_batch_tune(self._h, reps, threads, ctypes.byref(grid)))
         return list(threads), grid.value
 
+    def set_threads(self, threads):
+        """Tile shape per case (threads per line) instead of the tuner's choice."""
+        n = len(self.solvers)
+        t = (ctypes.c_int * n)(*[int(v) for v in threads])
+        _check(_bind().artra_hip_batch_set_threads(self._h, t))
+
     def set_indicator(self, params, logit_threshold=0.0):
         if params is None:
             _check(_bind().artra_hip_batch_set_indicator(self._h, None, 0.0))
@@ -71,9 +78,11 @@ class Batch:
         _check(_bind().artra_hip_batch_set_indicator(self._h, _fptr(p), float(logit_threshold)))
 
     def counts(self, reset=False):
-        c = (ctypes.c_ulonglong * 2)()
+        """(faces computed with the linear flux, faces classified, faces of
+        overflowed lines), as Solver.hybrid_counts."""
+        c = (ctypes.c_ulonglong * 3)()
         _check(_bind().artra_hip_batch_counts(self._h, c, 1 if reset else 0))
-        return int(c[0]), int(c[1])
+        return int(c[0]), int(c[1]), int(c[2])
 
     def march(self, steps, graph=True):
         """steps lockstep time steps of every case; returns the device milliseconds."""

@@ -261,6 +261,44 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
     }
 }
 
+// One Runge-Kutta stage of the gravity source operator (LLLU's PHI branch,
+// fluid_dynamics.c:177-214, with Phi from source_term.c:21-39): every
+// interior fluid node takes U_tm = a U_to + b (U_tn + dt Phi(U_tn)),
+// Phi = (0, rho g, rho g . u); sa.r is the stage's dt itself.  The update
+// never accumulates (the reference passes OPTSPLIT for this operator).
+template <bool FAST>
+__global__ __launch_bounds__(256) void source_stage(Grid g, StageArgs sa, double gx, double gy, double gz)
+{
+    const int ex = g.hi[0] - g.lo[0], ey = g.hi[1] - g.lo[1];
+    const long total = (long)ex * ey * (g.hi[2] - g.lo[2]);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int i = g.lo[0] + (int)(t % ex);
+    const int j = g.lo[1] + (int)((t / ex) % ey);
+    const int k = g.lo[2] + (int)(t / ((long)ex * ey));
+    const int idx = Index(g, i, j, k);
+    if (!sa.fluid[idx]) return;
+    double Un[5], Uo[5], phi[5];
+    LoadNode(sa.Utn, g.N, idx, Un);
+    if (sa.toIsTn) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) Uo[c] = Un[c];
+    } else {
+        LoadNode(sa.Uto, g.N, idx, Uo);
+    }
+    const double V[3] = {Un[1] / Un[0], Un[2] / Un[0], Un[3] / Un[0]};
+    const double fb[3] = {Un[0] * gx, Un[0] * gy, Un[0] * gz};
+    phi[0] = 0.0;
+    phi[1] = fb[0];
+    phi[2] = fb[1];
+    phi[3] = fb[2];
+    phi[4] = FAST ? fma(fb[0], V[0], fma(fb[1], V[1], fb[2] * V[2])) : fb[0] * V[0] + fb[1] * V[1] + fb[2] * V[2];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        if (FAST) sa.Utm[c * g.N + idx] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
+        else sa.Utm[c * g.N + idx] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
+    }
+}
 
 // Features and labels of every face along S of level TO (training data).
 // Face q of a line is the face between nodes lo-1+q and lo+q along S.
@@ -490,7 +528,8 @@ struct LineHybrid {
     const uint16_t *W1, *W2;        // bf16 weights
     const float *bias;              // b1[32], b2[32], w3[32]
     float b3, threshold;
-    unsigned long long *counts;     // kCountSlots slots of kCountStride: [0] faces on the linear flux, [1] faces evaluated
+    unsigned long long *counts;     // kCountSlots slots of kCountStride: [0] faces on the linear flux, [1] faces
+                                    // classified, [2] faces of overflowed lines
     float smoothVariation;          // faces varying less than this (relative) skip the network; 0: none
 };
 
@@ -511,21 +550,25 @@ __host__ __device__ __forceinline__ int LineQueueCap(int F)
     return F / 8 > 16 ? F / 8 : 16;
 }
 
-// LDS of one line: the states [5][n] and face fluxes [5][F]; for the
-// indicator also the velocities and total enthalpy of every node [4][n],
-// the per-field WENO5 numerators and denominators of the queued faces
-// [cap][5][2], the queue [cap], each face's queue position [F], the queue
-// length, the line's fill rules [2][kLineRules] and the FP32 density and
-// pressure of every node [2][n]; with u0 (the indicator's sweeps) then the
-// operator's initial state of the interior nodes [5][F - 1], which the
-// plain sweeps keep in registers.
-__host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool u0)
+// LDS of one line.  Every line: the states [5][n], the face fluxes [5][F]
+// and the fill rules [2][kLineRules].  With the indicator (hyb) also the
+// velocities and total enthalpy of every node [4][n], the per-field WENO5
+// numerators and denominators of the queued faces [cap][5][2], the
+// operator's initial state of the interior nodes [5][F - 1] (which the plain
+// sweeps keep in registers), the queue [cap], each face's queue position
+// [F], the queue length, and the FP32 density and pressure of every node
+// [2][n].
+__host__ __device__ __forceinline__ size_t LinePlainBytes(int n, int F)
+{
+    return ((size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) + 15) / 16 * 16;
+}
+
+__host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool hyb)
 {
     const int cap = LineQueueCap(F);
-    const size_t base = ((size_t)(9 * n + 5 * F + 10 * cap) * sizeof(double) +
-                         (size_t)(cap + F + 4 + 2 * kLineRules) * sizeof(int) +
-                         (size_t)2 * n * sizeof(float) + 15) / 16 * 16;
-    return base + (u0 ? (size_t)5 * (F - 1) * sizeof(double) : 0);
+    const size_t extra = (size_t)(4 * n + 10 * cap + 5 * (F - 1)) * sizeof(double) +
+                         (size_t)(cap + F + 4) * sizeof(int) + (size_t)2 * n * sizeof(float);
+    return LinePlainBytes(n, F) + (hyb ? (extra + 15) / 16 * 16 : 0);
 }
 
 // Node quantities the routed faces use: velocities and total enthalpy (FP64)
@@ -566,14 +609,14 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     const int n = la.n, F = la.hi - la.lo + 1, cap = LineQueueCap(F);
     double *st = (double *)mem;                     // [5][n]
     double *fb = st + 5 * n;                        // [5][F]
-    double *nodeV = fb + 5 * F;                     // [4][n]
+    int *rules = (int *)(fb + 5 * F);               // [2][kLineRules]: destinations, sources
+    double *nodeV = (double *)(mem + LinePlainBytes(n, F));  // [4][n] (HYB)
     double *hbuf = nodeV + 4 * n;                   // [cap][5 fields][numerator, denominator]
-    int *queue = (int *)(hbuf + 10 * cap);          // [cap]
+    double *u0 = hbuf + 10 * cap;                   // [5][F - 1]
+    int *queue = (int *)(u0 + 5 * (F - 1));         // [cap]
     int *fidx = queue + cap;                        // [F]
     int *queued = fidx + F;
-    int *rules = queued + 4;                        // [2][kLineRules]: destinations, sources
-    float *nodeRP = (float *)(rules + 2 * kLineRules); // [2][n]
-    double *u0 = (double *)(mem + LineLdsBytes(n, F, false));  // [5][F - 1] (HYB)
+    float *nodeRP = (float *)(queued + 4);          // [2][n]
     const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
     for (int p = tt; p < n; p += T) {
@@ -622,10 +665,11 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     // The indicator's pass over the faces of the line, once per directional
     // operator before its stages (the solution moves less than a cell within
     // one operator): the queue of troubled faces, each face's position in
-    // it, and whether the line routes (the queue within its capacity).
-    // Every thread runs the same trip count (whole wavefronts run the
-    // network).  The weight fragments are dead after it, so they hold no
-    // registers in the flux passes.
+    // it, and whether the line routes (its queue within capacity; decided
+    // per line, so a line's result does not depend on the lines it shares a
+    // block with).  Every thread runs the same trip count (whole wavefronts
+    // run the network).  The weight fragments are dead after it, so they
+    // hold no registers in the flux passes.
     auto classifyLine = [&](const artra::MlpWave &frags) {
         for (int m = 0; m < M; ++m) {
             const int q = tt + T * m;
@@ -642,13 +686,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 troubled = valid && !smooth && z > hy.threshold;
             }
             const unsigned long long mask = __ballot(troubled);
-            const unsigned long long seen = __ballot(valid && write);
             int slot = 0;
-            if (__lane_id() == 0) {
-                if (mask) slot = atomicAdd(queued, __popcll(mask));
-                atomicAdd(&tally[0], (unsigned long long)__popcll(seen & ~mask));
-                atomicAdd(&tally[1], (unsigned long long)__popcll(seen));
-            }
+            if (__lane_id() == 0 && mask) slot = atomicAdd(queued, __popcll(mask));
             slot = __shfl(slot, 0);
             const int pos = slot + __popcll(mask & ((1ull << __lane_id()) - 1ull));
             if (troubled && pos < cap) queue[pos] = q;
@@ -656,32 +695,24 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         }
         __syncthreads();
         nq = *queued;
-        // every line of a block takes the same path, so the barriers below match
-        routed = __syncthreads_and(nq <= cap) != 0;
+        routed = nq <= cap;
+        // tallies of the faces as they are actually computed: linear (routed
+        // lines), WENO5 because the line overflowed, and all faces
+        if (tt == 0 && write) {
+            atomicAdd(&tally[0], (unsigned long long)(routed ? F - nq : 0));
+            atomicAdd(&tally[1], (unsigned long long)F);
+            atomicAdd(&tally[2], (unsigned long long)(routed ? 0 : F));
+        }
     };
     if (HYB) classifyLine(frags);
     for (int s = 0; s < la.stages; ++s) {
-        if (!HYB) {
+        if (!HYB || !routed) {
+            // plain WENO5 on every face (with HYB: a line with too many
+            // troubled faces, as the plain sweep computes it)
             for (int m = 0; m < M; ++m) {
                 const int q = tt + T * m;
                 if (q >= F) break;
                 const int f = la.lo - 1 + q;   // face between f and f + 1
-                double w[NS][5], Fx[5];
-#pragma unroll
-                for (int j = 0; j < NS; ++j)
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - SL + j];
-                ConvFlux<S, NS, FAST>(mdl, w, Fx);
-#pragma unroll
-                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
-            }
-        } else if (!routed) {
-            // too many troubled faces on this line: WENO5 on every face, as
-            // the plain sweep computes it
-            for (int m = 0; m < M; ++m) {
-                const int q = tt + T * m;
-                if (q >= F) break;
-                const int f = la.lo - 1 + q;
                 double w[NS][5], Fx[5];
 #pragma unroll
                 for (int j = 0; j < NS; ++j)
@@ -757,25 +788,26 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                                                hbuf[10 * qi + 2 * k + 1]);
                 }
             }
-            {
-                __syncthreads();
-                // project the queued faces' fields back to fluxes
-                for (int e = tt; e < nq; e += T) {
-                    const int q = queue[e];
-                    const int f = la.lo - 1 + q;
-                    double UL[5], UR[5], h[5], Fx[5];
+        }
+        if (HYB) {
+            // every line of the block reaches this barrier, routed or not
+            __syncthreads();
+            // project the queued faces' fields back to fluxes
+            for (int e = tt; routed && e < nq; e += T) {
+                const int q = queue[e];
+                const int f = la.lo - 1 + q;
+                double UL[5], UR[5], h[5], Fx[5];
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        UL[c] = st[c * n + f];
-                        UR[c] = st[c * n + f + 1];
-                    }
-                    artra::fields_from_parts(hbuf + 10 * e, h);
-                    artra::FaceState fs;
-                    artra::face_state<S>(mdl, UL, UR, fs);
-                    artra::project_back<S>(fs, h, Fx);
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                for (int c = 0; c < 5; ++c) {
+                    UL[c] = st[c * n + f];
+                    UR[c] = st[c * n + f + 1];
                 }
+                artra::fields_from_parts(hbuf + 10 * e, h);
+                artra::FaceState fs;
+                artra::face_state<S>(mdl, UL, UR, fs);
+                artra::project_back<S>(fs, h, Fx);
+#pragma unroll
+                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
             }
         }
         __syncthreads();
@@ -832,12 +864,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : 2
                                                                                         LineHybrid hy)
 {
     extern __shared__ double lds[];
-    __shared__ unsigned long long tally[2];
+    __shared__ unsigned long long tally[3];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
     artra::MlpWave frags;
     if (HYB) {
         for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
-        if (threadIdx.x < 2) tally[threadIdx.x] = 0ull;
+        if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
         artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
     }
     const long line = XcdLine(blockIdx.x, gridDim.x);
@@ -846,6 +878,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : 2
         unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
         atomicAdd(slot, tally[0]);
         atomicAdd(slot + 1, tally[1]);
+        atomicAdd(slot + 2, tally[2]);
     }
 }
 
@@ -868,11 +901,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
                                                                                          FluxModel mdl, LineHybrid hy)
 {
     extern __shared__ double lds[];
-    __shared__ unsigned long long tally[2];
+    __shared__ unsigned long long tally[3];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
     if (HYB) {
         for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
-        if (threadIdx.x < 2) tally[threadIdx.x] = 0ull;
+        if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
     }
     for (long w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile bt = tiles[w];
@@ -884,7 +917,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         const int slot = threadIdx.x / bt.T, tt = threadIdx.x % bt.T;
         const bool write = slot < bt.count;
         const long line = bt.first + (write ? slot : bt.count - 1);
-        char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1, true);
+        char *mem = (char *)lds + (size_t)slot * LineLdsBytes(la.n, la.hi - la.lo + 1, HYB);
         LineBody<S, NS, true, HYB, 1>(la, mdl, hy, line, tt, bt.T, mem, write, tally, bias, frags);
         __syncthreads();
     }
@@ -892,6 +925,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
         atomicAdd(slot, tally[0]);
         atomicAdd(slot + 1, tally[1]);
+        atomicAdd(slot + 2, tally[2]);
     }
 }
 
@@ -1073,18 +1107,22 @@ __global__ __launch_bounds__(256) void cfl_batch(const CflChunk *chunks, int nch
     }
 }
 
+constexpr int kClock = 6;           // doubles per case in the batch clock
+
 // Time step of every case from its speeds, with the solver loop's
 // rectification to output instants and the end time (solve.c:65-78).
-// clock[c] = {now, end, rcInt, tmInt}; half[c] receives dt / 2, the step of
-// each Strang-split directional operator.
+// clock[c] = {now, end, rcInt, tmInt, stepC, stepN}; half[c] receives dt / 2,
+// the step of each Strang-split directional operator.  A case that has
+// reached its end time or its step limit stays as it is (dt = 0), and its
+// step count only advances with the case, as in the solver loop.
 __global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long *vmax, double *clock, double *dt,
                          double *half)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     const BatchCase &bc = cases[c];
-    double *ck = clock + 4 * c;
-    if (!(ck[0] < ck[1])) {         // reached its end time: the case stays as it is
+    double *ck = clock + kClock * c;
+    if (!(ck[0] < ck[1] && ck[4] < ck[5])) {      // reached its end time or step limit: the case stays as it is
         dt[c] = 0.0;
         half[c] = 0.0;
         return;
@@ -1110,6 +1148,7 @@ __global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long
     }
     ck[0] = now;
     ck[2] = rc;
+    ck[4] = ck[4] + 1.0;
     dt[c] = h;
     half[c] = 0.5 * h;
 }
@@ -1321,10 +1360,11 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
 
 // mode 0: the whole external boundary; 1 + s: only the ghosts a sweep along
 // axis s reads (valid between the stages of an inviscid split sweep).
-int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
+// The immersed-boundary ghost fill of one level, layer by layer
+// (immersed_boundary.c:270-340).
+int ImmersedFill(ArtraDevice *dev, int level)
 {
     double *U = Level(dev, level);
-    // immersed boundary first, layer by layer
     const int layers = (int)dev->layerStart.size() - 1;
     for (int r = 0; r < layers; ++r) {
         const int b = dev->layerStart[r], e = dev->layerStart[r + 1];
@@ -1336,6 +1376,14 @@ int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
         HIP_TRY(hipGetLastError());
         if (End(dev, slot)) return 1;
     }
+    return 0;
+}
+
+int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
+{
+    double *U = Level(dev, level);
+    // immersed boundary first
+    if (ImmersedFill(dev, level)) return 1;
     if (dev->gatherResolved && dev->useGather) {
         const long count = dev->gCount[mode];
         if (count <= 0) return 0;
@@ -1585,6 +1633,48 @@ int Stages(ArtraDevice *dev, double dt, int p)
     return 0;
 }
 
+// One stage of the gravity source operator on levels (to, tn) -> tm.
+int SourceStage(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int tm)
+{
+    const Grid &g = dev->g;
+    const long total = (long)(g.hi[0] - g.lo[0]) * (g.hi[1] - g.lo[1]) * (g.hi[2] - g.lo[2]);
+    if (total <= 0) return 0;
+    StageArgs sa{};
+    sa.Utn = Level(dev, tn);
+    sa.Uto = Level(dev, to);
+    sa.Utm = Level(dev, tm);
+    sa.fluid = dev->fluid;
+    sa.a = a;
+    sa.b = b;
+    sa.r = dt;
+    sa.toIsTn = (to == tn);
+    int slot;
+    if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (dev->exact) {
+        hipLaunchKernelGGL(source_stage<false>, grid, dim3(256), 0, dev->stream, g, sa, dev->par.g[0], dev->par.g[1],
+                           dev->par.g[2]);
+    } else {
+        hipLaunchKernelGGL(source_stage<true>, grid, dim3(256), 0, dev->stream, g, sa, dev->par.g[0], dev->par.g[1],
+                           dev->par.g[2]);
+    }
+    HIP_TRY(hipGetLastError());
+    return End(dev, slot);
+}
+
+// Runge-Kutta stages of the gravity source operator (fluid_dynamics.c:121-148
+// with s = PHI), each followed by the full boundary fill as in the reference.
+int SourceStages(ArtraDevice *dev, double dt)
+{
+    if (dev->par.tScheme == 0) {
+        if (SourceStage(dev, dt, 0.0, 1.0, 0, 0, 1) || TreatBoundary(dev, 1)) return 1;
+        return SourceStage(dev, dt, 1.0 / 2.0, 1.0 / 2.0, 0, 1, 0) || TreatBoundary(dev, 0);
+    }
+    if (SourceStage(dev, dt, 0.0, 1.0, 0, 0, 1) || TreatBoundary(dev, 1)) return 1;
+    if (SourceStage(dev, dt, 3.0 / 4.0, 1.0 / 4.0, 0, 1, 2) || TreatBoundary(dev, 2)) return 1;
+    return SourceStage(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0) || TreatBoundary(dev, 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1809,6 +1899,12 @@ int artra_dev_treat_boundary(ArtraDevice *dev, int level)
     return TreatBoundary(dev, level);
 }
 
+int artra_dev_treat_immersed(ArtraDevice *dev, int level)
+{
+    if (level < 0 || level > 2) return Fail("bad time level %d", level);
+    return ImmersedFill(dev, level);
+}
+
 int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 {
     unsigned long long init[3];
@@ -1840,18 +1936,21 @@ int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 int artra_dev_evolve(ArtraDevice *dev, double dt)
 {
     const double h = 0.5 * dt;
+    // gravity source half-steps around the fluid operators (fluid_dynamics.c:57-59, 112-114)
+    if (dev->par.sState && SourceStages(dev, h)) return 1;
     if (dev->par.multidim == 1) {
         if (Stages(dev, h, 3) || Stages(dev, h, 3)) return 1;
-        return 0;
+    } else {
+        static const int order[9][6] = {{2, 1, 0, 0, 1, 2},    {2, 1, 1, 2, -1, -1},   {2, 0, 0, 2, -1, -1},
+                                        {1, 0, 0, 1, -1, -1},  {-1, -1, -1, -1, -1, -1}, {2, 2, -1, -1, -1, -1},
+                                        {-1, -1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1}, {0, 0, -1, -1, -1, -1}};
+        const int c = dev->par.collapse;
+        if (c < 0 || c > 8 || order[c][0] < 0) return Fail("unsupported collapse flag %d", c);
+        for (int q = 0; q < 6 && order[c][q] >= 0; ++q) {
+            if (Stages(dev, h, order[c][q])) return 1;
+        }
     }
-    static const int order[9][6] = {{2, 1, 0, 0, 1, 2},    {2, 1, 1, 2, -1, -1},   {2, 0, 0, 2, -1, -1},
-                                    {1, 0, 0, 1, -1, -1},  {-1, -1, -1, -1, -1, -1}, {2, 2, -1, -1, -1, -1},
-                                    {-1, -1, -1, -1, -1, -1}, {1, 1, -1, -1, -1, -1}, {0, 0, -1, -1, -1, -1}};
-    const int c = dev->par.collapse;
-    if (c < 0 || c > 8 || order[c][0] < 0) return Fail("unsupported collapse flag %d", c);
-    for (int q = 0; q < 6 && order[c][q] >= 0; ++q) {
-        if (Stages(dev, h, order[c][q])) return 1;
-    }
+    if (dev->par.sState && SourceStages(dev, h)) return 1;
     if (dev->timing && dev->evUsed > 4096) return Harvest(dev);
     return 0;
 }
@@ -1982,23 +2081,21 @@ int artra_dev_fused_axes(ArtraDevice *dev)
 }
 
 // Sum of the tally slots (and their reset).
-static int SumCounts(unsigned long long *slots, unsigned long long counts[2], int reset)
+static int SumCounts(unsigned long long *slots, unsigned long long counts[3], int reset)
 {
     std::vector<unsigned long long> h((size_t)kCountSlots * kCountStride);
     HIP_TRY(hipMemcpy(h.data(), slots, kCountBytes, hipMemcpyDeviceToHost));
-    counts[0] = counts[1] = 0;
-    for (int q = 0; q < kCountSlots; ++q) {
-        counts[0] += h[(size_t)q * kCountStride];
-        counts[1] += h[(size_t)q * kCountStride + 1];
-    }
+    counts[0] = counts[1] = counts[2] = 0;
+    for (int q = 0; q < kCountSlots; ++q)
+        for (int v = 0; v < 3; ++v) counts[v] += h[(size_t)q * kCountStride + v];
     if (reset) HIP_TRY(hipMemset(slots, 0, kCountBytes));
     return 0;
 }
 
-int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset)
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[3], int reset)
 {
     if (!dev->hybCounts) {
-        counts[0] = counts[1] = 0;
+        counts[0] = counts[1] = counts[2] = 0;
         return 0;
     }
     HIP_TRY(hipStreamSynchronize(dev->stream));
@@ -2185,6 +2282,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
     for (int c = 0; c < count; ++c) {
         const ArtraDevice *dev = devs[c];
         if (dev->par.collapse != 0) return Fail("batched case %d: only three-dimensional cases are batched", c);
+        if (dev->par.sState) return Fail("batched case %d: the gravity source operator is not batched", c);
         for (int p = 0; p < 3; ++p) {
             if (!LineEligible(dev, p, true)) {
                 return Fail("batched case %d: axis %d cannot run as fused line sweeps (inviscid, split, no "
@@ -2277,7 +2375,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         hostArgs[p] = args;
     }
     if (hipMalloc(&b->vmax, 3 * count * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&b->clock, 4 * count * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b->clock, kClock * count * sizeof(double)) != hipSuccess ||
         hipMalloc(&b->dt, count * sizeof(double)) != hipSuccess ||
         hipMalloc(&b->half, count * sizeof(double)) != hipSuccess) {
         return fail(Fail("out of device memory"));
@@ -2288,7 +2386,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         if (hipMemcpy(b->dArgs[p], hostArgs[p].data(), count * sizeof(LineArgs), hipMemcpyHostToDevice) != hipSuccess)
             return fail(Fail("upload of the line arguments failed"));
     }
-    if (hipMemcpy(b->clock, clock, 4 * count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpy(b->clock, clock, kClock * count * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
         return fail(Fail("upload of the case clocks failed"));
     for (int p = 0; p < 3; ++p)
         if (BatchBuildTiles(b, p)) return fail(1);
@@ -2416,9 +2514,9 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
     return 0;
 }
 
-int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int reset)
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[3], int reset)
 {
-    counts[0] = counts[1] = 0;
+    counts[0] = counts[1] = counts[2] = 0;
     if (!b->counts) return 0;
     HIP_TRY(hipStreamSynchronize(b->stream));
     return SumCounts(b->counts, counts, reset);
@@ -2465,7 +2563,38 @@ int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms)
 int artra_dev_batch_clock(ArtraBatch *b, double *clock)
 {
     HIP_TRY(hipStreamSynchronize(b->stream));
-    HIP_TRY(hipMemcpy(clock, b->clock, 4 * b->C * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(clock, b->clock, kClock * b->C * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Tile shape of every case: threads[c] threads per line (64, 128 or 256, at
+// least the case's longest interior line, with 256 / T lines in LDS).
+int artra_dev_batch_set_threads(ArtraBatch *b, const int *threads)
+{
+    for (int c = 0; c < b->C; ++c) {
+        const Grid &g = b->cases[c]->g;
+        int longest = 0;
+        for (int p = 0; p < 3; ++p) longest = std::max(longest, g.hi[p] - g.lo[p]);
+        const int T = threads[c];
+        if ((T != 64 && T != 128 && T != 256) || T < longest)
+            return Fail("batched case %d: %d threads per line is not a tile shape for lines of %d nodes", c, T, longest);
+    }
+    const std::vector<int> saved = b->threads;
+    b->threads.assign(threads, threads + b->C);
+    for (int p = 0; p < 3; ++p) {
+        if (BatchBuildTiles(b, p)) {
+            b->threads = saved;
+            for (int q = 0; q < 3; ++q) BatchBuildTiles(b, q);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int artra_dev_batch_set_clock(ArtraBatch *b, const double *clock)
+{
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(b->clock, clock, kClock * b->C * sizeof(double), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -2483,7 +2612,7 @@ int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_ou
         HIP_TRY(hipMemcpyAsync(Level(dv, 1), Level(dv, 0), 5 * dv->g.N * sizeof(double), hipMemcpyDeviceToDevice,
                                b->stream));
     }
-    std::vector<double> clockSave(4 * b->C);
+    std::vector<double> clockSave(kClock * b->C);
     HIP_TRY(hipMemcpyAsync(clockSave.data(), b->clock, clockSave.size() * sizeof(double), hipMemcpyDeviceToHost,
                            b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));

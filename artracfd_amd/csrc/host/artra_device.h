@@ -32,6 +32,8 @@ typedef struct {
     int jacobMean;
     int fluxSplit;
     double gamma, gasR, cv, refMu, refT;
+    int sState;              /* gravity source operator on */
+    double g[DIMS];          /* gravity (nondimensional) */
     double d[DIMS], dd[DIMS];
     int typeBC[NBC];
     int normal[NBC][DIMS];
@@ -59,6 +61,8 @@ int artra_dev_download(ArtraDevice *dev, int level, double *soa);
 /* set every byte of the three levels to 0x01 (the solver's poison pattern) */
 int artra_dev_poison(ArtraDevice *dev);
 int artra_dev_treat_boundary(ArtraDevice *dev, int level);
+/* the immersed-boundary ghost fill alone */
+int artra_dev_treat_immersed(ArtraDevice *dev, int level);
 /* per-axis maxima of |u_s| + c over interior fluid nodes of level TO,
  * folded with the given floors (body speeds) */
 int artra_dev_max_speed(ArtraDevice *dev, const double floor[DIMS], double vmax[DIMS]);
@@ -84,8 +88,9 @@ int artra_dev_set_exact(ArtraDevice *dev, int exact);
 
 /* route WENO5 faces through the indicator (params NULL disables) */
 int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float threshold);
-/* faces sent to the linear flux and faces evaluated since the last reset */
-int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[2], int reset);
+/* faces computed with the linear flux, faces classified, and faces of lines
+ * that overflowed their queue (plain WENO5) since the last reset */
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[3], int reset);
 int artra_dev_set_fused(ArtraDevice *dev, int fused);
 int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation);
 int artra_dev_fused_axes(ArtraDevice *dev);
@@ -103,14 +108,17 @@ const char *artra_mlp_error(void);
 /* batched cases: many independent cases advanced together, one persistent
  * launch per directional operator over the lines of every case */
 typedef struct ArtraBatch ArtraBatch;
-/* clock[c] = {now, end, rcInt, tmInt} of case c; cfl[c] its CFL number */
+/* clock[c] = {now, end, rcInt, tmInt, stepC, stepN} of case c (6 doubles);
+ * cfl[c] its CFL number */
 int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *clock, const double *cfl,
                            ArtraBatch **out);
 void artra_dev_batch_destroy(ArtraBatch *b);
 int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float threshold);
-int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[2], int reset);
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[3], int reset);
 int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms);
 int artra_dev_batch_clock(ArtraBatch *b, double *clock);
+int artra_dev_batch_set_clock(ArtraBatch *b, const double *clock);
+int artra_dev_batch_set_threads(ArtraBatch *b, const int *threads);
 int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads, int *grid);
 
 /* tuning knobs (0 keeps the default) */

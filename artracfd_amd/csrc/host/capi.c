@@ -17,6 +17,22 @@
 #include "artra_hip.h"
 #include "artra_host.h"
 
+/* The drop-in symbols take the caller's structures as the reference lays
+ * them out (commons.h:128-268, x86-64 with gcc): sizes and a few offsets as
+ * the reference's own headers give them, so that a layout change fails the
+ * build instead of a caller. */
+#include <stddef.h>
+_Static_assert(sizeof(Node) == 136 && offsetof(Node, U) == 16, "Node layout differs from commons.h:128-134");
+_Static_assert(sizeof(Partition) == 1032 && offsetof(Partition, ns) == 104 && offsetof(Partition, typeBC) == 912,
+               "Partition layout differs from commons.h:136-159");
+_Static_assert(sizeof(Polyhedron) == 568 && offsetof(Polyhedron, V) == 128 && offsetof(Polyhedron, Fp) == 344,
+               "Polyhedron layout differs from commons.h:174-205");
+_Static_assert(sizeof(Geometry) == 32 && sizeof(Space) == 1072 && offsetof(Space, geo) == 8 &&
+               offsetof(Space, part) == 40, "Space/Geometry layout differs from commons.h:207-228");
+_Static_assert(sizeof(Time) == 104 && offsetof(Time, end) == 64, "Time layout differs from commons.h:230-243");
+_Static_assert(sizeof(Model) == 152 && offsetof(Model, gamma) == 64 && offsetof(Model, g) == 120,
+               "Model layout differs from commons.h:245-268");
+
 struct ArtraHipSolver {
     Space *space;
     const Model *model;
@@ -62,7 +78,6 @@ static void FieldToNodes(Space *space, int tn, const double *field, long N)
 
 static int Supported(const Space *space, const Model *model)
 {
-    if (0 != model->sState) return artra_set_error(2, "gravity source term is not supported");
     if (0 != model->psi) return artra_set_error(2, "moving bodies are not supported");
     for (int n = 0; n < space->geo.totN; ++n) {
         const Polyhedron *p = space->geo.poly + n;
@@ -143,6 +158,8 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
     par.cv = model->cv;
     par.refMu = model->refMu;
     par.refT = model->refT;
+    par.sState = model->sState;
+    for (int d = 0; d < DIMS; ++d) par.g[d] = model->g[d];
     for (int p = 0; p < NBC; ++p) {
         par.typeBC[p] = part->typeBC[p];
         for (int d = 0; d < DIMS; ++d) par.normal[p][d] = part->N[p][d];
@@ -189,6 +206,11 @@ int artra_hip_detach(ArtraHipSolver *s)
 int artra_hip_treat_boundary(ArtraHipSolver *s, int tn)
 {
     return artra_dev_treat_boundary(s->dev, tn) ? DevError() : 0;
+}
+
+int artra_hip_treat_immersed_boundary(ArtraHipSolver *s, int tn)
+{
+    return artra_dev_treat_immersed(s->dev, tn) ? DevError() : 0;
 }
 
 int artra_hip_compute_dt(ArtraHipSolver *s, const Time *time, Real *dt)
@@ -330,7 +352,7 @@ int artra_hip_set_indicator(ArtraHipSolver *s, const float *params, float logit_
     return artra_dev_set_indicator(s->dev, params, logit_threshold) ? DevError() : 0;
 }
 
-int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[2], int reset)
+int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[3], int reset)
 {
     return artra_dev_hybrid_counts(s->dev, counts, reset) ? DevError() : 0;
 }
@@ -414,6 +436,29 @@ void TreatBoundary(const int tn, Space *space, const Model *model)
     if (artra_hip_upload(s, tn) || artra_hip_treat_boundary(s, tn) || artra_hip_download(s, tn)) {
         fprintf(stderr, "error: %s\n", artra_hip_error());
         exit(EXIT_FAILURE);
+    }
+}
+
+void TreatImmersedBoundary(const int tn, Space *space, const Model *model)
+{
+    ArtraHipSolver *s = DropInSolver(space, model);
+    if (artra_hip_upload(s, tn) || artra_hip_treat_immersed_boundary(s, tn) || artra_hip_download(s, tn)) {
+        fprintf(stderr, "error: %s\n", artra_hip_error());
+        exit(EXIT_FAILURE);
+    }
+}
+
+void ComputeGeometricField(Space *space, const Model *model)
+{
+    if (0 != model->psi) {
+        fprintf(stderr, "error: moving bodies (phase interaction %d) are not supported by this library\n", model->psi);
+        exit(EXIT_FAILURE);
+    }
+    artra_compute_geometric_field(space, model);
+    /* the cached device solver's tables were built from the previous flags */
+    if (gDropIn.solver && gDropIn.space == space) {
+        artra_hip_detach(gDropIn.solver);
+        memset(&gDropIn, 0, sizeof gDropIn);
     }
 }
 
@@ -501,33 +546,51 @@ struct ArtraHipBatch {
     Time **times;
 };
 
+/* The batch clock of every case from its Time and solver: now, end, the
+ * time since the last output instant and the output interval (solve.c:57-64),
+ * the step count and the step limit. */
+enum { kClock = 6 };
+
+static void FillClock(ArtraHipSolver *const *solvers, Time *const *times, int count, double *clock)
+{
+    for (int c = 0; c < count; ++c) {
+        const Time *t = times[c];
+        double *ck = clock + kClock * c;
+        ck[0] = t->now;
+        ck[1] = t->end;
+        ck[2] = solvers[c]->rcInt;
+        ck[3] = (INT_MAX == t->dataW[PROSD]) ? t->end : t->end / (Real)(t->dataW[PROSD]);
+        ck[4] = (double)t->stepC;
+        ck[5] = (double)t->stepN;
+    }
+}
+
 int artra_hip_batch_create(ArtraHipSolver *const *solvers, Time *const *times, int count, ArtraHipBatch **out)
 {
     *out = NULL;
     if (count < 1) return artra_set_error(1, "a batch needs at least one case");
     ArtraHipBatch *hb = calloc(1, sizeof *hb);
     ArtraDevice **devs = malloc(count * sizeof *devs);
-    double *clock = malloc(4 * count * sizeof *clock);
+    double *clock = malloc(kClock * count * sizeof *clock);
     double *cfl = malloc(count * sizeof *cfl);
-    hb->solvers = malloc(count * sizeof *hb->solvers);
-    hb->times = malloc(count * sizeof *hb->times);
-    if (!devs || !clock || !cfl || !hb->solvers || !hb->times) {
-        free(devs); free(clock); free(cfl); free(hb->solvers); free(hb->times); free(hb);
+    if (hb) {
+        hb->solvers = malloc(count * sizeof *hb->solvers);
+        hb->times = malloc(count * sizeof *hb->times);
+    }
+    if (!hb || !devs || !clock || !cfl || !hb->solvers || !hb->times) {
+        free(devs); free(clock); free(cfl);
+        if (hb) { free(hb->solvers); free(hb->times); free(hb); }
         return artra_set_error(1, "out of host memory");
     }
     int rc = 0;
     for (int c = 0; c < count && !rc; ++c) {
-        const Time *t = times[c];
         if (solvers[c]->space->geo.totN > 0) rc = artra_set_error(1, "batched case %d has bodies; batches take fluid-only cases", c);
         devs[c] = solvers[c]->dev;
-        clock[4 * c + 0] = t->now;
-        clock[4 * c + 1] = t->end;
-        clock[4 * c + 2] = solvers[c]->rcInt;
-        clock[4 * c + 3] = (INT_MAX == t->dataW[PROSD]) ? t->end : t->end / (Real)(t->dataW[PROSD]);
-        cfl[c] = t->numCFL;
+        cfl[c] = times[c]->numCFL;
         hb->solvers[c] = solvers[c];
         hb->times[c] = times[c];
     }
+    FillClock(solvers, times, count, clock);
     if (!rc && artra_dev_batch_create(devs, count, clock, cfl, &hb->b)) rc = DevError();
     free(devs);
     free(clock);
@@ -558,36 +621,39 @@ int artra_hip_batch_tune(ArtraHipBatch *hb, int reps, int *threads, int *grid)
     return artra_dev_batch_tune(hb->b, reps, threads, grid) ? DevError() : 0;
 }
 
+int artra_hip_batch_set_threads(ArtraHipBatch *hb, const int *threads)
+{
+    return artra_dev_batch_set_threads(hb->b, threads) ? DevError() : 0;
+}
+
 int artra_hip_batch_set_indicator(ArtraHipBatch *hb, const float *params, float logit_threshold)
 {
     return artra_dev_batch_set_indicator(hb->b, params, logit_threshold) ? DevError() : 0;
 }
 
-int artra_hip_batch_counts(ArtraHipBatch *hb, unsigned long long counts[2], int reset)
+int artra_hip_batch_counts(ArtraHipBatch *hb, unsigned long long counts[3], int reset)
 {
     return artra_dev_batch_counts(hb->b, counts, reset) ? DevError() : 0;
 }
 
 int artra_hip_batch_march(ArtraHipBatch *hb, int steps, int use_graph, double *ms)
 {
-    for (int c = 0; c < hb->count; ++c) {
-        const Time *t = hb->times[c];
-        if (t->stepC + steps > t->stepN) {
-            return artra_set_error(1, "batched case %d has %d steps left, fewer than %d", c, t->stepN - t->stepC,
-                                   steps);
-        }
-    }
-    if (artra_dev_batch_march(hb->b, steps, use_graph, ms)) return DevError();
-    double *clock = malloc(4 * hb->count * sizeof *clock);
+    double *clock = malloc(kClock * hb->count * sizeof *clock);
     if (!clock) return artra_set_error(1, "out of host memory");
-    if (artra_dev_batch_clock(hb->b, clock)) {
+    /* the clocks as the cases' Time structures hold them now (a case may have
+     * been marched on its own since the last batch march) */
+    FillClock(hb->solvers, hb->times, hb->count, clock);
+    int rc = artra_dev_batch_set_clock(hb->b, clock) || artra_dev_batch_march(hb->b, steps, use_graph, ms) ||
+             artra_dev_batch_clock(hb->b, clock);
+    if (rc) {
         free(clock);
         return DevError();
     }
     for (int c = 0; c < hb->count; ++c) {
-        hb->times[c]->now = clock[4 * c + 0];
-        hb->times[c]->stepC += steps;
-        hb->solvers[c]->rcInt = clock[4 * c + 2];
+        const double *ck = clock + kClock * c;
+        hb->times[c]->now = ck[0];
+        hb->solvers[c]->rcInt = ck[2];
+        hb->times[c]->stepC = (int)ck[4];
     }
     free(clock);
     return 0;

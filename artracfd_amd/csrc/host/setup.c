@@ -79,7 +79,6 @@ int artra_compute_parameters(Time *time, Space *space, Model *model)
     model->refMu = model->refMu / (model->refRho * model->refV * model->refL);
     model->gasR = 1.0 / (model->gamma * model->refMa * model->refMa);
     model->cv = model->gasR / (model->gamma - 1.0);
-    if (0 != model->sState) return artra_set_error(2, "gravity source term is not supported by this library");
     if (0 != model->psi) return artra_set_error(2, "moving bodies (phase interaction) are not supported");
     return 0;
 }

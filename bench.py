@@ -162,8 +162,8 @@ def batched_both(with_indicator=True):
     plain = batched(use_indicator=False)
     if with_indicator:
         hyb = batched(use_indicator=True)
-        plain["hybrid"] = {k: hyb[k] for k in ("value", "serial_value", "wall_ms", "linear_face_fraction", "tuned",
-                                                "serial_ms_per_case")}
+        plain["hybrid"] = {k: hyb[k] for k in ("value", "serial_value", "wall_ms", "linear_face_fraction",
+                                                "overflow_face_fraction", "tuned", "serial_ms_per_case")}
         plain["hybrid"]["vs_plain"] = round(hyb["value"] / plain["value"], 3)
     return plain
 
@@ -198,7 +198,7 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
         t2 = time.perf_counter()
         dev_ms = b.march(steps, graph=True)
         wall_s = time.perf_counter() - t2
-        lin, seen = b.counts()
+        lin, seen, over = b.counts()
         b.close()
         for s in solvers:
             s.close()
@@ -230,6 +230,7 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
                                    "p90": round(float(np.percentile(pc, 90)), 3),
                                    "p99": round(float(np.percentile(pc, 99)), 3), "sum": round(float(pc.sum()), 3)},
             "linear_face_fraction": round(lin / max(seen, 1), 4) if params is not None else None,
+            "overflow_face_fraction": round(over / max(seen, 1), 4) if params is not None else None,
             "tuned": {"threads_per_line": {str(sd): sorted(set(t for t, d in zip(threads, drawn) if d["side"] == sd))
                                            for sd in sides}, "grid_blocks": grid},
             "setup_seconds": round(setup_s, 2), "tune_seconds": round(tune_s, 2),
@@ -321,7 +322,7 @@ def main():
         case = Case(case_dir)
         solver = Solver(case)
         solver.set_indicator(params, thr)
-        hdone, helapsed, htm, (lin, seen) = timed(solver, args.warmup, args.steps)
+        hdone, helapsed, htm, (lin, seen, over) = timed(solver, args.warmup, args.steps)
         solver.close()
         case.close()
         hvalue = cells * 18 * hdone / helapsed / 1e6
@@ -331,6 +332,7 @@ def main():
             "value": round(hvalue, 3), "unit": "Mcell-stage updates/s", "vs_plain": round(hvalue / value, 3),
             "ms_per_step": round(helapsed * 1e3 / max(hdone, 1), 4),
             "linear_face_fraction": round(lin / max(seen, 1), 4),
+            "overflow_face_fraction": round(over / max(seen, 1), 4),
             "rel_l1_rho_vs_plain_at_end": float(np.abs(rho_h - rho_p).sum() / np.abs(rho_p).sum()),
             "end_time_steps": {"plain": steps_p, "hybrid": steps_h},
             "end_time_seconds": {"plain": round(el_p, 3), "hybrid": round(el_h, 3)},

@@ -204,6 +204,9 @@ def cases():
                        [inflow(1.5, 0.3, 0, 0, 1.2), OUTFLOW, slip(), OUTFLOW, OUTFLOW, OUTFLOW], (24, 16, 1)),
         "v_walls": ((1, 1, 0, 0, 0, 0, 1), INVISCID,
                     [slip(), noslip(), noslip(0.0033), slip(0.0036), slip(), noslip()], (16, 12, 8)),
+        # gravity source term (source_term.c:21-39) with its half-step operators around the split sweeps
+        "v_gravity": ((1, 1, 0, 0, 0, 0, 1), (0, 0.0, 1, (0.3, -2.0, 0.5)), base_bcs, (24, 16, 4)),
+        "v_gravity2": ((0, 1, 1, 0, 0, 0, 1), (0, 0.02, 1, (-0.5, 1.5, 0)), base_bcs, (24, 16, 4)),
     }
     for tag, (num, mat, bcs, m) in variants.items():
         c[tag] = dict(case=case_text(((0, 0, 0), (1, 0.6, 0.016 if m[2] <= 4 else 0.4), m),

@@ -1,0 +1,4 @@
+count begin
+0
+0
+count end

@@ -6,9 +6,10 @@
  *
  *  1. Drop-in symbols with the reference's own signatures.  A solver built
  *     from the reference sources links against these instead of
- *     fluid_dynamics.c / boundary_treatment.c and runs unchanged; each call
- *     moves the node array to the device and back, so it is correct but
- *     bound by host<->device copies.
+ *     fluid_dynamics.c, boundary_treatment.c and immersed_boundary.c (and the
+ *     flux files only they call) and runs unchanged; each device call moves
+ *     the node array to the device and back, so it is correct but bound by
+ *     host<->device copies.
  *
  *  2. The resident API (artra_hip_*): attach once, keep the field in HBM,
  *     step with the device CFL reduction, copy back only for output.  Every
@@ -44,6 +45,20 @@ void EvolveFluidDynamics(const Real dt, Space *space, const Model *model);
  * Replaces boundary_treatment.c:28-65 (boundary_treatment.h:32). */
 void TreatBoundary(const int tn, Space *space, const Model *model);
 
+/* Fill the immersed-boundary ghost nodes of level tn only.  Replaces
+ * immersed_boundary.c:270-340 (immersed_boundary.h:39), called by the
+ * reference's solid dynamics (solid_dynamics.c:47). */
+void TreatImmersedBoundary(const int tn, Space *space, const Model *model);
+
+/* Classify every interior node against the bodies: did (inside body k),
+ * fid (closest facet), lid and gst (interfacial and ghost layers), on the
+ * host.  Replaces immersed_boundary.c:65-264 (immersed_boundary.h:32),
+ * called at initialisation (initialization.c:47).  Stationary bodies only:
+ * a model with moving bodies (psi != 0) is a fatal error, as everywhere in
+ * this library.  Drops the device solver cached for space, whose tables
+ * derive from the flags. */
+void ComputeGeometricField(Space *space, const Model *model);
+
 /* ----------------------------------------------------- 2. resident API */
 
 typedef struct ArtraHipSolver ArtraHipSolver;
@@ -56,6 +71,8 @@ int artra_hip_detach(ArtraHipSolver *solver);
 
 /* TreatBoundary on the device for time level tn (0, 1 or 2). */
 int artra_hip_treat_boundary(ArtraHipSolver *solver, int tn);
+/* The immersed-boundary part of it alone (TreatImmersedBoundary). */
+int artra_hip_treat_immersed_boundary(ArtraHipSolver *solver, int tn);
 
 /* The CFL time step of solve.c:108-153: numCFL * min_s(d_s / Vmax_s). */
 int artra_hip_compute_dt(ArtraHipSolver *solver, const Time *time, Real *dt);
@@ -138,8 +155,9 @@ const char *artra_hip_error(void);
  *
  * Cases must be three-dimensional, fluid-only, inviscid, split WENO5 with
  * the fast arithmetic and share the time scheme and flux model; they run in
- * lockstep and each must have `steps` steps left.  A case whose end time
- * falls inside the marched steps stops changing (dt = 0) from then on.
+ * lockstep.  A case that reaches its end time or its step limit (stepN)
+ * inside the marched steps stops changing (dt = 0) from then on, and its
+ * step count stops with it, as the solver loop's does (solve.c:65).
  */
 typedef struct ArtraHipBatch ArtraHipBatch;
 int artra_hip_batch_create(ArtraHipSolver *const *solvers, Time *const *times, int count, ArtraHipBatch **out);
@@ -148,11 +166,15 @@ int artra_hip_batch_destroy(ArtraHipBatch *batch);
  * persistent grid size; threads[count] and *grid receive the choice.  The
  * fields and clocks are restored afterwards. */
 int artra_hip_batch_tune(ArtraHipBatch *batch, int reps, int *threads, int *grid);
+/* Set the tile shape instead of tuning it: threads[c] threads per line of
+ * case c (64, 128 or 256, at least its longest interior line). */
+int artra_hip_batch_set_threads(ArtraHipBatch *batch, const int *threads);
 /* Learned indicator for every case (NULL: plain WENO5). */
 int artra_hip_batch_set_indicator(ArtraHipBatch *batch, const float *params, float logit_threshold);
-int artra_hip_batch_counts(ArtraHipBatch *batch, unsigned long long counts[2], int reset);
+int artra_hip_batch_counts(ArtraHipBatch *batch, unsigned long long counts[3], int reset);
 /* steps lockstep time steps; use_graph replays one captured step; *ms the
- * device time of the steps (may be NULL).  Each case's Time advances. */
+ * device time of the steps (may be NULL).  Each case starts from its Time
+ * (now, stepC) and solver clock as they stand, and its Time advances. */
 int artra_hip_batch_march(ArtraHipBatch *batch, int steps, int use_graph, double *ms);
 
 /* ------------------------------------------- learned troubled-face indicator */
@@ -199,8 +221,11 @@ int artra_hip_set_indicator(ArtraHipSolver *solver, const float *params, float l
  * the gate off) take the linear flux without the network, and a wavefront
  * of such faces skips the network altogether. */
 int artra_hip_set_indicator_options(ArtraHipSolver *solver, int every_stage, float smooth_variation);
-/* counts[0] faces sent to the linear flux, counts[1] faces evaluated (per classification) */
-int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[2], int reset);
+/* Face tallies per classification (once per directional operator):
+ * counts[0] faces computed with the linear flux, counts[1] faces
+ * classified, counts[2] faces of lines whose troubled faces overflowed the
+ * line's queue and which therefore ran plain WENO5 on every face. */
+int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[3], int reset);
 
 /* Train in place on the device (Adam, weighted logistic loss of
  * label > tau); loss_history[epoch] receives the last batch loss per epoch

@@ -12,7 +12,8 @@
  *
  *   oracle_compute_time_step      solve.c:108-153
  *   oracle_march                  solve.c:48-107 (dt rectification, no output)
- *   oracle_evolve_fluid_dynamics  fluid_dynamics.c:55-274
+ *   oracle_evolve_fluid_dynamics  fluid_dynamics.c:55-274 (with the gravity
+ *                                 source operator, source_term.c:21-39)
  *   oracle_compute_fhat           convective_flux.c:48-124, cfd_commons.c:79-246
  *   oracle_weno5 / oracle_weno3   weno5.c:35-62 / weno3.c:35-55
  *   oracle_compute_fvhat          diffusive_flux.c:46-312, cfd_commons.c:288-313
@@ -780,9 +781,49 @@ static void Stages(double dt, int p, Space *space, const Model *model)
     }
 }
 
-void oracle_evolve_fluid_dynamics(double dt, Space *space, const Model *model)
+/* Gravity source operator (source_term.c:21-39; LLLU's PHI branch,
+ * fluid_dynamics.c:177-214): every interior fluid node of the x-sweep
+ * ranges takes Um = a Uo + b (Un + dt Phi(Un)), Phi = (0, rho g, rho g . u),
+ * with the stage's dt itself as the factor (no 1/ds) and never the unsplit
+ * accumulation. */
+static void SourceSweep(double dt, double a, double b, int to, int tn, int tm, Space *space, const Model *model)
 {
-    const double h = 0.5 * dt;
+    const Partition *part = &space->part;
+    Node *node = space->node;
+    for (int k = part->np[X][Z][MIN]; k < part->np[X][Z][MAX]; ++k)
+        for (int j = part->np[X][Y][MIN]; j < part->np[X][Y][MAX]; ++j)
+            for (int i = part->np[X][X][MIN]; i < part->np[X][X][MAX]; ++i) {
+                Node *nd = node + IDX(k, j, i, part->n[Y], part->n[X]);
+                if (0 != nd->did) continue;
+                const double *Un = nd->U[tn];
+                const double V[3] = {Un[1] / Un[0], Un[2] / Un[0], Un[3] / Un[0]};
+                const double fb[3] = {Un[0] * model->g[X], Un[0] * model->g[Y], Un[0] * model->g[Z]};
+                const double Phi[DIMU] = {0.0, fb[X], fb[Y], fb[Z], fb[X] * V[X] + fb[Y] * V[Y] + fb[Z] * V[Z]};
+                double *Um = nd->U[tm];
+                const double *Uo = nd->U[to];
+                for (int n = 0; n < DIMU; ++n) Um[n] = a * Uo[n] + b * (Un[n] + dt * Phi[n]);
+            }
+}
+
+static void SourceStages(double dt, Space *space, const Model *model)
+{
+    if (0 == model->tScheme) {
+        SourceSweep(dt, 0.0, 1.0, TO, TO, TN, space, model);
+        oracle_treat_boundary(TN, space, model);
+        SourceSweep(dt, 1.0 / 2.0, 1.0 / 2.0, TO, TN, TO, space, model);
+        oracle_treat_boundary(TO, space, model);
+    } else {
+        SourceSweep(dt, 0.0, 1.0, TO, TO, TN, space, model);
+        oracle_treat_boundary(TN, space, model);
+        SourceSweep(dt, 3.0 / 4.0, 1.0 / 4.0, TO, TN, TM, space, model);
+        oracle_treat_boundary(TM, space, model);
+        SourceSweep(dt, 1.0 / 3.0, 2.0 / 3.0, TO, TM, TO, space, model);
+        oracle_treat_boundary(TO, space, model);
+    }
+}
+
+static void FluidOperators(double h, Space *space, const Model *model)
+{
     if (OPTBYOPT == model->multidim) {
         Stages(h, DIMS, space, model);
         Stages(h, DIMS, space, model);
@@ -801,6 +842,16 @@ void oracle_evolve_fluid_dynamics(double dt, Space *space, const Model *model)
     const int c = space->part.collapse;
     if (c < 0 || c > 8) return;
     for (int q = 0; q < 6 && order[c][q] >= 0; ++q) Stages(h, order[c][q], space, model);
+}
+
+/* EvolveFluidDynamics (fluid_dynamics.c:55-116): the gravity source
+ * half-steps (57-59, 112-114) around the split or unsplit operators. */
+void oracle_evolve_fluid_dynamics(double dt, Space *space, const Model *model)
+{
+    const double h = 0.5 * dt;
+    if (0 != model->sState) SourceStages(h, space, model);
+    FluidOperators(h, space, model);
+    if (0 != model->sState) SourceStages(h, space, model);
 }
 
 double oracle_compute_time_step(const Time *time, const Space *space, const Model *model)

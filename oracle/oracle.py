@@ -20,10 +20,13 @@ _lib = None
 
 
 def build():
-    """Compile liboracle.so (and the reference harness where its sources exist)."""
+    """Compile liboracle.so, and where the reference sources exist the
+    reference harness and the drop-in driver (the reference's objects linked
+    with libartra_hip.so, which must be built first)."""
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir("/root/reference"):
         subprocess.run(["make", "-s", "-C", HERE, "ref", "-j8"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "dropin"], check=True)
 
 
 def library():

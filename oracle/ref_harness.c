@@ -13,6 +13,12 @@
  *   - writes full-precision FP64 dumps of the padded node array, and
  *   - writes per-function input/output vectors for the flux building blocks.
  *
+ * Built with -DARTRA_DROPIN it is instead linked against the reference
+ * objects minus the ones libartra_hip replaces (see oracle/Makefile,
+ * target dropin) plus libartra_hip.so: the reference's own driver then runs
+ * on the GPU through the drop-in symbols (only the "run" and "layout"
+ * modes; the per-function vectors need the replaced objects).
+ *
  * Dump layout ("ARTRAST1"): int32 n[3], ng[3], steps, 0; double now, dt;
  * int32 did/fid/lid/gst[N]; double U[level TO][5][N] (component-major).
  */
@@ -34,6 +40,8 @@
 #include "weno.h"
 #include "cfd_commons.h"
 #include "computational_geometry.h"
+#include <stddef.h>
+#include "abi_fields.h"
 
 /* made visible by compiling solve.c with -Dstatic= */
 extern Real ComputeTimeStep(const Time *, const Space *, const Model *);
@@ -155,6 +163,7 @@ static int RunCase(int argc, char **argv)
     return 0;
 }
 
+#ifndef ARTRA_DROPIN
 /* splitmix64: deterministic inputs for the per-function vectors */
 static uint64_t rngState = 0;
 static uint64_t NextU64(void)
@@ -309,11 +318,23 @@ static int Vectors(int argc, char **argv)
     return 0;
 }
 
+#endif /* ARTRA_DROPIN */
+
+/* sizeof and offsetof of the structures of the drop-in ABI (commons.h) */
+static int Layout(void)
+{
+    ABI_LIST
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    if (argc < 2) Die("usage: artra_ref run|vectors ...");
+    if (argc < 2) Die("usage: artra_ref run|vectors|layout ...");
     if (0 == strcmp(argv[1], "run")) return RunCase(argc, argv);
+#ifndef ARTRA_DROPIN
     if (0 == strcmp(argv[1], "vectors")) return Vectors(argc, argv);
+#endif
+    if (0 == strcmp(argv[1], "layout")) return Layout();
     Die("unknown mode");
     return 2;
 }

@@ -11,6 +11,7 @@ compressed numpy archives:
                (U_init) and after the run (U_final), FP64
   vectors.npz  WENO5/WENO3, face fluxes on a random grid, primitive maps,
                point-triangle distances
+  abi_layout.txt  sizeof/offsetof of the drop-in structures (commons.h)
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -30,7 +31,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = {
     "sod": 0, "tgv16": 20, "cyl64": 60, "sphere20": 20,
     "v_base": 10, "v_weno3": 10, "v_rk2": 10, "v_roe": 10, "v_sw": 10, "v_unsplit": 10,
-    "v_mix": 10, "v_visc": 10, "v_collapse": 10, "v_walls": 10,
+    "v_mix": 10, "v_visc": 10, "v_collapse": 10, "v_walls": 10, "v_gravity": 10, "v_gravity2": 10,
 }
 
 
@@ -81,6 +82,16 @@ def make_vectors(work, seed=20261202, count=400):
     print("vectors", count, nf)
 
 
+def make_layout():
+    """sizeof/offsetof of the drop-in structures as the reference's own
+    commons.h lays them out (the harness's layout dump)."""
+    import subprocess
+    out = subprocess.run([oracle.REF, "layout"], check=True, capture_output=True, text=True).stdout
+    with open(os.path.join(HERE, "abi_layout.txt"), "w") as fp:
+        fp.write(out)
+    print("layout", len(out.splitlines()), "entries")
+
+
 def main(argv):
     if not os.path.exists(oracle.REF):
         oracle.build()
@@ -89,10 +100,13 @@ def main(argv):
         for name in names:
             if name == "vectors":
                 make_vectors(work)
+            elif name == "layout":
+                make_layout()
             else:
                 make_case(name, CASES[name], work)
         if not argv:
             make_vectors(work)
+            make_layout()
 
 
 if __name__ == "__main__":

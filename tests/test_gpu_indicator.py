@@ -186,8 +186,9 @@ def test_all_troubled_routing_is_plain_weno5(model, tmp_path):
     name = _flux_model_variant(tmp_path, model)
     print("hybrid all-WENO shockvortex64", model, flush=True)
     U0, d0, _ = _evolve(name, 4)
-    U1, d1, (lin, seen) = _evolve(name, 4, _params(3), threshold=-1e30, gate=0.0)
+    U1, d1, (lin, seen, over) = _evolve(name, 4, _params(3), threshold=-1e30, gate=0.0)
     assert lin == 0 and seen > 0
+    assert over == seen                 # 65 faces per line, more than the 16-face queue: every line overflows
     np.testing.assert_array_equal(d1, d0)
     assert np.array_equal(U1, U0)
 
@@ -201,8 +202,8 @@ def test_queued_faces_are_plain_weno5(tmp_path):
     d = indicator.riemann_case(str(tmp_path / "riemann12"), np.random.default_rng(7), n=12, steps=30)
     print("hybrid all-WENO through the queue, riemann 12x12", flush=True)
     U0, d0, _ = _evolve(d, 30)
-    U1, d1, (lin, seen) = _evolve(d, 30, _params(3), threshold=-1e30, gate=0.0)
-    assert lin == 0 and seen > 0
+    U1, d1, (lin, seen, over) = _evolve(d, 30, _params(3), threshold=-1e30, gate=0.0)
+    assert lin == 0 and seen > 0 and over == 0
     np.testing.assert_array_equal(d1, d0)
     assert np.array_equal(U1, U0)
 
@@ -218,8 +219,8 @@ def test_all_linear_on_smooth_data_is_close_to_weno5(tmp_path):
     Ui = init.interior(init.field())
     init.close()
     Uw, dw, _ = _evolve(d, 20)
-    Ul, dl, (lin, seen) = _evolve(d, 20, _params(3), threshold=1e30)
-    assert lin == seen and seen > 0
+    Ul, dl, (lin, seen, over) = _evolve(d, 20, _params(3), threshold=1e30)
+    assert lin == seen and seen > 0 and over == 0
     np.testing.assert_allclose(dl, dw, rtol=1e-8)    # the speeds differ at the level the fluxes do
     change = np.abs(Uw[0] - Ui[0]).max()
     assert change > 1e-5
@@ -235,7 +236,7 @@ def test_trained_indicator_on_shock_vortex():
     params, meta = indicator.load()
     thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
     Up, dp, _ = _evolve("shockvortex64", 1 << 20)
-    Uh, dh, (lin, seen) = _evolve("shockvortex64", 1 << 20, params, thr)
+    Uh, dh, (lin, seen, over) = _evolve("shockvortex64", 1 << 20, params, thr)
     l1 = np.abs(Uh[0] - Up[0]).sum() / np.abs(Up[0]).sum()
     print("steps", len(dp), len(dh), "rel L1 rho", l1, "linear fraction", lin / seen)
     assert l1 <= 1e-3
@@ -254,7 +255,7 @@ def test_trained_indicator_with_roe_and_steger_warming(model, tmp_path):
     params, meta = indicator.load()
     thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
     Up, dp, _ = _evolve(name, 20)
-    Uh, dh, (lin, seen) = _evolve(name, 20, params, thr)
+    Uh, dh, (lin, seen, over) = _evolve(name, 20, params, thr)
     l1 = np.abs(Uh[0] - Up[0]).sum() / np.abs(Up[0]).sum()
     print("rel L1 rho", l1, "linear fraction", lin / seen)
     assert l1 <= 1e-3
@@ -288,3 +289,89 @@ def test_trained_network_gains_over_routing_without_it():
     assert lin[1] == 1.0 and lin[0] > 1e-3
     assert gate[0] <= 1e-3
     assert net[1] >= gate[1] + 0.1
+
+
+def _ramp_case(tmp_path, numerical):
+    """A density ramp rho = 1 + 0.5 x advected at u = 0.3 with uniform
+    pressure, 128 cells along x (outflow ends), 4 x 4 periodic cells across:
+    away from the ends every stencil along x is a linear ramp, so WENO5's
+    smoothness indicators are all equal and its weights linear."""
+    d = tmp_path / ("ramp_" + "".join(map(str, numerical)))
+    d.mkdir()
+    lines = ["space begin", "0, 0, 0", "1, 0.03125, 0.03125", "128, 4, 4", "space end",
+             "time begin", "0", "1000000000.0", "0.6", "10", "1", "0", "time end",
+             "numerical begin", *map(str, numerical), "numerical end",
+             "material begin", "0", "0.0", "0", "0, 0, 0", "material end",
+             "reference begin", "1.0", "1.0", "1.0", "1.0", "reference end",
+             "initialization begin", "1+0.5*x", "0.3", "0", "0", "1", "initialization end"]
+    for name, kind in zip(("west", "east", "south", "north", "front", "back"),
+                          ("outflow", "outflow", "periodic", "periodic", "periodic", "periodic")):
+        lines += [f"{name} boundary begin", kind, f"{name} boundary end"]
+    (d / "artracfd.case").write_text("\n".join(lines) + "\n")
+    (d / "artracfd.geo").write_text("count begin\n0\n0\ncount end\n")
+    return str(d)
+
+
+@pytest.mark.parametrize("numerical", [(1, 1, 0, 0, 0, 0, 1), (1, 1, 0, 1, 0, 0, 1), (1, 1, 0, 0, 1, 0, 1)],
+                         ids=["llf", "roe", "sw"])
+def test_all_linear_routing_equals_weno5_on_linear_data(tmp_path, numerical):
+    """Every face routed to the linear flux (threshold never reached) against
+    plain WENO5 on linear data, one time step: away from the outflow ends
+    (which reach at most 18 nodes in: two x operators of three stages, three
+    nodes each) the two agree to 1e-13 of each variable group's magnitude,
+    for the mean-state and Roe-average Lax-Friedrichs and the Steger-Warming
+    splits.  Pins the routed kernel's linear flux (stencil coefficients,
+    face state from the per-node quantities, eigenvalue split)."""
+    d = _ramp_case(tmp_path, numerical)
+    print("all-linear vs WENO5 on a density ramp", numerical, flush=True)
+    from artracfd_amd import Case
+    c = Case(d)
+    U0 = c.interior(c.field())
+    c.close()
+    Uw, dw, _ = _evolve(d, 1)
+    Ul, dl, (lin, seen, over) = _evolve(d, 1, _params(3), threshold=1e30, gate=0.0)
+    assert lin == seen and seen > 0 and over == 0
+    mid = slice(24, -24)
+    change = np.abs(Uw[0, ..., mid] - U0[0, ..., mid]).max()
+    assert change > 1e-4, change
+    for sl in (slice(0, 1), slice(1, 4), slice(4, 5)):
+        scale = np.abs(Uw[sl, ..., mid]).max()
+        diff = np.abs(Ul[sl, ..., mid] - Uw[sl, ..., mid]).max()
+        print(sl, diff / scale)
+        assert diff <= 1e-13 * scale, (sl, diff, scale)
+    np.testing.assert_allclose(dl, dw, rtol=1e-14)
+
+
+def test_trained_indicator_at_256_to_end_time():
+    """The learned-hybrid workload itself: the 256^3 shock-vortex case to
+    t = 0.6 with the committed weights against plain WENO5 on the same GPU:
+    the same number of steps, density within 1e-3 relative L1, and most faces
+    on the linear flux."""
+    from artracfd_amd import indicator
+    print("trained indicator shockvortex256 to t = 0.6", flush=True)
+    params, meta = indicator.load()
+    thr = float(meta.get("logit_threshold", indicator.LOGIT_THRESHOLD))
+    rp, dp, _ = _evolve_density("shockvortex256")
+    rh, dh, (lin, seen, over) = _evolve_density("shockvortex256", params, thr)
+    l1 = float(np.abs(rh - rp).sum() / np.abs(rp).sum())
+    print("steps", len(dp), len(dh), "rel L1 rho", l1, "linear", lin / seen, "overflowed", over / seen, flush=True)
+    assert len(dh) == len(dp) and len(dp) > 100
+    assert l1 <= 1e-3
+    assert lin / seen >= 0.9
+
+
+def _evolve_density(name, params=None, threshold=0.0):
+    """Density at the case's end time, the dt sequence and the face tallies
+    (the full field of a 256^3 case is not kept)."""
+    from artracfd_amd import Case, Solver
+    case = Case(case_dir(name))
+    solver = Solver(case)
+    if params is not None:
+        solver.set_indicator(params, threshold)
+    solver.hybrid_counts(reset=True)
+    done, dts = solver.march()
+    rho = case.interior(solver.field())[0].copy()
+    counts = solver.hybrid_counts()
+    solver.close()
+    case.close()
+    return rho, dts, counts
