@@ -280,9 +280,9 @@ def test_trained_network_gains_over_routing_without_it():
 
     print("gain over routing without the network, shockvortex64", flush=True)
     Up, _, _ = _evolve("shockvortex64", 1 << 20)
-    Un, _, (ln, sn) = _evolve("shockvortex64", 1 << 20, params, thr)
-    Ug, _, (lg, sg) = _evolve("shockvortex64", 1 << 20, params, -1e30)      # gate only
-    Ua, _, (la, sa) = _evolve("shockvortex64", 1 << 20, params, 1e30, gate=0.0)  # all linear
+    Un, _, (ln, sn, _) = _evolve("shockvortex64", 1 << 20, params, thr)
+    Ug, _, (lg, sg, _) = _evolve("shockvortex64", 1 << 20, params, -1e30)      # gate only
+    Ua, _, (la, sa, _) = _evolve("shockvortex64", 1 << 20, params, 1e30, gate=0.0)  # all linear
     net, gate, lin = (l1(Un, Up), ln / sn), (l1(Ug, Up), lg / sg), (l1(Ua, Up), la / sa)
     print("network", net, "gate only", gate, "all linear", lin)
     assert net[0] <= 1e-3
