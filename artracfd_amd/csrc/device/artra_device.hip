@@ -58,6 +58,13 @@ constexpr int kXBlock = 256;        // threads (faces) per sweep_x block
 constexpr int kMarchX = 64;         // x extent of a sweep_march block (one wavefront)
 constexpr int kMarchT = 4;          // transverse extent of a sweep_march block
 constexpr int kMaxGroup = 8;        // boxes per bc_group launch
+constexpr int kMarchChunk = ARTRA_DEV_MARCH_CHUNK;   // steps per device-stepped march call (the record's length)
+
+// bytes of a device march's scratch with a record of cap steps (see MarchScratch)
+constexpr size_t MarchScratchBytes(int cap)
+{
+    return (size_t)(4 + 6 + 2 + cap) * sizeof(double);
+}
 
 struct Grid {
     int nx, ny, nz;
@@ -74,7 +81,20 @@ struct StageArgs {
     int accumulate;                 // U_tm += b r Phi (unsplit, non-x)
     int toIsTn;                     // U_to and U_tn are the same level
     double dd[3];
+    const double *step;             // when set, the operator's step on the device: r = *step * dds
+    double dds;
 };
+
+// The stage's r from the device-resident step when there is one; false when
+// that step is zero (a march that has reached its end time: nothing moves).
+__device__ __forceinline__ bool ResolveStep(StageArgs &sa)
+{
+    if (!sa.step) return true;
+    const double h = *sa.step;
+    if (h == 0.0) return false;
+    sa.r = h * sa.dds;
+    return true;
+}
 
 __device__ __forceinline__ int Index(const Grid &g, int i, int j, int k)
 {
@@ -158,6 +178,7 @@ __device__ __forceinline__ void LoadViscousNeighbours(const Grid &g, const doubl
 template <int NS, bool VISC, bool FAST>
 __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxModel mdl)
 {
+    if (!ResolveStep(sa)) return;
     constexpr int SL = (NS == 6) ? -2 : -1;
     constexpr int K0 = -SL;
     __shared__ double sF[5][kXBlock];
@@ -215,6 +236,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
 template <int S, int NS, bool VISC, bool FAST>
 __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArgs sa, FluxModel mdl, int segment)
 {
+    if (!ResolveStep(sa)) return;
     constexpr int SL = (NS == 6) ? -2 : -1;
     constexpr int K0 = -SL;
     constexpr int T2 = (S == 1) ? 2 : 1;               // the other transverse axis
@@ -269,6 +291,7 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
 template <bool FAST>
 __global__ __launch_bounds__(256) void source_stage(Grid g, StageArgs sa, double gx, double gy, double gz)
 {
+    if (!ResolveStep(sa)) return;
     const int ex = g.hi[0] - g.lo[0], ey = g.hi[1] - g.lo[1];
     const long total = (long)ex * ey * (g.hi[2] - g.lo[2]);
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -543,6 +566,16 @@ __device__ __forceinline__ long XcdLine(long b, long nb)
     return (x < rem) ? x * (q + 1) + i : rem * (q + 1) + (x - rem) * q + i;
 }
 
+// A value every lane of the wavefront holds alike, moved to scalar
+// registers (it then holds no vector registers through the stage loop).
+__device__ __forceinline__ double Uniform(double x)
+{
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)bits);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(bits >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Troubled faces a line can queue: more than this and the line runs plain
 // WENO5 for the operator (nothing is gained by routing then).
 __host__ __device__ __forceinline__ int LineQueueCap(int F)
@@ -550,24 +583,24 @@ __host__ __device__ __forceinline__ int LineQueueCap(int F)
     return F / 8 > 16 ? F / 8 : 16;
 }
 
-// LDS of one line.  Every line: the states [5][n], the face fluxes [5][F]
-// and the fill rules [2][kLineRules].  With the indicator (hyb) also the
-// velocities and total enthalpy of every node [4][n], the per-field WENO5
-// numerators and denominators of the queued faces [cap][5][2], the
-// operator's initial state of the interior nodes [5][F - 1] (which the plain
-// sweeps keep in registers), the queue [cap], each face's queue position
-// [F], the queue length, and the FP32 density and pressure of every node
-// [2][n].
+// LDS of one line.  Every line: the states [5][n], the face fluxes [5][F],
+// the operator's initial state of the interior nodes [5][F - 1] and the fill
+// rules [2][kLineRules].  With the indicator (hyb) also the velocities and
+// total enthalpy of every node [4][n], the per-field WENO5 numerators and
+// denominators of the queued faces [cap][5][2], the queue [cap], each face's
+// queue position [F], the queue length, and the FP32 density and pressure
+// of every node [2][n].
 __host__ __device__ __forceinline__ size_t LinePlainBytes(int n, int F)
 {
-    return ((size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) + 15) / 16 * 16;
+    return ((size_t)(5 * n + 5 * F + 5 * (F - 1)) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) + 15) /
+           16 * 16;
 }
 
 __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool hyb)
 {
     const int cap = LineQueueCap(F);
-    const size_t extra = (size_t)(4 * n + 10 * cap + 5 * (F - 1)) * sizeof(double) +
-                         (size_t)(cap + F + 4) * sizeof(int) + (size_t)2 * n * sizeof(float);
+    const size_t extra = (size_t)(4 * n + 10 * cap) * sizeof(double) + (size_t)(cap + F + 4) * sizeof(int) +
+                         (size_t)2 * n * sizeof(float);
     return LinePlainBytes(n, F) + (hyb ? (extra + 15) / 16 * 16 : 0);
 }
 
@@ -578,11 +611,11 @@ __device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const d
 {
     double u[5], nv[4];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) u[c] = st[c * n + p];
+    for (int c = 0; c < 5; ++c) u[c] = st[p * 5 + c];
     if (features) artra::node_rho_p(mdl.gamma, u, nodeRP[p], nodeRP[n + p]);
     artra::node_velocity_enthalpy(mdl.gamma, u, nv);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) nodeV[c * n + p] = nv[c];
+    for (int c = 0; c < 4; ++c) nodeV[p * 4 + c] = nv[c];
 }
 
 // All stages of the directional operator on one line, run by the T threads
@@ -607,13 +640,13 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 {
     constexpr int SL = (NS == 6) ? 2 : 1;           // stencil nodes left of the face's left node
     const int n = la.n, F = la.hi - la.lo + 1, cap = LineQueueCap(F);
-    double *st = (double *)mem;                     // [5][n]
-    double *fb = st + 5 * n;                        // [5][F]
-    int *rules = (int *)(fb + 5 * F);               // [2][kLineRules]: destinations, sources
-    double *nodeV = (double *)(mem + LinePlainBytes(n, F));  // [4][n] (HYB)
+    double *st = (double *)mem;                     // [n][5]
+    double *fb = st + 5 * n;                        // [F][5]
+    double *u0 = fb + 5 * F;                        // [F - 1][5]
+    int *rules = (int *)(u0 + 5 * (F - 1));         // [2][kLineRules]: destinations, sources
+    double *nodeV = (double *)(mem + LinePlainBytes(n, F));  // [n][4] (HYB)
     double *hbuf = nodeV + 4 * n;                   // [cap][5 fields][numerator, denominator]
-    double *u0 = hbuf + 10 * cap;                   // [5][F - 1]
-    int *queue = (int *)(u0 + 5 * (F - 1));         // [cap]
+    int *queue = (int *)(hbuf + 10 * cap);          // [cap]
     int *fidx = queue + cap;                        // [F]
     int *queued = fidx + F;
     float *nodeRP = (float *)(queued + 4);          // [2][n]
@@ -622,14 +655,12 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     for (int p = tt; p < n; p += T) {
         double v[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) st[c * n + p] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
-        if (HYB) {
-            LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
-            if (p >= la.lo && p < la.hi) {
+        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
+        if (p >= la.lo && p < la.hi) {
 #pragma unroll
-                for (int c = 0; c < 5; ++c) u0[c * (F - 1) + p - la.lo] = v[c];
-            }
+            for (int c = 0; c < 5; ++c) u0[(p - la.lo) * 5 + c] = v[c];
         }
+        if (HYB) LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
     }
     // the line's fill rules into LDS: read per lane from the argument
     // block once here, where the field loads hide the latency
@@ -646,19 +677,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     }
     if (HYB && tt == 0) *queued = 0;
     __syncthreads();
-    // the operator's initial state: in registers, or (HYB, whose flux
-    // passes leave no registers for it) in LDS
-    double U0[K][5];
-    if (!HYB) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = own[k] ? la.lo + tt + T * k : la.lo;
-#pragma unroll
-            for (int c = 0; c < 5; ++c) U0[k][c] = st[c * n + i];
-        }
-    }
     const int M = (F + T - 1) / T;
-    const double r = la.dt ? *la.dt * la.dd : la.r;
+    const double r = Uniform(la.dt ? *la.dt * la.dd : la.r);
     if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
     bool routed = false;                            // this operator routes faces (HYB, queue within cap)
     int nq = 0;
@@ -713,14 +733,19 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const int q = tt + T * m;
                 if (q >= F) break;
                 const int f = la.lo - 1 + q;   // face between f and f + 1
-                double w[NS][5], Fx[5];
+                double Fx[5];
+                if (FAST && NS == 6) {
+                    artra::weno_flux_line<S>(mdl, st, f, Fx);
+                } else {
+                    double w[NS][5];
 #pragma unroll
-                for (int j = 0; j < NS; ++j)
+                    for (int j = 0; j < NS; ++j)
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - SL + j];
-                ConvFlux<S, NS, FAST>(mdl, w, Fx);
+                        for (int c = 0; c < 5; ++c) w[j][c] = st[(f - SL + j) * 5 + c];
+                    ConvFlux<S, NS, FAST>(mdl, w, Fx);
+                }
 #pragma unroll
-                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
             }
         } else {
             // one pass over the linear fluxes of the routed faces and the
@@ -737,7 +762,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 #pragma unroll
                         for (int j = 0; j < 6; ++j)
 #pragma unroll
-                            for (int c = 0; c < 5; ++c) w[j][c] = st[c * n + f - 2 + j];
+                            for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
                         artra::linear_sum_diff(w, Sg, Dl);
                     }
                     artra::FaceState fs;
@@ -745,8 +770,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                         double nL[4], nR[4];
 #pragma unroll
                         for (int c = 0; c < 4; ++c) {
-                            nL[c] = nodeV[c * n + f];
-                            nR[c] = nodeV[c * n + f + 1];
+                            nL[c] = nodeV[f * 4 + c];
+                            nR[c] = nodeV[(f + 1) * 4 + c];
                         }
                         artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
                         artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
@@ -754,8 +779,8 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                         double UL[5], UR[5];
 #pragma unroll
                         for (int c = 0; c < 5; ++c) {
-                            UL[c] = st[c * n + f];
-                            UR[c] = st[c * n + f + 1];
+                            UL[c] = st[f * 5 + c];
+                            UR[c] = st[(f + 1) * 5 + c];
                         }
                         artra::face_state<S>(mdl, UL, UR, fs);
                         if (mdl.sw) {
@@ -771,20 +796,20 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                         }
                     }
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                    for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
                 } else {
                     const int it = e - F, qi = it / 5, k = it - 5 * qi;
                     const int f = la.lo - 1 + queue[qi];
                     double UL[5], UR[5];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) {
-                        UL[c] = st[c * n + f];
-                        UR[c] = st[c * n + f + 1];
+                        UL[c] = st[f * 5 + c];
+                        UR[c] = st[(f + 1) * 5 + c];
                     }
                     artra::FaceState fs;
                     artra::face_state<S>(mdl, UL, UR, fs);
                     artra::face_scales(fs);
-                    artra::weno_field_parts<S>(fs, st, n, f, k, mdl.sw != 0, hbuf[10 * qi + 2 * k],
+                    artra::weno_field_parts<S>(fs, st, f, k, mdl.sw != 0, hbuf[10 * qi + 2 * k],
                                                hbuf[10 * qi + 2 * k + 1]);
                 }
             }
@@ -799,15 +824,15 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 double UL[5], UR[5], h[5], Fx[5];
 #pragma unroll
                 for (int c = 0; c < 5; ++c) {
-                    UL[c] = st[c * n + f];
-                    UR[c] = st[c * n + f + 1];
+                    UL[c] = st[f * 5 + c];
+                    UR[c] = st[(f + 1) * 5 + c];
                 }
                 artra::fields_from_parts(hbuf + 10 * e, h);
                 artra::FaceState fs;
                 artra::face_state<S>(mdl, UL, UR, fs);
                 artra::project_back<S>(fs, h, Fx);
 #pragma unroll
-                for (int c = 0; c < 5; ++c) fb[c * F + q] = Fx[c];
+                for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
             }
         }
         __syncthreads();
@@ -821,11 +846,11 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             const int q = i - la.lo;
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
-                const double FL = fb[c * F + q], FR = fb[c * F + q + 1];
+                const double FL = fb[q * 5 + c], FR = fb[(q + 1) * 5 + c];
                 const double phi = FAST ? FL - FR : ((FL - FR) + 0.0) - 0.0;
-                const double Un = st[c * n + i];
-                const double Uo = (s == 0) ? Un : HYB ? u0[c * (F - 1) + q] : U0[k][c];
-                st[c * n + i] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
+                const double Un = st[i * 5 + c];
+                const double Uo = (s == 0) ? Un : u0[q * 5 + c];
+                st[i * 5 + c] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
             }
             if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, false);
         }
@@ -837,12 +862,12 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 double v[5];
                 if (q >= 0) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) v[c] = st[c * n + q];
+                    for (int c = 0; c < 5; ++c) v[c] = st[q * 5 + c];
                 } else {
                     Conservative(la.gamma, la.given[-1 - q], v);
                 }
 #pragma unroll
-                for (int c = 0; c < 5; ++c) st[c * n + d] = v[c];
+                for (int c = 0; c < 5; ++c) st[d * 5 + c] = v[c];
                 if (HYB) LineNodeQuantities(mdl, st, n, d, nodeV, nodeRP, false);
             }
             __syncthreads();
@@ -854,15 +879,16 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         if (!fl[k]) continue;
         const int i = la.lo + tt + T * k;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[c * n + i];
+        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[i * 5 + c];
     }
 }
 
 // One line per block.
 template <int S, int NS, bool FAST, bool HYB, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : 2))) void rk_line(LineArgs la, FluxModel mdl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : FAST ? 4 : 2))) void rk_line(LineArgs la, FluxModel mdl,
                                                                                         LineHybrid hy)
 {
+    if (la.dt && *la.dt == 0.0) return;             // a device-stepped march past its end time
     extern __shared__ double lds[];
     __shared__ unsigned long long tally[3];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
@@ -1107,32 +1133,22 @@ __global__ __launch_bounds__(256) void cfl_batch(const CflChunk *chunks, int nch
     }
 }
 
-constexpr int kClock = 6;           // doubles per case in the batch clock
+constexpr int kClock = 6;           // doubles per case clock: now, end, rcInt, tmInt, stepC, stepN
 
-// Time step of every case from its speeds, with the solver loop's
-// rectification to output instants and the end time (solve.c:65-78).
-// clock[c] = {now, end, rcInt, tmInt, stepC, stepN}; half[c] receives dt / 2,
-// the step of each Strang-split directional operator.  A case that has
-// reached its end time or its step limit stays as it is (dt = 0), and its
-// step count only advances with the case, as in the solver loop.
-__global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long *vmax, double *clock, double *dt,
-                         double *half)
+// One step of the solver loop's clock (solve.c:59-78): dt = CFL min_s(d_s /
+// vmax_s), rectified to the output instants and the end time.  ck = {now,
+// end, rcInt, tmInt, stepC, stepN}.  A clock that has reached its end time
+// or its step limit stays as it is and gives dt = 0; the step count only
+// advances with the clock, as in the solver loop.
+__device__ __forceinline__ double AdvanceClock(const double d[3], double cfl, const unsigned long long *vmax, double *ck)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    const BatchCase &bc = cases[c];
-    double *ck = clock + kClock * c;
-    if (!(ck[0] < ck[1] && ck[4] < ck[5])) {      // reached its end time or step limit: the case stays as it is
-        dt[c] = 0.0;
-        half[c] = 0.0;
-        return;
-    }
+    if (!(ck[0] < ck[1] && ck[4] < ck[5])) return 0.0;
     double v[3];
 #pragma unroll
-    for (int s = 0; s < 3; ++s) v[s] = __longlong_as_double((long long)vmax[3 * c + s]);
-    const double tx = bc.d[0] / v[0], ty = bc.d[1] / v[1], tz = bc.d[2] / v[2];
+    for (int s = 0; s < 3; ++s) v[s] = __longlong_as_double((long long)vmax[s]);
+    const double tx = d[0] / v[0], ty = d[1] / v[1], tz = d[2] / v[2];
     const double tyz = (ty < tz) ? ty : tz;
-    double h = bc.cfl * ((tx < tyz) ? tx : tyz);
+    double h = cfl * ((tx < tyz) ? tx : tyz);
     double now = ck[0], rc = ck[2];
     const double end = ck[1], tm = ck[3];
     if (rc + h > tm) {
@@ -1149,8 +1165,37 @@ __global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long
     ck[0] = now;
     ck[2] = rc;
     ck[4] = ck[4] + 1.0;
+    return h;
+}
+
+// Time step of every case of a batch; half[c] receives dt / 2, the step of
+// each Strang-split directional operator.
+__global__ void dt_batch(const BatchCase *cases, int C, const unsigned long long *vmax, double *clock, double *dt,
+                         double *half)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const BatchCase &bc = cases[c];
+    const double h = AdvanceClock(bc.d, bc.cfl, vmax + 3 * c, clock + kClock * c);
     dt[c] = h;
     half[c] = 0.5 * h;
+}
+
+// Cell sizes and CFL number of a single-case device march.
+struct StepGeometry {
+    double d[3];
+    double cfl;
+};
+
+// Time step of a single case on the device: step[0] = dt, step[1] = dt / 2
+// (the operators' step), hist[0] = dt (the march's record of its steps).
+__global__ void dt_single(StepGeometry sg, const unsigned long long *vmax, double *clock, double *step, double *hist)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double h = AdvanceClock(sg.d, sg.cfl, vmax, clock);
+    step[0] = h;
+    step[1] = 0.5 * h;
+    hist[0] = h;
 }
 
 struct GatherCase {
@@ -1196,6 +1241,15 @@ __global__ void fill_u64(unsigned long long *p, int n, unsigned long long v)
     if (t < n) p[t] = v;
 }
 
+// The three per-axis speed maxima set to their floors (the body speeds).
+__global__ void init_speeds(unsigned long long *vmax, unsigned long long x, unsigned long long y, unsigned long long z)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    vmax[0] = x;
+    vmax[1] = y;
+    vmax[2] = z;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- device state
@@ -1205,6 +1259,8 @@ struct ArtraDevice {
     double *U = nullptr;            // [3][5][N]
     uint8_t *fluid = nullptr;
     unsigned long long *vmax = nullptr;
+    double *marchBuf = nullptr;     // device march: speeds, clock, step, step record (kMarchChunk)
+    const double *stepDev = nullptr; // when set, the operators read their step from here (device march)
     hipStream_t stream = nullptr;
     std::vector<BcGroup> groups[4];          // mode 0: full fill; 1 + s: faces of axis s only
     BcArgs bc{};
@@ -1335,6 +1391,8 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
         sa.accumulate = (dev->par.multidim == 1 && s != 0);
         sa.toIsTn = (to == tn);
         for (int q = 0; q < 3; ++q) sa.dd[q] = dev->par.dd[q];
+        sa.step = dev->stepDev;
+        sa.dds = dev->par.dd[s];
         int slot;
         if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
         const bool visc = dev->mdl.refMu > 0.0;
@@ -1508,7 +1566,7 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
         la.a[2] = 1.0 / 3.0; la.b[2] = 2.0 / 3.0;
     }
     la.r = dt * dev->par.dd[p];
-    la.dt = nullptr;
+    la.dt = dev->stepDev;
     la.dd = dev->par.dd[p];
     la.gamma = dev->par.gamma;
     const ArtraDevice::LineRules &R = dev->lines[p];
@@ -1648,6 +1706,8 @@ int SourceStage(ArtraDevice *dev, double dt, double a, double b, int to, int tn,
     sa.b = b;
     sa.r = dt;
     sa.toIsTn = (to == tn);
+    sa.step = dev->stepDev;
+    sa.dds = 1.0;
     int slot;
     if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
     const dim3 grid((unsigned)((total + 255) / 256));
@@ -1673,6 +1733,85 @@ int SourceStages(ArtraDevice *dev, double dt)
     if (SourceStage(dev, dt, 0.0, 1.0, 0, 0, 1) || TreatBoundary(dev, 1)) return 1;
     if (SourceStage(dev, dt, 3.0 / 4.0, 1.0 / 4.0, 0, 1, 2) || TreatBoundary(dev, 2)) return 1;
     return SourceStage(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0) || TreatBoundary(dev, 0);
+}
+
+// Per-axis speed maxima of level TO folded with the floors (body speeds),
+// into vmax on the device (solve.c:108-153).
+int EnqueueSpeeds(ArtraDevice *dev, const double floor[3], unsigned long long *vmax)
+{
+    unsigned long long init[3];
+    for (int s = 0; s < 3; ++s) {
+        const double f = (floor[s] > 0.0) ? floor[s] : 0.0;
+        memcpy(init + s, &f, sizeof f);
+    }
+    int slot;
+    if (Begin(dev, ARTRA_T_CFL, &slot)) return 1;
+    hipLaunchKernelGGL(init_speeds, dim3(1), dim3(1), 0, dev->stream, vmax, init[0], init[1], init[2]);
+    const Grid &g = dev->g;
+    const long total = (long)(g.hi[0] - g.lo[0]) * (g.hi[1] - g.lo[1]) * (g.hi[2] - g.lo[2]);
+    long blocks = (total + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(cfl_max, dim3((unsigned)blocks), dim3(256), 0, dev->stream, Level(dev, 0), dev->fluid, g,
+                       dev->par.gamma, dev->par.gasR, vmax);
+    HIP_TRY(hipGetLastError());
+    return End(dev, slot);
+}
+
+// Device memory of a device-stepped march: the speed maxima (3 x u64, one
+// spare), the clock (kClock), the step (dt, dt / 2) and the record of up to
+// cap steps, all 8-byte words.
+struct MarchScratch {
+    unsigned long long *vmax;
+    double *clock;
+    double *step;
+    double *hist;
+    int cap;
+};
+
+MarchScratch MarchScratchAt(void *base, int cap)
+{
+    MarchScratch m;
+    m.vmax = (unsigned long long *)base;
+    m.clock = (double *)base + 4;
+    m.step = m.clock + kClock;
+    m.hist = m.step + 2;
+    m.cap = cap;
+    return m;
+}
+
+// steps time steps with the dt computed on the device: per step the CFL
+// maxima, the clock's step (dt_single) and EvolveFluidDynamics reading its
+// step from device memory, with no host round trip in between.  clock
+// (in/out) is the solver loop's clock, dts[i] the steps taken (0 once the
+// clock has reached its end), *advanced the steps that moved the clock.
+int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kClock], const double floor[3], double cfl,
+               int steps, double *dts, int *advanced)
+{
+    *advanced = 0;
+    if (steps <= 0) return 0;
+    if (steps > ms.cap) return Fail("a device march takes at most %d steps per call (%d asked)", ms.cap, steps);
+    const double step0 = clock[4];
+    HIP_TRY(hipMemcpyAsync(ms.clock, clock, kClock * sizeof(double), hipMemcpyHostToDevice, dev->stream));
+    StepGeometry sg;
+    for (int s = 0; s < 3; ++s) sg.d[s] = dev->par.d[s];
+    sg.cfl = cfl;
+    dev->stepDev = ms.step + 1;
+    int rc = 0;
+    for (int i = 0; i < steps && !rc; ++i) {
+        rc = EnqueueSpeeds(dev, floor, ms.vmax);
+        if (rc) break;
+        hipLaunchKernelGGL(dt_single, dim3(1), dim3(1), 0, dev->stream, sg, ms.vmax, ms.clock, ms.step, ms.hist + i);
+        if (hipGetLastError() != hipSuccess) rc = Fail("dt_single launch failed");
+        if (!rc) rc = artra_dev_evolve(dev, 0.0);
+    }
+    dev->stepDev = nullptr;
+    if (rc) return 1;
+    HIP_TRY(hipMemcpyAsync(clock, ms.clock, kClock * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipMemcpyAsync(dts, ms.hist, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    *advanced = (int)(clock[4] - step0);
+    return Harvest(dev);
 }
 
 }  // namespace
@@ -1711,6 +1850,7 @@ int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevic
     if (e == hipSuccess) e = hipMalloc(&dev->U, fieldBytes);
     if (e == hipSuccess) e = hipMalloc(&dev->fluid, (size_t)g.N);
     if (e == hipSuccess) e = hipMalloc(&dev->vmax, 4 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&dev->marchBuf, MarchScratchBytes(kMarchChunk));
     if (e == hipSuccess) e = hipMemcpy(dev->fluid, fluid, (size_t)g.N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         artra_dev_destroy(dev);
@@ -1748,6 +1888,7 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->U);
     hipFree(dev->fluid);
     hipFree(dev->vmax);
+    hipFree(dev->marchBuf);
     hipFree(dev->ghosts);
     hipFree(dev->snode);
     hipFree(dev->sw);
@@ -1907,30 +2048,18 @@ int artra_dev_treat_immersed(ArtraDevice *dev, int level)
 
 int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 {
-    unsigned long long init[3];
-    for (int s = 0; s < 3; ++s) {
-        const double f = (floor[s] > 0.0) ? floor[s] : 0.0;
-        memcpy(init + s, &f, sizeof f);
-    }
-    int slot;
-    if (Begin(dev, ARTRA_T_CFL, &slot)) return 1;
-    for (int s = 0; s < 3; ++s) {
-        hipLaunchKernelGGL(fill_u64, dim3(1), dim3(1), 0, dev->stream, dev->vmax + s, 1, init[s]);
-    }
-    const Grid &g = dev->g;
-    const long total = (long)(g.hi[0] - g.lo[0]) * (g.hi[1] - g.lo[1]) * (g.hi[2] - g.lo[2]);
-    long blocks = (total + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(cfl_max, dim3((unsigned)blocks), dim3(256), 0, dev->stream, Level(dev, 0), dev->fluid, g,
-                       dev->par.gamma, dev->par.gasR, dev->vmax);
-    HIP_TRY(hipGetLastError());
-    if (End(dev, slot)) return 1;
+    if (EnqueueSpeeds(dev, floor, dev->vmax)) return 1;
     unsigned long long host[3];
     HIP_TRY(hipMemcpyAsync(host, dev->vmax, sizeof host, hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     for (int s = 0; s < 3; ++s) memcpy(vmax + s, host + s, sizeof(double));
     return Harvest(dev);
+}
+
+int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[3], double cfl, int steps,
+                          double *dts, int *advanced)
+{
+    return MarchSteps(dev, MarchScratchAt(dev->marchBuf, kMarchChunk), clock, floor, cfl, steps, dts, advanced);
 }
 
 int artra_dev_evolve(ArtraDevice *dev, double dt)

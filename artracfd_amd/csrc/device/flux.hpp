@@ -288,63 +288,60 @@ __host__ __device__ __forceinline__ void viscous_flux(const FluxModel &mdl, cons
 //    eigenvectors: with K = q U0 - vel.mom + U4 and M = un U0 - U_{1+S},
 //    W = (bK + dM, U0 - 2bK [entropy] / U_{1+t} - 2b vel_t K [shear], bK - dM),
 //    12 instead of 50 operations per stencil node;
-//  * WENO weights with a single division: with d_k = (eps + IS_k)^2,
-//    omega_k = C_k prod_{j!=k} d_j / sum_l C_l prod_{j!=l} d_j;
+//  * WENO weights without their own divisions: with d_k = (eps + IS_k)^2,
+//    omega_k = C_k prod_{j!=k} d_j / sum_l C_l prod_{j!=l} d_j, and the two
+//    split sides of a field summed over one common denominator
+//    (weno5_field_parts);
 //  * fused multiply-adds throughout.
 namespace artra {
 namespace fast {
 
-// WENO5 of one split side as the numerator and denominator of
-// (t0 Q0 + t1 Q1 + t2 Q2) / (6 (t0 + t1 + t2)), t_k = C_k prod_{j != k} (eps + IS_j)^2.
-__host__ __device__ __forceinline__ void weno5_parts(double f0, double f1, double f2, double f3, double f4, double &num,
-                                                     double &den)
+// One characteristic field of the split WENO5 flux, h = WENO5(lp W0..lp W4)
+// + WENO5(ln W5..ln W1), from the field's six stencil values W0..W5 and its
+// split eigenvalues, as num / den: Jiang-Shu WENO5 (weno5 above) of the
+// scaled values lp W and ln W, arranged for fewer operations and one
+// division per field (the two sides over a common denominator):
+//  * the smoothness indicators are quadratic forms, IS(l f) = l^2 IS(f), so
+//    they are formed once on W (their second differences shared between the
+//    two sides) and scaled by lp^2 or ln^2;
+//  * 12 IS = 13 a^2 + 3 b^2 and eps are scaled by 12 together, a common
+//    factor that cancels in the weights t_k = C_k prod_{j != k} (eps + IS_j)^2
+//    (C scaled by 10 likewise);
+//  * the candidate values are the central node plus differences,
+//    6 q_k = 6 W2 + R_k (6 W3 + R_k on the mirrored side), so
+//    h = lp W2 + ln W3 + (lp N+ D- + ln N- D+) / (6 D+ D-), N = sum t_k R_k,
+//    D = sum t_k, returned as num = base * den + (...).
+// den >= 6 (1.2e-5)^8: no underflow; it overflows only for smoothness
+// indicators far beyond any flow state (about 1e18).
+__host__ __device__ __forceinline__ void weno5_field_parts(const double W[6], double lp, double ln, double &num,
+                                                           double &den)
 {
-    const double eps = 1.0e-6;
-    const double a0 = fma(-2.0, f1, f0 + f2), b0 = fma(3.0, f2, fma(-4.0, f1, f0));
-    const double a1 = fma(-2.0, f2, f1 + f3), b1 = f1 - f3;
-    const double a2 = fma(-2.0, f3, f2 + f4), b2 = fma(3.0, f2, fma(-4.0, f3, f4));
-    const double IS0 = fma(13.0 / 12.0 * a0, a0, 0.25 * b0 * b0);
-    const double IS1 = fma(13.0 / 12.0 * a1, a1, 0.25 * b1 * b1);
-    const double IS2 = fma(13.0 / 12.0 * a2, a2, 0.25 * b2 * b2);
-    const double e0 = eps + IS0, e1 = eps + IS1, e2 = eps + IS2;
-    const double d0 = e0 * e0, d1 = e1 * e1, d2 = e2 * e2;
-    const double t0 = 0.1 * (d1 * d2), t1 = 0.6 * (d0 * d2), t2 = 0.3 * (d0 * d1);
-    const double Q0 = fma(2.0, f0, fma(-7.0, f1, 11.0 * f2));
-    const double Q1 = fma(5.0, f2, fma(2.0, f3, -f1));
-    const double Q2 = fma(2.0, f2, fma(5.0, f3, -f4));
-    num = fma(t0, Q0, fma(t1, Q1, t2 * Q2));
-    den = 6.0 * (t0 + t1 + t2);
-}
-
-__host__ __device__ __forceinline__ double weno5(double f0, double f1, double f2, double f3, double f4)
-{
-    double num, den;
-    weno5_parts(f0, f1, f2, f3, f4, num, den);
-    return num / den;
-}
-
-// The sum of the two split sides of one characteristic field over one
-// common denominator, n+/d+ + n-/d- = (n+ d- + n- d+) / (d+ d-): one
-// division instead of two.  d+ and d- are at least 6e-25 (every eps + IS is
-// at least 1e-6), so the product cannot underflow; it would overflow only
-// for smoothness indicators above about 1e38.
-__host__ __device__ __forceinline__ void weno5_sum_parts(double p0, double p1, double p2, double p3, double p4,
-                                                         double m0, double m1, double m2, double m3, double m4,
-                                                         double &num, double &den)
-{
-    double np, dp, nm, dm;
-    weno5_parts(p0, p1, p2, p3, p4, np, dp);
-    weno5_parts(m0, m1, m2, m3, m4, nm, dm);
-    num = fma(np, dm, nm * dp);
-    den = dp * dm;
-}
-
-__host__ __device__ __forceinline__ double weno5_sum(double p0, double p1, double p2, double p3, double p4, double m0,
-                                                     double m1, double m2, double m3, double m4)
-{
-    double num, den;
-    weno5_sum_parts(p0, p1, p2, p3, p4, m0, m1, m2, m3, m4, num, den);
-    return num / den;
+    const double eps12 = 12.0e-6;
+    const double d0 = W[1] - W[0], d1 = W[2] - W[1], d2 = W[3] - W[2], d3 = W[4] - W[3], d4 = W[5] - W[4];
+    const double D0 = d1 - d0, D1 = d2 - d1, D2 = d3 - d2, D3 = d4 - d3;   // second differences
+    const double A0 = (13.0 * D0) * D0, A1 = (13.0 * D1) * D1, A2 = (13.0 * D2) * D2, A3 = (13.0 * D3) * D3;
+    // first-derivative terms of the six sub-stencils: plus side (W0..W4), minus side (W5..W1)
+    const double bp0 = fma(2.0, d1, D0), bp1 = d1 + d2, bp2 = fma(-2.0, d2, D2);
+    const double bm0 = fma(-2.0, d3, D3), bm1 = d2 + d3, bm2 = fma(2.0, d2, D1);
+    const double lp2 = lp * lp, ln2 = ln * ln;
+    const double ep0 = fma(lp2, fma(3.0 * bp0, bp0, A0), eps12);
+    const double ep1 = fma(lp2, fma(3.0 * bp1, bp1, A1), eps12);
+    const double ep2 = fma(lp2, fma(3.0 * bp2, bp2, A2), eps12);
+    const double em0 = fma(ln2, fma(3.0 * bm0, bm0, A3), eps12);
+    const double em1 = fma(ln2, fma(3.0 * bm1, bm1, A2), eps12);
+    const double em2 = fma(ln2, fma(3.0 * bm2, bm2, A1), eps12);
+    const double gp0 = ep0 * ep0, gp1 = ep1 * ep1, gp2 = ep2 * ep2;
+    const double gm0 = em0 * em0, gm1 = em1 * em1, gm2 = em2 * em2;
+    const double tp0 = gp1 * gp2, tp1 = 6.0 * (gp0 * gp2), tp2 = 3.0 * (gp0 * gp1);
+    const double tm0 = gm1 * gm2, tm1 = 6.0 * (gm0 * gm2), tm2 = 3.0 * (gm0 * gm1);
+    // 6 q_k - 6 (central node) of each side's candidates
+    const double Rp0 = fma(5.0, d1, -2.0 * d0), Rp1 = fma(2.0, d2, d1), Rp2 = fma(4.0, d2, -d3);
+    const double Rm0 = fma(2.0, d4, -5.0 * d3), Rm1 = -fma(2.0, d2, d3), Rm2 = fma(-4.0, d2, d1);
+    const double Np = fma(tp0, Rp0, fma(tp1, Rp1, tp2 * Rp2)), Dp = tp0 + tp1 + tp2;
+    const double Nm = fma(tm0, Rm0, fma(tm1, Rm1, tm2 * Rm2)), Dm = tm0 + tm1 + tm2;
+    den = 6.0 * (Dp * Dm);
+    const double base = fma(lp, W[2], ln * W[3]);
+    num = fma(base, den, fma(lp * Np, Dm, (ln * Nm) * Dp));
 }
 
 // Two quotients from one division: a/b and c/d as a (d q) and c (b q), q = 1/(b d).
@@ -437,13 +434,10 @@ __host__ __device__ __forceinline__ void convective_flux(const FluxModel &mdl, c
     if (NS == 6) {
         double num[5], den[5];
 #pragma unroll
-        for (int r = 0; r < 5; ++r)
-            weno5_sum_parts(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2], lp[r] * W[r][3], lp[r] * W[r][4],
-                            ln[r] * W[r][5], ln[r] * W[r][4], ln[r] * W[r][3], ln[r] * W[r][2], ln[r] * W[r][1],
-                            num[r], den[r]);
-        divide_pair(num[0], den[0], num[1], den[1], h[0], h[1]);
-        divide_pair(num[2], den[2], num[3], den[3], h[2], h[3]);
-        h[4] = num[4] / den[4];
+        for (int r = 0; r < 5; ++r) weno5_field_parts(W[r], lp[r], ln[r], num[r], den[r]);
+        divide_pair(num[0], den[0], num[4], den[4], h[0], h[4]);
+        divide_pair(num[1], den[1], num[2], den[2], h[1], h[2]);
+        h[3] = num[3] / den[3];
     } else {
 #pragma unroll
         for (int r = 0; r < 5; ++r) {

@@ -108,7 +108,7 @@ __device__ __forceinline__ void face_features(double gamma, const double st[NS][
     features_from(rho, p, x);
 }
 
-// sum_k |omega_k - C_k| of one WENO5 evaluation (same weights as fast::weno5).
+// sum_k |omega_k - C_k| of one WENO5 evaluation (Jiang-Shu weights, C = 0.1, 0.6, 0.3).
 __host__ __device__ __forceinline__ double weno5_weight_deviation(double f0, double f1, double f2, double f3, double f4)
 {
     const double eps = 1.0e-6;
@@ -336,10 +336,10 @@ __host__ __device__ __forceinline__ void linear_flux(const FaceState &fs, const 
 
 // The WENO5 sum of one characteristic field of the face flux, h_r =
 // WENO5(lp W_r) over the nodes f-2 .. f+2 plus WENO5(ln W_r) over the
-// mirrored nodes f+3 .. f-1 (for a line held as [5][n] in memory), as a
+// mirrored nodes f+3 .. f-1 (for a line held node-major, [n][5]), as a
 // numerator and denominator (fs needs face_scales applied).
 template <int S>
-__device__ __forceinline__ void weno_field_parts(const FaceState &fs, const double *st, int n, int f, int r, bool sw,
+__device__ __forceinline__ void weno_field_parts(const FaceState &fs, const double *st, int f, int r, bool sw,
                                                  double &num, double &den)
 {
     const double vel[3] = {fs.u, fs.v, fs.w};
@@ -364,27 +364,83 @@ __device__ __forceinline__ void weno_field_parts(const FaceState &fs, const doub
 #pragma unroll
     for (int m = 0; m < 6; ++m) {
         const int node = f - 2 + m;
-        const double U0 = st[node], U1 = st[n + node], U2 = st[2 * n + node], U3 = st[3 * n + node],
-                     U4 = st[4 * n + node];
+        const double *Un = st + node * 5;
+        const double U0 = Un[0], U1 = Un[1], U2 = Un[2], U3 = Un[3], U4 = Un[4];
         const double K = fma(fs.q, U0, U4) - fma(fs.u, U1, fma(fs.v, U2, fs.w * U3));
-        const double M = fma(fs.un, U0, -st[(1 + S) * n + node]);
+        const double M = fma(fs.un, U0, -Un[1 + S]);
         const double bK = fs.b * K, dM = fs.d * M;
-        const double Ui = st[idx * n + node];
+        const double Ui = Un[idx];
         w[m] = edge ? bK + sgn * dM : fma(cK, K, Ui);
     }
-    fast::weno5_sum_parts(lp * w[0], lp * w[1], lp * w[2], lp * w[3], lp * w[4], ln * w[5], ln * w[4], ln * w[3],
-                          ln * w[2], ln * w[1], num, den);
+    fast::weno5_field_parts(w, lp, ln, num, den);
 }
 
-// The field sums of a queued face from their numerators and denominators
+// The field sums of a face from their numerators and denominators
 // num/den[r] = parts[2 r], parts[2 r + 1], with the divisions
-// fast::convective_flux makes (fields 0 and 1, and 2 and 3, share one), so
+// fast::convective_flux makes (fields 0 and 4, and 1 and 2, share one), so
 // that h is bit-identical to it.
-__device__ __forceinline__ void fields_from_parts(const double *parts, double h[5])
+__host__ __device__ __forceinline__ void fields_from_parts(const double *parts, double h[5])
 {
-    fast::divide_pair(parts[0], parts[1], parts[2], parts[3], h[0], h[1]);
-    fast::divide_pair(parts[4], parts[5], parts[6], parts[7], h[2], h[3]);
-    h[4] = parts[8] / parts[9];
+    fast::divide_pair(parts[0], parts[1], parts[8], parts[9], h[0], h[4]);
+    fast::divide_pair(parts[2], parts[3], parts[4], parts[5], h[1], h[2]);
+    h[3] = parts[6] / parts[7];
+}
+
+// The fast WENO5 flux of the face between nodes f and f + 1 of a line held
+// node-major, [n][5] (in LDS), bit-identical to fast::convective_flux on the same
+// six nodes.  The fields are formed one at a time from the per-node
+// projections K and M (acoustic fields 0 and 4 first, then the entropy and
+// shear fields, whose node components are read again), so that only one
+// field's stencil values are live: the sweep then fits in 128 registers.
+template <int S>
+__device__ __forceinline__ void weno_flux_line(const FluxModel &mdl, const double *st, int f, double F[5])
+{
+    FaceState fs;
+    {
+        double UL[5], UR[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            UL[c] = st[f * 5 + c];
+            UR[c] = st[(f + 1) * 5 + c];
+        }
+        face_state<S>(mdl, UL, UR, fs);
+    }
+    face_scales(fs);
+    double K[6], M[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int node = f - 2 + m;
+        const double *Un = st + node * 5;
+        const double U0 = Un[0], U1 = Un[1], U2 = Un[2], U3 = Un[3], U4 = Un[4];
+        K[m] = fma(fs.q, U0, U4) - fma(fs.u, U1, fma(fs.v, U2, fs.w * U3));
+        M[m] = fma(fs.un, U0, -Un[1 + S]);
+    }
+    double parts[10];
+    {
+        double wa[6], wb[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            const double bK = fs.b * K[m], dM = fs.d * M[m];
+            wa[m] = bK + dM;
+            wb[m] = bK - dM;
+        }
+        fast::weno5_field_parts(wa, fs.lp[0], fs.ln[0], parts[0], parts[1]);
+        fast::weno5_field_parts(wb, fs.lp[4], fs.ln[4], parts[8], parts[9]);
+    }
+    const double vel[3] = {fs.u, fs.v, fs.w};
+    const double m2b = -2.0 * fs.b;
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const double cK = (t == S) ? m2b : m2b * vel[t];
+        const int idx = (t == S) ? 0 : 1 + t;
+        double w[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) w[m] = fma(cK, K[m], st[(f - 2 + m) * 5 + idx]);
+        fast::weno5_field_parts(w, fs.lp[1 + t], fs.ln[1 + t], parts[2 + 2 * t], parts[3 + 2 * t]);
+    }
+    double h[5];
+    fields_from_parts(parts, h);
+    project_back<S>(fs, h, F);
 }
 
 // Full WENO5 flux from the same face state (matches fast::convective_flux).
@@ -406,8 +462,7 @@ __host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
         const double lp = fs.lp[r], ln = fs.ln[r];
-        fast::weno5_sum_parts(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3], lp * W[r][4], ln * W[r][5],
-                              ln * W[r][4], ln * W[r][3], ln * W[r][2], ln * W[r][1], num[r], den[r]);
+        fast::weno5_field_parts(W[r], lp, ln, num[r], den[r]);
         if (label) {
             dev = fmax(dev, weno5_weight_deviation(lp * W[r][0], lp * W[r][1], lp * W[r][2], lp * W[r][3],
                                                    lp * W[r][4]));
@@ -416,9 +471,13 @@ __host__ __device__ __forceinline__ void weno_flux(FaceState fs, const double st
         }
     }
     if (label) *label = dev;
-    fast::divide_pair(num[0], den[0], num[1], den[1], h[0], h[1]);
-    fast::divide_pair(num[2], den[2], num[3], den[3], h[2], h[3]);
-    h[4] = num[4] / den[4];
+    double parts[10];
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        parts[2 * r] = num[r];
+        parts[2 * r + 1] = den[r];
+    }
+    fields_from_parts(parts, h);
     project_back<S>(fs, h, F);
 }
 

@@ -68,6 +68,14 @@ int artra_dev_treat_immersed(ArtraDevice *dev, int level);
 int artra_dev_max_speed(ArtraDevice *dev, const double floor[DIMS], double vmax[DIMS]);
 /* one EvolveFluidDynamics step of size dt */
 int artra_dev_evolve(ArtraDevice *dev, double dt);
+/* up to ARTRA_DEV_MARCH_CHUNK whole time steps with dt computed on the
+ * device (CFL maxima folded with the floors, the solver loop's clock
+ * clock = {now, end, rcInt, tmInt, stepC, stepN}, EvolveFluidDynamics),
+ * no host round trip between steps; clock is updated, dts[i] receives each
+ * step's dt (0 once the clock stopped), *advanced the steps that moved it */
+#define ARTRA_DEV_MARCH_CHUNK 64
+int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
+                          double *dts, int *advanced);
 int artra_dev_sync(ArtraDevice *dev);
 /* kernel timing: when on, every launch is bracketed by events on the
  * solver's stream; totals are in milliseconds. */

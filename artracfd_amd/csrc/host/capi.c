@@ -213,12 +213,12 @@ int artra_hip_treat_immersed_boundary(ArtraHipSolver *s, int tn)
     return artra_dev_treat_immersed(s->dev, tn) ? DevError() : 0;
 }
 
-int artra_hip_compute_dt(ArtraHipSolver *s, const Time *time, Real *dt)
+/* Body speeds of the CFL condition (solve.c:121-131): per axis the largest
+ * |V_s| + max(|W_t1|, |W_t2|) r over the polyhedra. */
+static void BodySpeedFloor(const ArtraHipSolver *s, double floor[DIMS])
 {
-    const Partition *part = &s->space->part;
     const Geometry *geo = &s->space->geo;
-    /* body speeds first (solve.c:121-131) */
-    double floor[DIMS] = {0.0, 0.0, 0.0};
+    floor[X] = floor[Y] = floor[Z] = 0.0;
     for (int n = 0; n < geo->totN; ++n) {
         const Polyhedron *p = geo->poly + n;
         const double V[DIMS] = {fabs(p->V[TO][X]) + fmax(fabs(p->W[TO][Y]), fabs(p->W[TO][Z])) * p->r,
@@ -228,6 +228,13 @@ int artra_hip_compute_dt(ArtraHipSolver *s, const Time *time, Real *dt)
             if (floor[d] < V[d]) floor[d] = V[d];
         }
     }
+}
+
+int artra_hip_compute_dt(ArtraHipSolver *s, const Time *time, Real *dt)
+{
+    const Partition *part = &s->space->part;
+    double floor[DIMS];
+    BodySpeedFloor(s, floor);
     double vmax[DIMS];
     if (artra_dev_max_speed(s->dev, floor, vmax)) return DevError();
     const double tx = part->d[X] / vmax[X], ty = part->d[Y] / vmax[Y], tz = part->d[Z] / vmax[Z];
@@ -241,31 +248,49 @@ int artra_hip_evolve(ArtraHipSolver *s, Real dt)
     return artra_dev_evolve(s->dev, dt) ? DevError() : 0;
 }
 
+/* Steps the device may take in one call without passing the end time: the
+ * remaining time over the last step, less a margin of two steps for a
+ * growing dt (a step past the end on the device is a no-op anyway). */
+static int ChunkSteps(const Time *time, Real lastDt, int left)
+{
+    int k = ARTRA_DEV_MARCH_CHUNK;
+    if (k > left) k = left;
+    if (time->stepN - time->stepC < k) k = time->stepN - time->stepC;
+    if (lastDt > 0.0) {
+        const double safe = (time->end - time->now) / lastDt - 2.0;
+        if (safe < (double)k) k = (safe >= 1.0) ? (int)safe : 1;
+    } else {
+        k = 1;
+    }
+    return (k < 1) ? 1 : k;
+}
+
 int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_done, Real *dts)
 {
     const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
-    Real rcInt = s->rcInt;
+    double floor[DIMS];
+    BodySpeedFloor(s, floor);
+    double rec[ARTRA_DEV_MARCH_CHUNK];
+    Real lastDt = 0.0;
     int done = 0;
+    /* the time loop of solve.c:59-107 with the dt of every step computed on
+     * the device (bit-identical to artra_hip_compute_dt and the loop's
+     * rectification), in chunks that the host checks between */
     while (time->now < time->end && time->stepC < time->stepN && done < max_steps) {
-        ++time->stepC;
-        Real dt;
-        if (artra_hip_compute_dt(s, time, &dt)) return 1;
-        if (rcInt + dt > tmInt) {
-            dt = tmInt - rcInt;
-            rcInt = 0.0;
-        } else {
-            rcInt = rcInt + dt;
+        const int k = ChunkSteps(time, lastDt, max_steps - done);
+        double clock[6] = {time->now, time->end, s->rcInt, tmInt, (double)time->stepC, (double)time->stepN};
+        int adv = 0;
+        if (artra_dev_march_steps(s->dev, clock, floor, time->numCFL, k, rec, &adv)) return DevError();
+        time->now = clock[0];
+        s->rcInt = clock[2];
+        time->stepC += adv;
+        for (int i = 0; i < adv; ++i) {
+            if (dts) dts[done + i] = rec[i];
         }
-        time->now = time->now + dt;
-        if (time->now > time->end) {
-            dt = time->end - (time->now - dt);
-            time->now = time->end;
-        }
-        if (artra_hip_evolve(s, dt)) return 1;
-        if (dts) dts[done] = dt;
-        ++done;
+        if (adv > 0) lastDt = rec[adv - 1];
+        done += adv;
+        if (adv < k) break;          /* the clock stopped inside the chunk */
     }
-    s->rcInt = rcInt;
     if (steps_done) *steps_done = done;
     return artra_hip_synchronize(s);
 }
