@@ -58,6 +58,7 @@ constexpr int kXBlock = 256;        // threads (faces) per sweep_x block
 constexpr int kMarchX = 64;         // x extent of a sweep_march block (one wavefront)
 constexpr int kMarchT = 4;          // transverse extent of a sweep_march block
 constexpr int kMaxGroup = 8;        // boxes per bc_group launch
+constexpr int kListBlocks = 768;    // persistent blocks of rk_line_list (3 per CU)
 constexpr int kMarchChunk = ARTRA_DEV_MARCH_CHUNK;   // steps per device-stepped march call (the record's length)
 
 // bytes of a device march's scratch with a record of cap steps (see MarchScratch)
@@ -908,6 +909,214 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : F
     }
 }
 
+// The learned hybrid's lean pass: one line per block, T >= interior nodes
+// (one node and at most two faces per thread).  The line is loaded and its
+// faces classified exactly as LineBody classifies them; a line with any
+// troubled face is not computed here but appended to defer[] (it is read
+// again by rk_line_list, the full hybrid kernel), every other line runs all
+// RK stages with the linear fifth-order flux only: no WENO5 code, no network
+// fragments past the classification, no face queue, LDS for the states, the
+// face fluxes, the fill rules and the FP32 features only.  Its faces are
+// bit-identical to the routed faces of LineBody (the same node quantities,
+// face state and closed-form flux), so a line's result does not depend on
+// the kernel that ran it.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rk_lean(LineArgs la, FluxModel mdl,
+                                                                                      LineHybrid hy, int *defer,
+                                                                                      int *ndefer,
+                                                                                      unsigned long long *deferTotal)
+{
+    if (la.dt && *la.dt == 0.0) return;             // a device-stepped march past its end time
+    extern __shared__ double lds[];
+    __shared__ float bias[3 * artra::kHidden];
+    __shared__ int troubledAny;
+    const int tt = threadIdx.x, T = blockDim.x;
+    const int n = la.n, F = la.hi - la.lo + 1;
+    double *st = lds;                                   // [n][5]
+    double *fb = st + 5 * n;                            // [F][5]
+    int *rules = (int *)(fb + 5 * F);                   // [2][kLineRules]
+    float *nodeRP = (float *)(rules + 2 * kLineRules);  // [2][n]
+    const long line = XcdLine(blockIdx.x, gridDim.x);
+    const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
+    const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
+    artra::MlpWave frags;
+    artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
+    for (int e = tt; e < 3 * artra::kHidden; e += T) bias[e] = hy.bias[e];
+    for (int p = tt; p < n; p += T) {
+        double v[5];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
+        artra::node_rho_p(mdl.gamma, v, nodeRP[p], nodeRP[n + p]);
+    }
+    for (int e = tt; e < la.nrules; e += T) {
+        rules[e] = la.rdst[e];
+        rules[kLineRules + e] = la.rsrc[e];
+    }
+    const int i = la.lo + tt;                       // this thread's node
+    const bool own = i < la.hi;
+    const bool fl = own && la.fluid[base + (long)i * la.stride];
+    if (tt == 0) troubledAny = 0;
+    __syncthreads();
+    // classification: the faces q = tt and q = tt + T (same trip count on
+    // every thread, whole wavefronts run the network)
+    const int M = (F + T - 1) / T;
+    for (int m = 0; m < M; ++m) {
+        const int q = tt + T * m;
+        const bool valid = q < F;
+        const int f = la.lo - 1 + (valid ? q : F - 1);
+        const bool smooth = hy.smoothVariation > 0.0f &&
+                            artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
+        bool troubled = false;
+        if (__ballot(valid && !smooth)) {
+            float x[artra::kFeat];
+            artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
+            const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
+            troubled = valid && !smooth && z > hy.threshold;
+        }
+        if (__ballot(troubled) && __lane_id() == 0) troubledAny = 1;
+    }
+    __syncthreads();
+    if (troubledAny) {
+        if (tt == 0) {
+            defer[atomicAdd(ndefer, 1)] = (int)line;
+            atomicAdd(deferTotal, 1ull);
+        }
+        return;
+    }
+    if (tt == 0) {
+        unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
+        atomicAdd(slot, (unsigned long long)F);
+        atomicAdd(slot + 1, (unsigned long long)F);
+    }
+    double U0[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) U0[c] = st[(own ? i : la.lo) * 5 + c];
+    const double r = Uniform(la.dt ? *la.dt * la.dd : la.r);
+    for (int s = 0; s < la.stages; ++s) {
+        for (int m = 0; m < M; ++m) {
+            const int q = tt + T * m;
+            if (q >= F) break;
+            const int f = la.lo - 1 + q;
+            double Sg[5], Dl[5], Fx[5];
+            {
+                double w[6][5];
+#pragma unroll
+                for (int j = 0; j < 6; ++j)
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
+                artra::linear_sum_diff(w, Sg, Dl);
+            }
+            double UL[5], UR[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                UL[c] = st[f * 5 + c];
+                UR[c] = st[(f + 1) * 5 + c];
+            }
+            artra::FaceState fs;
+            if (!mdl.roe && !mdl.sw) {
+                double nL[4], nR[4];
+                artra::node_velocity_enthalpy(mdl.gamma, UL, nL);
+                artra::node_velocity_enthalpy(mdl.gamma, UR, nR);
+                artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
+                artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+            } else {
+                artra::face_state<S>(mdl, UL, UR, fs);
+                if (mdl.sw) {
+                    double Up[5], Um[5];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) {
+                        Up[c] = 0.5 * (Sg[c] + Dl[c]);
+                        Um[c] = 0.5 * (Sg[c] - Dl[c]);
+                    }
+                    artra::linear_flux_from<S>(fs, Up, Um, Fx);
+                } else {
+                    artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
+        }
+        __syncthreads();
+        // node update (fluid_dynamics.c:243-274)
+        if (fl) {
+            const double a = la.a[s], b = la.b[s];
+            const int q = i - la.lo;
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double phi = fb[q * 5 + c] - fb[(q + 1) * 5 + c];
+                const double Un = st[i * 5 + c];
+                const double Uo = (s == 0) ? Un : U0[c];
+                st[i * 5 + c] = fma(a, Uo, b * fma(r, phi, Un));
+            }
+        }
+        __syncthreads();
+        if (s + 1 < la.stages) {
+            for (int e = tt; e < la.nrules; e += T) {
+                const int d = rules[e], q = rules[kLineRules + e];
+                double v[5];
+                if (q >= 0) {
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) v[c] = st[q * 5 + c];
+                } else {
+                    Conservative(la.gamma, la.given[-1 - q], v);
+                }
+#pragma unroll
+                for (int c = 0; c < 5; ++c) st[d * 5 + c] = v[c];
+            }
+            __syncthreads();
+        }
+    }
+    if (fl) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[i * 5 + c];
+    }
+}
+
+// LDS of rk_lean: states [n][5], face fluxes [F][5], fill rules, FP32
+// density and pressure [2][n].
+__host__ __device__ __forceinline__ size_t LeanLdsBytes(int n, int F)
+{
+    return (size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) +
+           (size_t)2 * n * sizeof(float);
+}
+
+// The full hybrid kernel over a list of lines (those rk_lean deferred):
+// blocks walk the list, whose length is read on the device.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void rk_line_list(LineArgs la,
+                                                                                          FluxModel mdl,
+                                                                                          LineHybrid hy,
+                                                                                          const int *list,
+                                                                                          const int *count)
+{
+    if (la.dt && *la.dt == 0.0) return;
+    const int total = *count;
+    if ((int)blockIdx.x >= total) return;
+    extern __shared__ double lds[];
+    __shared__ unsigned long long tally[3];
+    __shared__ float bias[3 * artra::kHidden];
+    for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
+    if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        // the network's fragments are loaded per line (cache hits) and die
+        // after its classification; an opaque memory effect keeps the loads
+        // from being hoisted out of the loop, where they would hold
+        // registers through every stage
+        asm volatile("" ::: "memory");
+        artra::MlpWave frags;
+        artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
+        LineBody<S, 6, true, true, 1>(la, mdl, hy, list[w], threadIdx.x, blockDim.x, (char *)lds, true, tally, bias,
+                                      frags);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        unsigned long long *slot = hy.counts + (blockIdx.x % kCountSlots) * kCountStride;
+        atomicAdd(slot, tally[0]);
+        atomicAdd(slot + 1, tally[1]);
+        atomicAdd(slot + 2, tally[2]);
+    }
+}
+
 // Many independent cases in one persistent launch: blocks of 256 threads
 // walk a tile list; a tile is P = 256 / T lines of one case, each run by T
 // threads (the tile shape picked per grid size).  Lines past the case's end
@@ -1284,6 +1493,13 @@ struct ArtraDevice {
     float smoothVariation = 1.0e-3f; // relative stencil variation below which the network is skipped
     float mlpB3 = 0.0f;
     unsigned long long *hybCounts = nullptr;
+    // lean pass of the hybrid (rk_lean, then rk_line_list over the deferred lines)
+    int *deferList = nullptr;       // lines the lean pass deferred (capacity: lines of the longest axis list)
+    int *deferCount = nullptr;      // [1] their number in the current launch
+    unsigned long long *deferTotal = nullptr;   // [3] deferred lines per axis since the last policy update
+    int leanFirst[3] = {1, 1, 1};   // per axis: run the lean pass first (1) or the hybrid kernel on every line (0)
+    int leanRetry[3] = {0, 0, 0};   // policy updates until an axis running the hybrid kernel directly tries again
+    long leanLines[3] = {0, 0, 0};  // lines sent through the lean pass since the last policy update
     float *mlpBias = nullptr;       // b1, b2, w3 contiguous (96 FP32)
     // fused directional sweeps (rk_line)
     int fused = 1;                  // allowed by the caller
@@ -1588,9 +1804,36 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
     }
     int slot;
     if (Begin(dev, ARTRA_T_SWEEP, &slot)) return 1;
-    if (p == 0) LaunchLineS<0>(dev, sh, la, hy);
-    else if (p == 1) LaunchLineS<1>(dev, sh, la, hy);
-    else LaunchLineS<2>(dev, sh, la, hy);
+    const size_t leanLds = LeanLdsBytes(la.n, la.hi - la.lo + 1);
+    if (dev->hybrid && sh.K == 1 && dev->leanFirst[p] && leanLds <= 64 * 1024) {
+        // lean pass over every line, then the full hybrid kernel over the
+        // lines it deferred (the list's length stays on the device)
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, dev->stream, dev->deferCount, 1L, 0);
+        const unsigned listGrid = (unsigned)std::min<long>(sh.lines, kListBlocks);
+        if (p == 0) {
+            hipLaunchKernelGGL(rk_lean<0>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
+                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
+            hipLaunchKernelGGL(rk_line_list<0>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
+                               dev->deferList, dev->deferCount);
+        } else if (p == 1) {
+            hipLaunchKernelGGL(rk_lean<1>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
+                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
+            hipLaunchKernelGGL(rk_line_list<1>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
+                               dev->deferList, dev->deferCount);
+        } else {
+            hipLaunchKernelGGL(rk_lean<2>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
+                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
+            hipLaunchKernelGGL(rk_line_list<2>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
+                               dev->deferList, dev->deferCount);
+        }
+        dev->leanLines[p] += sh.lines;
+    } else if (p == 0) {
+        LaunchLineS<0>(dev, sh, la, hy);
+    } else if (p == 1) {
+        LaunchLineS<1>(dev, sh, la, hy);
+    } else {
+        LaunchLineS<2>(dev, sh, la, hy);
+    }
     HIP_TRY(hipGetLastError());
     return End(dev, slot);
 }
@@ -1735,6 +1978,29 @@ int SourceStages(ArtraDevice *dev, double dt)
     return SourceStage(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0) || TreatBoundary(dev, 0);
 }
 
+// The lean pass's per-axis policy, from the lines it deferred since the
+// last update (the stream must be idle): an axis where most lines have
+// troubled faces (a shock across every line) runs the hybrid kernel on
+// every line, and tries the lean pass again a few updates later.
+int UpdateLeanPolicy(ArtraDevice *dev)
+{
+    if (!dev->deferTotal) return 0;
+    unsigned long long deferred[4];
+    HIP_TRY(hipMemcpy(deferred, dev->deferTotal, sizeof deferred, hipMemcpyDeviceToHost));
+    for (int q = 0; q < 3; ++q) {
+        if (dev->leanLines[q] > 0) {
+            dev->leanFirst[q] = 2 * (long)deferred[q] < dev->leanLines[q];
+            if (!dev->leanFirst[q]) dev->leanRetry[q] = 8;
+        } else if (!dev->leanFirst[q] && --dev->leanRetry[q] <= 0) {
+            dev->leanFirst[q] = 1;
+        }
+        dev->leanLines[q] = 0;
+    }
+    hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->deferTotal, 4, 0ull);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 // Per-axis speed maxima of level TO folded with the floors (body speeds),
 // into vmax on the device (solve.c:108-153).
 int EnqueueSpeeds(ArtraDevice *dev, const double floor[3], unsigned long long *vmax)
@@ -1811,6 +2077,7 @@ int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kClock], c
     HIP_TRY(hipMemcpyAsync(dts, ms.hist, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     *advanced = (int)(clock[4] - step0);
+    if (dev->hybrid && UpdateLeanPolicy(dev)) return 1;
     return Harvest(dev);
 }
 
@@ -1900,6 +2167,9 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->mlpW1);
     hipFree(dev->mlpW2);
     hipFree(dev->hybCounts);
+    hipFree(dev->deferList);
+    hipFree(dev->deferCount);
+    hipFree(dev->deferTotal);
     hipFree(dev->mlpBias);
     if (dev->stream) hipStreamDestroy(dev->stream);
     delete dev;
@@ -2164,6 +2434,17 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
         HIP_TRY(hipMalloc(&dev->mlpW2, 1024 * sizeof(uint16_t)));
         HIP_TRY(hipMalloc(&dev->mlpBias, 96 * sizeof(float)));
         HIP_TRY(hipMalloc(&dev->hybCounts, kCountBytes));
+        long most = 1;
+        for (int q = 0; q < 3; ++q) most = std::max(most, LineShapeOf(dev, q, true).lines);
+        HIP_TRY(hipMalloc(&dev->deferList, most * sizeof(int)));
+        HIP_TRY(hipMalloc(&dev->deferCount, 4 * sizeof(int)));
+        HIP_TRY(hipMalloc(&dev->deferTotal, 4 * sizeof(unsigned long long)));
+    }
+    HIP_TRY(hipMemset(dev->deferTotal, 0, 4 * sizeof(unsigned long long)));
+    for (int q = 0; q < 3; ++q) {
+        dev->leanFirst[q] = 1;
+        dev->leanRetry[q] = 0;
+        dev->leanLines[q] = 0;
     }
     std::vector<uint16_t> w1(512), w2(1024);
     for (int q = 0; q < 512; ++q) w1[q] = artra::f32_to_bf16(params[ARTRA_MLP_W1 + q]);
@@ -2228,6 +2509,7 @@ int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[3], int 
         return 0;
     }
     HIP_TRY(hipStreamSynchronize(dev->stream));
+    if (UpdateLeanPolicy(dev)) return 1;
     return SumCounts(dev->hybCounts, counts, reset);
 }
 
