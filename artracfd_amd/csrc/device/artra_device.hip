@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "flux.hpp"
@@ -66,6 +67,18 @@ constexpr size_t MarchScratchBytes(int cap)
 {
     return (size_t)(4 + 6 + 2 + cap) * sizeof(double);
 }
+
+// Scratch of the allocation-free march (artra_dev_march_noalloc): speed
+// maxima, clock, step and a record of up to kNoAllocSteps steps, laid out
+// as MarchScratch.  Every GPU the library runs on holds its own copy until
+// the process ends; calls that overlap on one GPU share it and race.
+constexpr int kNoAllocSteps = ARTRA_DEV_NOALLOC_STEPS;
+static __device__ __align__(256) double scratch[4 + 6 + 2 + kNoAllocSteps];
+static_assert(sizeof scratch == MarchScratchBytes(kNoAllocSteps), "scratch layout");
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchLock;
+void *gScratch[kMaxDevices];        // per GPU: the device address of its copy of scratch
 
 struct Grid {
     int nx, ny, nz;
@@ -1470,6 +1483,7 @@ struct ArtraDevice {
     unsigned long long *vmax = nullptr;
     double *marchBuf = nullptr;     // device march: speeds, clock, step, step record (kMarchChunk)
     const double *stepDev = nullptr; // when set, the operators read their step from here (device march)
+    int ordinal = 0;                // the GPU this solver lives on
     hipStream_t stream = nullptr;
     std::vector<BcGroup> groups[4];          // mode 0: full fill; 1 + s: faces of axis s only
     BcArgs bc{};
@@ -2095,7 +2109,21 @@ int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevic
     *out = nullptr;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return Fail("no HIP device is visible");
+    int ordinal = 0;
+    HIP_TRY(hipGetDevice(&ordinal));
+    if (ordinal < 0 || ordinal >= kMaxDevices) return Fail("device ordinal %d beyond the %d supported", ordinal, kMaxDevices);
+    {
+        // the address of this GPU's copy of the allocation-free march's
+        // scratch, looked up once per device (the current one)
+        std::lock_guard<std::mutex> hold(gScratchLock);
+        if (!gScratch[ordinal]) {
+            void *p = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            gScratch[ordinal] = p;
+        }
+    }
     ArtraDevice *dev = new ArtraDevice();
+    dev->ordinal = ordinal;
     dev->par = *par;
     Grid &g = dev->g;
     g.nx = par->n[0];
@@ -2330,6 +2358,19 @@ int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[
                           double *dts, int *advanced)
 {
     return MarchSteps(dev, MarchScratchAt(dev->marchBuf, kMarchChunk), clock, floor, cfl, steps, dts, advanced);
+}
+
+int artra_dev_march_noalloc(ArtraDevice *dev, double clock[6], const double floor[3], double cfl, int steps,
+                            double *dts, int *advanced)
+{
+    *advanced = 0;
+    if (steps < 0 || steps > kNoAllocSteps) {
+        return Fail("the allocation-free march takes 0 to %d steps per call (%d asked)", kNoAllocSteps, steps);
+    }
+    if (dev->timing) return Fail("kernel timing is on: its events are allocated, so the allocation-free march refuses it");
+    void *base = gScratch[dev->ordinal];
+    if (!base) return Fail("no scratch address for device %d", dev->ordinal);
+    return MarchSteps(dev, MarchScratchAt(base, kNoAllocSteps), clock, floor, cfl, steps, dts, advanced);
 }
 
 int artra_dev_evolve(ArtraDevice *dev, double dt)

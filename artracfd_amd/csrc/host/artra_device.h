@@ -76,6 +76,11 @@ int artra_dev_evolve(ArtraDevice *dev, double dt);
 #define ARTRA_DEV_MARCH_CHUNK 64
 int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
                           double *dts, int *advanced);
+/* the same with the device scratch in a static per-GPU array (no device
+ * allocation at all); at most ARTRA_DEV_NOALLOC_STEPS steps per call */
+#define ARTRA_DEV_NOALLOC_STEPS ARTRA_NOALLOC_MAX_STEPS
+int artra_dev_march_noalloc(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
+                            double *dts, int *advanced);
 int artra_dev_sync(ArtraDevice *dev);
 /* kernel timing: when on, every launch is bracketed by events on the
  * solver's stream; totals are in milliseconds. */

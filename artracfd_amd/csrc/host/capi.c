@@ -295,6 +295,34 @@ int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_don
     return artra_hip_synchronize(s);
 }
 
+int artra_hip_march_noalloc(ArtraHipSolver *s, Time *time, int max_steps, int *steps_done, Real *dts)
+{
+    if (steps_done) *steps_done = 0;
+    if (max_steps < 0 || max_steps > ARTRA_NOALLOC_MAX_STEPS) {
+        return artra_set_error(1, "artra_hip_march_noalloc: max_steps %d outside 0..%d", max_steps,
+                               ARTRA_NOALLOC_MAX_STEPS);
+    }
+    if (!(time->now < time->end && time->stepC < time->stepN) || max_steps == 0) return 0;
+    const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
+    double floor[DIMS];
+    BodySpeedFloor(s, floor);
+    int k = max_steps;
+    if (time->stepN - time->stepC < k) k = time->stepN - time->stepC;
+    /* every step in one device call: steps past the end time are no-ops */
+    double rec[ARTRA_NOALLOC_MAX_STEPS];
+    double clock[6] = {time->now, time->end, s->rcInt, tmInt, (double)time->stepC, (double)time->stepN};
+    int adv = 0;
+    if (artra_dev_march_noalloc(s->dev, clock, floor, time->numCFL, k, rec, &adv)) return DevError();
+    time->now = clock[0];
+    s->rcInt = clock[2];
+    time->stepC += adv;
+    for (int i = 0; i < adv; ++i) {
+        if (dts) dts[i] = rec[i];
+    }
+    if (steps_done) *steps_done = adv;
+    return 0;
+}
+
 int artra_hip_download(ArtraHipSolver *s, int tn)
 {
     if (artra_dev_download(s->dev, tn, s->staging)) return DevError();

@@ -85,6 +85,18 @@ int artra_hip_evolve(ArtraHipSolver *solver, Real dt);
  * max_steps steps were taken.  dts (may be NULL) receives each step's dt. */
 int artra_hip_march(ArtraHipSolver *solver, Time *time, int max_steps, int *steps_done, Real *dts);
 
+/* The time loop of artra_hip_march for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  Same steps, same dt
+ * sequence and same field as artra_hip_march, all max_steps steps enqueued
+ * in one call with no host round trip between them (steps past the end
+ * time change nothing).  Its device scratch (CFL speed maxima, clock, the
+ * record of the steps) is a static array of each GPU, not an allocation:
+ * max_steps is at most ARTRA_NOALLOC_MAX_STEPS, kernel timing must be off,
+ * and calls that overlap on one GPU share that array and race. */
+#define ARTRA_NOALLOC_MAX_STEPS 1024
+int artra_hip_march_noalloc(ArtraHipSolver *solver, Time *time, int max_steps, int *steps_done, Real *dts);
+
 /* Copy a time level between the device and the node array of space. */
 int artra_hip_download(ArtraHipSolver *solver, int tn);
 int artra_hip_upload(ArtraHipSolver *solver, int tn);
