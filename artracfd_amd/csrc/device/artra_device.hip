@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
 }
 
 template <int S, int NS, bool VISC, bool FAST>
-__global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArgs sa, FluxModel mdl, int segment)
+__global__ __launch_bounds__(kMarchX *kMarchT) __attribute__((amdgpu_waves_per_eu(2))) void sweep_march(Grid g, StageArgs sa, FluxModel mdl, int segment)
 {
     if (!ResolveStep(sa)) return;
     constexpr int SL = (NS == 6) ? -2 : -1;
@@ -282,12 +282,14 @@ __global__ __launch_bounds__(kMarchX *kMarchT) void sweep_march(Grid g, StageArg
         }
         LoadNode(sa.Utn, g.N, idx + (SL + NS - 1) * stride, st[NS - 1]);
         double FR[5], FvR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        ConvFlux<S, NS, FAST>(mdl, st, FR);
         if (VISC) {
+            // the viscous flux first: its neighbour values are dead before
+            // the characteristic projection needs its registers
             double nb[8][5];
             LoadViscousNeighbours<S>(g, sa.Utn, idx, nb);
             ViscFlux<S, FAST>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvR);
         }
+        ConvFlux<S, NS, FAST>(mdl, st, FR);
         if (sa.fluid[idx]) UpdateNode<VISC, FAST>(sa, g.N, idx, FL, FR, FvL, FvR, st[K0]);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
@@ -528,10 +530,8 @@ __global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__res
 // interior is written back once.  Valid where a stage along S reads nothing
 // but its own line: inviscid, split, no immersed body, and a between-stage
 // fill that only copies along the line (derived from the resolved gather).
-// With HYB the learned indicator routes each face: every face first takes
-// the linear upwind flux, faces flagged troubled are queued in LDS and
-// recomputed with full WENO5 by the whole block, so WENO5 work stays dense
-// across lanes however the troubled faces are scattered.
+// With HYB the learned indicator routes each face: smooth faces take the
+// linear fifth-order upwind flux, troubled ones the full WENO5 flux.
 constexpr int kLineRules = 12;
 // The face tallies of the indicator are spread over slots on separate
 // 128-byte lines (block b adds into slot b mod kCountSlots): one counter
@@ -590,20 +590,11 @@ __device__ __forceinline__ double Uniform(double x)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Troubled faces a line can queue: more than this and the line runs plain
-// WENO5 for the operator (nothing is gained by routing then).
-__host__ __device__ __forceinline__ int LineQueueCap(int F)
-{
-    return F / 8 > 16 ? F / 8 : 16;
-}
-
-// LDS of one line.  Every line: the states [5][n], the face fluxes [5][F],
-// the operator's initial state of the interior nodes [5][F - 1] and the fill
-// rules [2][kLineRules].  With the indicator (hyb) also the velocities and
-// total enthalpy of every node [4][n], the per-field WENO5 numerators and
-// denominators of the queued faces [cap][5][2], the queue [cap], each face's
-// queue position [F], the queue length, and the FP32 density and pressure
-// of every node [2][n].
+// LDS of one line.  Every line: the states [n][5], the face fluxes [F][5],
+// the operator's initial state of the interior nodes [F - 1][5] and the fill
+// rules [2][kLineRules].  With the indicator (hyb) also the FP32 density and
+// pressure of every node [2][n] (the network's features) and one troubled
+// flag per face [F].
 __host__ __device__ __forceinline__ size_t LinePlainBytes(int n, int F)
 {
     return ((size_t)(5 * n + 5 * F + 5 * (F - 1)) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) + 15) /
@@ -612,24 +603,77 @@ __host__ __device__ __forceinline__ size_t LinePlainBytes(int n, int F)
 
 __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool hyb)
 {
-    const int cap = LineQueueCap(F);
-    const size_t extra = (size_t)(4 * n + 10 * cap) * sizeof(double) + (size_t)(cap + F + 4) * sizeof(int) +
-                         (size_t)2 * n * sizeof(float);
+    const size_t extra = (size_t)2 * n * sizeof(float) + (size_t)F;
     return LinePlainBytes(n, F) + (hyb ? (extra + 15) / 16 * 16 : 0);
 }
 
-// Node quantities the routed faces use: velocities and total enthalpy (FP64)
-// and, when the indicator will classify, density and pressure (FP32).
-__device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const double *st, int n, int p, double *nodeV,
-                                                   float *nodeRP, bool features)
+// The linear fifth-order face flux of the learned hybrid at the face between
+// nodes f and f + 1 of a line held node-major in LDS: the six-node linear
+// combinations, the face state (arithmetic mean of the two nodes' velocities
+// and total enthalpies, or the model's average) and the closed-form local
+// Lax-Friedrichs flux (the characteristic form for Steger-Warming).  The
+// one definition every hybrid kernel uses, so that a face's flux does not
+// depend on the kernel that computed it.
+template <int S>
+__device__ __forceinline__ void LinearFaceFlux(const FluxModel &mdl, const double *st, int f, double Fx[5])
 {
-    double u[5], nv[4];
+    double Sg[5], Dl[5];
+    {
+        double w[6][5];
 #pragma unroll
-    for (int c = 0; c < 5; ++c) u[c] = st[p * 5 + c];
-    if (features) artra::node_rho_p(mdl.gamma, u, nodeRP[p], nodeRP[n + p]);
-    artra::node_velocity_enthalpy(mdl.gamma, u, nv);
+        for (int j = 0; j < 6; ++j)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) nodeV[p * 4 + c] = nv[c];
+            for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
+        artra::linear_sum_diff(w, Sg, Dl);
+    }
+    double UL[5], UR[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) {
+        UL[c] = st[f * 5 + c];
+        UR[c] = st[(f + 1) * 5 + c];
+    }
+    artra::FaceState fs;
+    if (!mdl.roe && !mdl.sw) {
+        double nL[4], nR[4];
+        artra::node_velocity_enthalpy(mdl.gamma, UL, nL);
+        artra::node_velocity_enthalpy(mdl.gamma, UR, nR);
+        artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
+        artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+    } else {
+        artra::face_state<S>(mdl, UL, UR, fs);
+        if (mdl.sw) {
+            double Up[5], Um[5];
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                Up[c] = 0.5 * (Sg[c] + Dl[c]);
+                Um[c] = 0.5 * (Sg[c] - Dl[c]);
+            }
+            artra::linear_flux_from<S>(fs, Up, Um, Fx);
+        } else {
+            artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
+        }
+    }
+}
+
+// Whether the face q of a line (between nodes lo - 1 + q and lo + q) is
+// troubled: its density and pressure stencils vary by more than the
+// smoothness gate allows and the network's logit exceeds the threshold.
+// Every lane of the wavefront must call it (the network runs on whole
+// wavefronts); valid = false for lanes past the last face.  A wavefront
+// whose faces are all near-uniform skips the network.
+__device__ __forceinline__ bool ClassifyFace(const LineHybrid &hy, const float *nodeRP, int n, int f, bool valid,
+                                             const artra::MlpWave &frags, const float *bias)
+{
+    const bool smooth = hy.smoothVariation > 0.0f &&
+                        artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
+    bool troubled = false;
+    if (__ballot(valid && !smooth)) {
+        float x[artra::kFeat];
+        artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
+        const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
+        troubled = valid && !smooth && z > hy.threshold;
+    }
+    return troubled;
 }
 
 // All stages of the directional operator on one line, run by the T threads
@@ -638,32 +682,27 @@ __device__ __forceinline__ void LineNodeQuantities(const FluxModel &mdl, const d
 // the stages are separated by block barriers.  With write = false the line
 // is computed but not stored (padding slots of a batched block).
 //
-// With HYB, the indicator classifies the faces once per operator into a
-// queue of troubled faces; each stage then runs one
-// balanced pass over F + 5 x queued work items - the linear flux of every
-// routed face and the five per-field WENO5 items (numerator and denominator
-// of each field's sum) of every queued face - a short pass dividing and
-// projecting the queued faces' fields back, and
-// the node update, which also refreshes the node quantities of the next
-// stage.  A line with more troubled faces than the queue holds runs the
-// plain WENO5 flux on every face for that operator.
+// With HYB, the indicator classifies the faces once per operator (at its
+// first RK stage, before the stage loop: the solution moves less than a
+// cell within one operator, and the network's fragments are then dead in
+// the flux passes) and every stage computes the linear flux on the smooth
+// faces and the full WENO5 flux on the troubled ones.  A wavefront with no
+// troubled face runs the linear flux only; one holding troubled faces runs
+// both branches for its lanes, so a line crossed by a shock pays full
+// WENO5 only on the wavefronts around the shock.
 template <int S, int NS, bool FAST, bool HYB, int K>
 __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &mdl, const LineHybrid &hy, long line,
                                          int tt, int T, char *mem, bool write, unsigned long long *tally,
                                          const float *bias, const artra::MlpWave &frags)
 {
     constexpr int SL = (NS == 6) ? 2 : 1;           // stencil nodes left of the face's left node
-    const int n = la.n, F = la.hi - la.lo + 1, cap = LineQueueCap(F);
+    const int n = la.n, F = la.hi - la.lo + 1;
     double *st = (double *)mem;                     // [n][5]
     double *fb = st + 5 * n;                        // [F][5]
     double *u0 = fb + 5 * F;                        // [F - 1][5]
     int *rules = (int *)(u0 + 5 * (F - 1));         // [2][kLineRules]: destinations, sources
-    double *nodeV = (double *)(mem + LinePlainBytes(n, F));  // [n][4] (HYB)
-    double *hbuf = nodeV + 4 * n;                   // [cap][5 fields][numerator, denominator]
-    int *queue = (int *)(hbuf + 10 * cap);          // [cap]
-    int *fidx = queue + cap;                        // [F]
-    int *queued = fidx + F;
-    float *nodeRP = (float *)(queued + 4);          // [2][n]
+    float *nodeRP = (float *)(mem + LinePlainBytes(n, F));   // [2][n] (HYB)
+    uint8_t *tflag = (uint8_t *)(nodeRP + 2 * n);   // [F] (HYB)
     const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
     for (int p = tt; p < n; p += T) {
@@ -674,7 +713,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 #pragma unroll
             for (int c = 0; c < 5; ++c) u0[(p - la.lo) * 5 + c] = v[c];
         }
-        if (HYB) LineNodeQuantities(mdl, st, n, p, nodeV, nodeRP, true);
+        if (HYB) artra::node_rho_p(mdl.gamma, v, nodeRP[p], nodeRP[n + p]);
     }
     // the line's fill rules into LDS: read per lane from the argument
     // block once here, where the field loads hide the latency
@@ -689,170 +728,55 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         own[k] = i < la.hi;
         fl[k] = own[k] && la.fluid[base + (long)i * la.stride];
     }
-    if (HYB && tt == 0) *queued = 0;
     __syncthreads();
     const int M = (F + T - 1) / T;
     const double r = Uniform(la.dt ? *la.dt * la.dd : la.r);
     if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
-    bool routed = false;                            // this operator routes faces (HYB, queue within cap)
-    int nq = 0;
-    // The indicator's pass over the faces of the line, once per directional
-    // operator before its stages (the solution moves less than a cell within
-    // one operator): the queue of troubled faces, each face's position in
-    // it, and whether the line routes (its queue within capacity; decided
-    // per line, so a line's result does not depend on the lines it shares a
-    // block with).  Every thread runs the same trip count (whole wavefronts
-    // run the network).  The weight fragments are dead after it, so they
-    // hold no registers in the flux passes.
-    auto classifyLine = [&](const artra::MlpWave &frags) {
+    if (HYB) {
+        // classification (every thread runs the same trip count)
+        unsigned long long troubledFaces = 0, faces = 0;
         for (int m = 0; m < M; ++m) {
             const int q = tt + T * m;
             const bool valid = q < F;
             const int f = la.lo - 1 + (valid ? q : F - 1);
-            // waves whose faces are all near-uniform skip the network
-            const bool smooth = hy.smoothVariation > 0.0f &&
-                                artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
-            bool troubled = false;
-            if (__ballot(valid && !smooth)) {
-                float x[artra::kFeat];
-                artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
-                const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
-                troubled = valid && !smooth && z > hy.threshold;
-            }
-            const unsigned long long mask = __ballot(troubled);
-            int slot = 0;
-            if (__lane_id() == 0 && mask) slot = atomicAdd(queued, __popcll(mask));
-            slot = __shfl(slot, 0);
-            const int pos = slot + __popcll(mask & ((1ull << __lane_id()) - 1ull));
-            if (troubled && pos < cap) queue[pos] = q;
-            if (valid) fidx[q] = troubled ? pos : -1;
+            const bool troubled = ClassifyFace(hy, nodeRP, n, f, valid, frags, bias);
+            if (valid) tflag[q] = troubled;
+            troubledFaces += __popcll(__ballot(troubled));
+            faces += __popcll(__ballot(valid));
+        }
+        // tallies per wavefront: faces on the linear flux, faces classified,
+        // faces on the full WENO5 flux
+        if (__lane_id() == 0 && write) {
+            atomicAdd(&tally[0], faces - troubledFaces);
+            atomicAdd(&tally[1], faces);
+            if (troubledFaces) atomicAdd(&tally[2], troubledFaces);
         }
         __syncthreads();
-        nq = *queued;
-        routed = nq <= cap;
-        // tallies of the faces as they are actually computed: linear (routed
-        // lines), WENO5 because the line overflowed, and all faces
-        if (tt == 0 && write) {
-            atomicAdd(&tally[0], (unsigned long long)(routed ? F - nq : 0));
-            atomicAdd(&tally[1], (unsigned long long)F);
-            atomicAdd(&tally[2], (unsigned long long)(routed ? 0 : F));
-        }
-    };
-    if (HYB) classifyLine(frags);
+    }
     for (int s = 0; s < la.stages; ++s) {
-        if (!HYB || !routed) {
-            // plain WENO5 on every face (with HYB: a line with too many
-            // troubled faces, as the plain sweep computes it)
-            for (int m = 0; m < M; ++m) {
-                const int q = tt + T * m;
-                if (q >= F) break;
-                const int f = la.lo - 1 + q;   // face between f and f + 1
-                double Fx[5];
-                if (FAST && NS == 6) {
-                    artra::weno_flux_line<S>(mdl, st, f, Fx);
-                } else {
-                    double w[NS][5];
+        for (int m = 0; m < M; ++m) {
+            const int q = tt + T * m;
+            if (q >= F) break;
+            const int f = la.lo - 1 + q;   // face between f and f + 1
+            double Fx[5];
+            if (HYB && !tflag[q]) {
+                LinearFaceFlux<S>(mdl, st, f, Fx);
+            } else if (FAST && NS == 6) {
+                artra::weno_flux_line<S>(mdl, st, f, Fx);
+            } else {
+                double w[NS][5];
 #pragma unroll
-                    for (int j = 0; j < NS; ++j)
+                for (int j = 0; j < NS; ++j)
 #pragma unroll
-                        for (int c = 0; c < 5; ++c) w[j][c] = st[(f - SL + j) * 5 + c];
-                    ConvFlux<S, NS, FAST>(mdl, w, Fx);
-                }
-#pragma unroll
-                for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
+                    for (int c = 0; c < 5; ++c) w[j][c] = st[(f - SL + j) * 5 + c];
+                ConvFlux<S, NS, FAST>(mdl, w, Fx);
             }
-        } else {
-            // one pass over the linear fluxes of the routed faces and the
-            // per-field WENO5 items of the queued ones
-            const int total = F + 5 * nq;
-            for (int e = tt; e < total; e += T) {
-                if (e < F) {
-                    const int q = e;
-                    if (fidx[q] >= 0) continue;
-                    const int f = la.lo - 1 + q;
-                    double Sg[5], Dl[5], Fx[5];
-                    {
-                        double w[6][5];
 #pragma unroll
-                        for (int j = 0; j < 6; ++j)
-#pragma unroll
-                            for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
-                        artra::linear_sum_diff(w, Sg, Dl);
-                    }
-                    artra::FaceState fs;
-                    if (!mdl.roe && !mdl.sw) {
-                        double nL[4], nR[4];
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            nL[c] = nodeV[f * 4 + c];
-                            nR[c] = nodeV[(f + 1) * 4 + c];
-                        }
-                        artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
-                        artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
-                    } else {
-                        double UL[5], UR[5];
-#pragma unroll
-                        for (int c = 0; c < 5; ++c) {
-                            UL[c] = st[f * 5 + c];
-                            UR[c] = st[(f + 1) * 5 + c];
-                        }
-                        artra::face_state<S>(mdl, UL, UR, fs);
-                        if (mdl.sw) {
-                            double Up[5], Um[5];
-#pragma unroll
-                            for (int c = 0; c < 5; ++c) {
-                                Up[c] = 0.5 * (Sg[c] + Dl[c]);
-                                Um[c] = 0.5 * (Sg[c] - Dl[c]);
-                            }
-                            artra::linear_flux_from<S>(fs, Up, Um, Fx);
-                        } else {
-                            artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
-                        }
-                    }
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
-                } else {
-                    const int it = e - F, qi = it / 5, k = it - 5 * qi;
-                    const int f = la.lo - 1 + queue[qi];
-                    double UL[5], UR[5];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        UL[c] = st[f * 5 + c];
-                        UR[c] = st[(f + 1) * 5 + c];
-                    }
-                    artra::FaceState fs;
-                    artra::face_state<S>(mdl, UL, UR, fs);
-                    artra::face_scales(fs);
-                    artra::weno_field_parts<S>(fs, st, f, k, mdl.sw != 0, hbuf[10 * qi + 2 * k],
-                                               hbuf[10 * qi + 2 * k + 1]);
-                }
-            }
-        }
-        if (HYB) {
-            // every line of the block reaches this barrier, routed or not
-            __syncthreads();
-            // project the queued faces' fields back to fluxes
-            for (int e = tt; routed && e < nq; e += T) {
-                const int q = queue[e];
-                const int f = la.lo - 1 + q;
-                double UL[5], UR[5], h[5], Fx[5];
-#pragma unroll
-                for (int c = 0; c < 5; ++c) {
-                    UL[c] = st[f * 5 + c];
-                    UR[c] = st[(f + 1) * 5 + c];
-                }
-                artra::fields_from_parts(hbuf + 10 * e, h);
-                artra::FaceState fs;
-                artra::face_state<S>(mdl, UL, UR, fs);
-                artra::project_back<S>(fs, h, Fx);
-#pragma unroll
-                for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
-            }
+            for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
         }
         __syncthreads();
         // node updates (fluid_dynamics.c:243-274); each thread reads and writes only its own nodes
         const double a = la.a[s], b = la.b[s];
-        const bool refresh = HYB && s + 1 < la.stages;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (!fl[k]) continue;
@@ -866,7 +790,6 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 const double Uo = (s == 0) ? Un : u0[q * 5 + c];
                 st[i * 5 + c] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
             }
-            if (refresh) LineNodeQuantities(mdl, st, n, i, nodeV, nodeRP, false);
         }
         __syncthreads();
         if (s + 1 < la.stages) {
@@ -882,7 +805,6 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
                 }
 #pragma unroll
                 for (int c = 0; c < 5; ++c) st[d * 5 + c] = v[c];
-                if (HYB) LineNodeQuantities(mdl, st, n, d, nodeV, nodeRP, false);
             }
             __syncthreads();
         }
@@ -899,7 +821,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
 
 // One line per block.
 template <int S, int NS, bool FAST, bool HYB, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : FAST ? 4 : 2))) void rk_line(LineArgs la, FluxModel mdl,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : FAST ? 3 : 2))) void rk_line(LineArgs la, FluxModel mdl,
                                                                                         LineHybrid hy)
 {
     if (la.dt && *la.dt == 0.0) return;             // a device-stepped march past its end time
@@ -924,15 +846,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : F
 
 // The learned hybrid's lean pass: one line per block, T >= interior nodes
 // (one node and at most two faces per thread).  The line is loaded and its
-// faces classified exactly as LineBody classifies them; a line with any
-// troubled face is not computed here but appended to defer[] (it is read
-// again by rk_line_list, the full hybrid kernel), every other line runs all
-// RK stages with the linear fifth-order flux only: no WENO5 code, no network
-// fragments past the classification, no face queue, LDS for the states, the
-// face fluxes, the fill rules and the FP32 features only.  Its faces are
-// bit-identical to the routed faces of LineBody (the same node quantities,
-// face state and closed-form flux), so a line's result does not depend on
-// the kernel that ran it.
+// faces classified exactly as LineBody classifies them (ClassifyFace); a
+// line with any troubled face is not computed here but appended to defer[]
+// (rk_line_list runs it again with the full hybrid body), every other line
+// runs all RK stages with the linear fifth-order flux only (LinearFaceFlux):
+// no WENO5 code and no network fragments past the classification, so it
+// holds fewer registers and runs more lines per compute unit.  Its faces
+// are those LineBody computes for the same line, bit for bit.
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rk_lean(LineArgs la, FluxModel mdl,
                                                                                       LineHybrid hy, int *defer,
@@ -977,15 +897,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
         const int q = tt + T * m;
         const bool valid = q < F;
         const int f = la.lo - 1 + (valid ? q : F - 1);
-        const bool smooth = hy.smoothVariation > 0.0f &&
-                            artra::near_uniform(nodeRP + f - 2, nodeRP + n + f - 2, hy.smoothVariation);
-        bool troubled = false;
-        if (__ballot(valid && !smooth)) {
-            float x[artra::kFeat];
-            artra::features_from(nodeRP + f - 2, nodeRP + n + f - 2, x);
-            const float z = artra::mlp_logit_wave(x, frags, bias, hy.b3);
-            troubled = valid && !smooth && z > hy.threshold;
-        }
+        const bool troubled = ClassifyFace(hy, nodeRP, n, f, valid, frags, bias);
         if (__ballot(troubled) && __lane_id() == 0) troubledAny = 1;
     }
     __syncthreads();
@@ -1009,43 +921,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
         for (int m = 0; m < M; ++m) {
             const int q = tt + T * m;
             if (q >= F) break;
-            const int f = la.lo - 1 + q;
-            double Sg[5], Dl[5], Fx[5];
-            {
-                double w[6][5];
-#pragma unroll
-                for (int j = 0; j < 6; ++j)
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
-                artra::linear_sum_diff(w, Sg, Dl);
-            }
-            double UL[5], UR[5];
-#pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                UL[c] = st[f * 5 + c];
-                UR[c] = st[(f + 1) * 5 + c];
-            }
-            artra::FaceState fs;
-            if (!mdl.roe && !mdl.sw) {
-                double nL[4], nR[4];
-                artra::node_velocity_enthalpy(mdl.gamma, UL, nL);
-                artra::node_velocity_enthalpy(mdl.gamma, UR, nR);
-                artra::face_state_mean<S>(mdl.gamma, nL, nR, fs);
-                artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
-            } else {
-                artra::face_state<S>(mdl, UL, UR, fs);
-                if (mdl.sw) {
-                    double Up[5], Um[5];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) {
-                        Up[c] = 0.5 * (Sg[c] + Dl[c]);
-                        Um[c] = 0.5 * (Sg[c] - Dl[c]);
-                    }
-                    artra::linear_flux_from<S>(fs, Up, Um, Fx);
-                } else {
-                    artra::linear_flux_llf_sd<S>(mdl.gamma, fs, Sg, Dl, Fx);
-                }
-            }
+            double Fx[5];
+            LinearFaceFlux<S>(mdl, st, la.lo - 1 + q, Fx);
 #pragma unroll
             for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
         }
@@ -1159,7 +1036,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         const BatchTile bt = tiles[w];
         const LineArgs &la = args[bt.c];
         // loaded per tile (L2 hits, issued ahead of the field loads) so that
-        // they hold no registers past the tile's classification
+        // they hold no registers past the tile's classification; an opaque
+        // memory effect keeps the loads from being hoisted out of the loop
+        if (HYB) asm volatile("" ::: "memory");
         artra::MlpWave frags;
         if (HYB) artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
         const int slot = threadIdx.x / bt.T, tt = threadIdx.x % bt.T;
@@ -1241,6 +1120,93 @@ __global__ __launch_bounds__(128) void ib_layer(double *U, long N, const GhostDe
     double out[5];
     Conservative(gamma, UoG, out);
     StoreNode(U, N, gh.node, out);
+}
+
+// ----------------------------------------------------------- surface force
+// Per layer-2 ghost node of a body: the pressure force p N, the wall shear
+// mu (V_rel - (V_rel . N) N) / |pG - pO| of a no-slip body in a viscous
+// flow, and their torque r x (Fp + Fv) (solid_dynamics.c:110-141), in the
+// reference's operation order; terms[t] = Fp[3], Fv[3], Tt[3], p.
+__global__ __launch_bounds__(256) void force_terms(const double *U, long N, const ArtraForceNode *nodes, int count,
+                                                   const ArtraForceBody *bodies, double gamma, double gasR,
+                                                   double refMu, double refT, double *terms)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const ArtraForceNode e = nodes[t];
+    const ArtraForceBody &b = bodies[e.body];
+    double Un[5], Uo[6], Fp[3], Fv[3], Fs[3], Tt[3];
+    LoadNode(U, N, e.node, Un);
+    Primitive(gamma, gasR, Un, Uo);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) Fp[s] = Uo[4] * e.N[s];
+    if (b.viscous) {
+        const double Ts = Uo[5] * refT;
+        const double mu = refMu * (1.458e-6 * pow(Ts, 1.5) / (Ts + 110.4));
+        double V[3];
+        V[0] = b.W[1] * e.r[2] - b.W[2] * e.r[1];
+        V[1] = b.W[2] * e.r[0] - b.W[0] * e.r[2];
+        V[2] = b.W[0] * e.r[1] - b.W[1] * e.r[0];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) V[s] = Uo[1 + s] - (b.V[s] + V[s]);
+        const double Vn = V[0] * e.N[0] + V[1] * e.N[1] + V[2] * e.N[2];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) Fv[s] = mu * (V[s] - Vn * e.N[s]) / e.dist;
+    } else {
+        Fv[0] = Fv[1] = Fv[2] = 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) Fs[s] = Fp[s] + Fv[s];
+    Tt[0] = e.r[1] * Fs[2] - e.r[2] * Fs[1];
+    Tt[1] = e.r[2] * Fs[0] - e.r[0] * Fs[2];
+    Tt[2] = e.r[0] * Fs[1] - e.r[1] * Fs[0];
+    double *o = terms + (size_t)t * 10;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        o[s] = Fp[s];
+        o[3 + s] = Fv[s];
+        o[6 + s] = Tt[s];
+    }
+    o[9] = Uo[4];
+}
+
+// Per body, the sums of its nodes' terms in table order (the reference's
+// loop order, so the sums are those of its serial loop) and the pressure
+// statistics: fvar = (p of the first node, sum (p - p0), sum (p - p0)^2).
+__global__ void force_sum(const ArtraForceBody *bodies, int nbody, const double *terms, double *sums)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbody) return;
+    const ArtraForceBody bd = bodies[b];
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double fvar[3] = {0.0, 0.0, 0.0};
+    for (int t = bd.first; t < bd.first + bd.count; ++t) {
+        const double *o = terms + (size_t)t * 10;
+        if (t == bd.first) fvar[0] = o[9];
+        fvar[1] = fvar[1] + o[9] - fvar[0];
+        fvar[2] = fvar[2] + (o[9] - fvar[0]) * (o[9] - fvar[0]);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) acc[q] = acc[q] + o[q];
+    }
+    double *out = sums + (size_t)b * 12;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[q] = acc[q];
+    out[9] = fvar[0];
+    out[10] = fvar[1];
+    out[11] = fvar[2];
+}
+
+// Primitive variables of level TO at a list of nodes (the probes).
+__global__ __launch_bounds__(256) void gather_primitive(const double *U, long N, const int *nodes, int count,
+                                                        double gamma, double gasR, double *prim)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    double Un[5], Uo[6];
+    LoadNode(U, N, nodes[t], Un);
+    Primitive(gamma, gasR, Un, Uo);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) prim[(size_t)t * 6 + c] = Uo[c];
 }
 
 // ------------------------------------------------------------- time step
@@ -1482,6 +1448,15 @@ struct ArtraDevice {
     uint8_t *fluid = nullptr;
     unsigned long long *vmax = nullptr;
     double *marchBuf = nullptr;     // device march: speeds, clock, step, step record (kMarchChunk)
+    // surface force and probes
+    ArtraForceNode *forceNodes = nullptr;
+    ArtraForceBody *forceBodies = nullptr;
+    double *forceTerms = nullptr;   // [forceCount][10]
+    double *forceSums = nullptr;    // [forceBodyCount][12]
+    int forceCount = 0, forceBodyCount = 0;
+    int *probeNodes = nullptr;      // gather buffers, grown as needed
+    double *probePrim = nullptr;
+    int probeCap = 0;
     const double *stepDev = nullptr; // when set, the operators read their step from here (device march)
     int ordinal = 0;                // the GPU this solver lives on
     hipStream_t stream = nullptr;
@@ -2184,6 +2159,12 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->fluid);
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
+    hipFree(dev->forceNodes);
+    hipFree(dev->forceBodies);
+    hipFree(dev->forceTerms);
+    hipFree(dev->forceSums);
+    hipFree(dev->probeNodes);
+    hipFree(dev->probePrim);
     hipFree(dev->ghosts);
     hipFree(dev->snode);
     hipFree(dev->sw);
@@ -2352,6 +2333,76 @@ int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
     HIP_TRY(hipStreamSynchronize(dev->stream));
     for (int s = 0; s < 3; ++s) memcpy(vmax + s, host + s, sizeof(double));
     return Harvest(dev);
+}
+
+int artra_dev_set_force_table(ArtraDevice *dev, const ArtraForceNode *nodes, int count, const ArtraForceBody *bodies,
+                              int nbody)
+{
+    hipFree(dev->forceNodes);
+    hipFree(dev->forceBodies);
+    hipFree(dev->forceTerms);
+    hipFree(dev->forceSums);
+    dev->forceNodes = nullptr;
+    dev->forceBodies = nullptr;
+    dev->forceTerms = nullptr;
+    dev->forceSums = nullptr;
+    dev->forceCount = dev->forceBodyCount = 0;
+    if (nbody <= 0) return 0;
+    HIP_TRY(hipMalloc(&dev->forceBodies, nbody * sizeof(ArtraForceBody)));
+    HIP_TRY(hipMalloc(&dev->forceSums, (size_t)nbody * 12 * sizeof(double)));
+    HIP_TRY(hipMemcpy(dev->forceBodies, bodies, nbody * sizeof(ArtraForceBody), hipMemcpyHostToDevice));
+    if (count > 0) {
+        HIP_TRY(hipMalloc(&dev->forceNodes, count * sizeof(ArtraForceNode)));
+        HIP_TRY(hipMalloc(&dev->forceTerms, (size_t)count * 10 * sizeof(double)));
+        HIP_TRY(hipMemcpy(dev->forceNodes, nodes, count * sizeof(ArtraForceNode), hipMemcpyHostToDevice));
+    }
+    dev->forceCount = count;
+    dev->forceBodyCount = nbody;
+    return 0;
+}
+
+int artra_dev_surface_force(ArtraDevice *dev, double *sums)
+{
+    if (dev->forceBodyCount <= 0) return 0;
+    if (dev->forceCount > 0) {
+        hipLaunchKernelGGL(force_terms, dim3((dev->forceCount + 255) / 256), dim3(256), 0, dev->stream,
+                           Level(dev, 0), dev->g.N, dev->forceNodes, dev->forceCount, dev->forceBodies,
+                           dev->par.gamma, dev->par.gasR, dev->par.refMu, dev->par.refT, dev->forceTerms);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(force_sum, dim3((dev->forceBodyCount + 63) / 64), dim3(64), 0, dev->stream, dev->forceBodies,
+                       dev->forceBodyCount, dev->forceTerms, dev->forceSums);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(sums, dev->forceSums, (size_t)dev->forceBodyCount * 12 * sizeof(double),
+                           hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    return 0;
+}
+
+int artra_dev_gather_primitive(ArtraDevice *dev, const int *nodes, int count, double *prim)
+{
+    if (count <= 0) return 0;
+    if (count > dev->probeCap) {
+        hipFree(dev->probeNodes);
+        hipFree(dev->probePrim);
+        dev->probeNodes = nullptr;
+        dev->probePrim = nullptr;
+        dev->probeCap = 0;
+        HIP_TRY(hipMalloc(&dev->probeNodes, (size_t)count * sizeof(int)));
+        HIP_TRY(hipMalloc(&dev->probePrim, (size_t)count * 6 * sizeof(double)));
+        dev->probeCap = count;
+    }
+    for (int t = 0; t < count; ++t) {
+        if (nodes[t] < 0 || nodes[t] >= dev->g.N) return Fail("probe node %d outside the grid", nodes[t]);
+    }
+    HIP_TRY(hipMemcpyAsync(dev->probeNodes, nodes, (size_t)count * sizeof(int), hipMemcpyHostToDevice, dev->stream));
+    hipLaunchKernelGGL(gather_primitive, dim3((count + 255) / 256), dim3(256), 0, dev->stream, Level(dev, 0), dev->g.N,
+                       dev->probeNodes, count, dev->par.gamma, dev->par.gasR, dev->probePrim);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(prim, dev->probePrim, (size_t)count * 6 * sizeof(double), hipMemcpyDeviceToHost,
+                           dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    return 0;
 }
 
 int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[3], double cfl, int steps,

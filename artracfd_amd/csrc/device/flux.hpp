@@ -413,32 +413,55 @@ __host__ __device__ __forceinline__ void convective_flux(const FluxModel &mdl, c
             ln[r] = 0.5 * (lam[r] - star);
         }
     }
-    // characteristic variables of every stencil node
+    // characteristic variables: per stencil node K = q U0 - vel.mom + U4 and
+    // M = un U0 - U_{1+S}; the fields are formed one at a time from them
+    // (acoustic fields 0 and 4, then the entropy and shear fields), so that
+    // only one field's stencil values are live at once
     const double m2b = -2.0 * b;
     double sh[3];
 #pragma unroll
     for (int t = 0; t < 3; ++t) sh[t] = m2b * vel[t];
-    double W[5][NS];
+    double K[NS], M[NS];
 #pragma unroll
     for (int m = 0; m < NS; ++m) {
         const double *U = st[m];
-        const double K = fma(q, U[0], U[4]) - fma(u, U[1], fma(v, U[2], w * U[3]));
-        const double M = fma(un, U[0], -U[1 + S]);
-        const double bK = b * K, dM = d * M;
-        W[0][m] = bK + dM;
-        W[4][m] = bK - dM;
-#pragma unroll
-        for (int t = 0; t < 3; ++t) W[1 + t][m] = (t == S) ? fma(m2b, K, U[0]) : fma(sh[t], K, U[1 + t]);
+        K[m] = fma(q, U[0], U[4]) - fma(u, U[1], fma(v, U[2], w * U[3]));
+        M[m] = fma(un, U[0], -U[1 + S]);
     }
     double h[5];
     if (NS == 6) {
         double num[5], den[5];
+        {
+            double wa[6], wb[6];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) weno5_field_parts(W[r], lp[r], ln[r], num[r], den[r]);
+            for (int m = 0; m < 6; ++m) {
+                const double bK = b * K[m], dM = d * M[m];
+                wa[m] = bK + dM;
+                wb[m] = bK - dM;
+            }
+            weno5_field_parts(wa, lp[0], ln[0], num[0], den[0]);
+            weno5_field_parts(wb, lp[4], ln[4], num[4], den[4]);
+        }
         divide_pair(num[0], den[0], num[4], den[4], h[0], h[4]);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            double wf[6];
+#pragma unroll
+            for (int m = 0; m < 6; ++m) wf[m] = (t == S) ? fma(m2b, K[m], st[m][0]) : fma(sh[t], K[m], st[m][1 + t]);
+            weno5_field_parts(wf, lp[1 + t], ln[1 + t], num[1 + t], den[1 + t]);
+        }
         divide_pair(num[1], den[1], num[2], den[2], h[1], h[2]);
         h[3] = num[3] / den[3];
     } else {
+        double W[5][NS];
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+            const double bK = b * K[m], dM = d * M[m];
+            W[0][m] = bK + dM;
+            W[4][m] = bK - dM;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) W[1 + t][m] = (t == S) ? fma(m2b, K[m], st[m][0]) : fma(sh[t], K[m], st[m][1 + t]);
+        }
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
             const double hp = weno3(lp[r] * W[r][0], lp[r] * W[r][1], lp[r] * W[r][2]);

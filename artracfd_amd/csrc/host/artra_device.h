@@ -40,6 +40,30 @@ typedef struct {
     double varBC[NBC][VARBC];
 } ArtraDevParams;
 
+/* One layer-2 ghost node of a body in the surface-force integration
+ * (solid_dynamics.c:96-148): its padded-grid index, the body's slot in the
+ * body table, the normal at its boundary point, the boundary point relative
+ * to the body centre and its distance from the ghost node. */
+typedef struct {
+    int node;
+    int body;
+    double N[DIMS];
+    double r[DIMS];
+    double dist;
+} ArtraForceNode;
+
+/* A body of the surface-force integration: its nodes [first, first+count)
+ * in the reference's (k, j, i) order, whether the wall shear counts
+ * (viscous flow and no-slip body), its velocity and angular velocity. */
+typedef struct {
+    int first;
+    int count;
+    int viscous;
+    int pad;
+    double V[DIMS];
+    double W[DIMS];
+} ArtraForceBody;
+
 /* kernel timing categories */
 enum { ARTRA_T_SWEEP = 0, ARTRA_T_BC = 1, ARTRA_T_IB = 2, ARTRA_T_CFL = 3, ARTRA_T_COUNT = 4 };
 
@@ -82,6 +106,16 @@ int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[
 int artra_dev_march_noalloc(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
                             double *dts, int *advanced);
 int artra_dev_sync(ArtraDevice *dev);
+/* surface force: the table once, then per call the per-body sums of the
+ * pressure force, wall shear and torque over its nodes in table order and
+ * the running pressure statistics of solid_dynamics.c:120-123, sums[b][12]
+ * = Fp[3], Fv[3], Tt[3], fvar[3] */
+int artra_dev_set_force_table(ArtraDevice *dev, const ArtraForceNode *nodes, int count, const ArtraForceBody *bodies,
+                              int nbody);
+int artra_dev_surface_force(ArtraDevice *dev, double *sums);
+/* primitive variables (rho, u, v, w, p, T) of level TO at count nodes:
+ * prim[count][6] (the probes) */
+int artra_dev_gather_primitive(ArtraDevice *dev, const int *nodes, int count, double *prim);
 /* kernel timing: when on, every launch is bracketed by events on the
  * solver's stream; totals are in milliseconds. */
 int artra_dev_set_timing(ArtraDevice *dev, int on);

@@ -16,6 +16,7 @@
 #include "artra_device.h"
 #include "artra_hip.h"
 #include "artra_host.h"
+#include "solver.h"
 
 /* The drop-in symbols take the caller's structures as the reference lays
  * them out (commons.h:128-268, x86-64 with gcc): sizes and a few offsets as
@@ -33,15 +34,6 @@ _Static_assert(sizeof(Time) == 104 && offsetof(Time, end) == 64, "Time layout di
 _Static_assert(sizeof(Model) == 152 && offsetof(Model, gamma) == 64 && offsetof(Model, g) == 120,
                "Model layout differs from commons.h:245-268");
 
-struct ArtraHipSolver {
-    Space *space;
-    const Model *model;
-    ArtraDevice *dev;
-    ArtraGhostTable ghosts;
-    long N;
-    double *staging;          /* [5][N] host buffer for AoS <-> SoA */
-    Real rcInt;               /* time since the last output instant (solve.c:64) */
-};
 
 struct ArtraCase {
     Time time;
@@ -52,6 +44,11 @@ struct ArtraCase {
 static int DevError(void)
 {
     return artra_set_error(1, "%s", artra_dev_error());
+}
+
+int artra_dev_failed(void)
+{
+    return DevError();
 }
 
 const char *artra_hip_error(void)
@@ -187,6 +184,10 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         artra_hip_detach(s);
         return 1;
     }
+    if (artra_build_force_table(s)) {
+        artra_hip_detach(s);
+        return 1;
+    }
     const char *exact = getenv("ARTRA_EXACT");
     if (exact && exact[0] == '1') artra_dev_set_exact(s->dev, 1);
     *out = s;
@@ -199,6 +200,7 @@ int artra_hip_detach(ArtraHipSolver *s)
     artra_dev_destroy(s->dev);
     artra_free_ghost_table(&s->ghosts);
     free(s->staging);
+    free(s->force);
     free(s);
     return 0;
 }

@@ -1,0 +1,39 @@
+/*
+ * solver.h - the resident solver's host state, shared by the C-ABI entry
+ * points (capi.c) and the output side (output.c: surface force, probes,
+ * field files, restart, the solver loop with its output cadence).
+ */
+#ifndef ARTRA_SOLVER_H_
+#define ARTRA_SOLVER_H_
+
+#include "artra_device.h"
+#include "artra_host.h"
+
+/* A body whose surface force the solver integrates (state <= 0,
+ * solid_dynamics.c:86-88): its index in Geometry, the interfacial layer-2
+ * node count that calibrates the sum, and its layer-2 ghost nodes. */
+typedef struct {
+    int body;
+    int lidN;
+    int gstN;
+} ArtraForceBodyInfo;
+
+struct ArtraHipSolver {
+    Space *space;
+    const Model *model;
+    ArtraDevice *dev;
+    ArtraGhostTable ghosts;
+    long N;
+    double *staging;          /* [5][N] host buffer for AoS <-> SoA */
+    Real rcInt;               /* time since the last output instant (solve.c:64) */
+    int forceBodies;          /* bodies in force[] */
+    ArtraForceBodyInfo *force;
+};
+
+/* Build the surface-force table of the solver's bodies and hand it to the
+ * device (at attach). */
+int artra_build_force_table(ArtraHipSolver *s);
+/* Error from the device side, as an artra_set_error code. */
+int artra_dev_failed(void);
+
+#endif /* ARTRA_SOLVER_H_ */

@@ -56,6 +56,8 @@ def library():
         "artra_hip_evolve": (ctypes.c_int, [vp, ctypes.c_double]),
         "artra_hip_march": (ctypes.c_int, [vp, vp, ctypes.c_int, ip, dp]),
         "artra_hip_march_noalloc": (ctypes.c_int, [vp, vp, ctypes.c_int, ip, dp]),
+        "artra_hip_surface_force": (ctypes.c_int, [vp]),
+        "artra_hip_write_probe_data": (ctypes.c_int, [vp, vp, ctypes.c_int]),
         "artra_hip_download": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_upload": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_read_field": (ctypes.c_int, [vp, ctypes.c_int, dp]),

@@ -172,7 +172,7 @@ def batched_both(with_indicator=True):
     if with_indicator:
         hyb = batched(use_indicator=True)
         plain["hybrid"] = {k: hyb[k] for k in ("value", "serial_value", "wall_ms", "linear_face_fraction",
-                                                "overflow_face_fraction", "tuned", "serial_ms_per_case")}
+                                                "weno5_face_fraction", "tuned", "serial_ms_per_case")}
         plain["hybrid"]["vs_plain"] = round(hyb["value"] / plain["value"], 3)
     return plain
 
@@ -239,7 +239,7 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
                                    "p90": round(float(np.percentile(pc, 90)), 3),
                                    "p99": round(float(np.percentile(pc, 99)), 3), "sum": round(float(pc.sum()), 3)},
             "linear_face_fraction": round(lin / max(seen, 1), 4) if params is not None else None,
-            "overflow_face_fraction": round(over / max(seen, 1), 4) if params is not None else None,
+            "weno5_face_fraction": round(over / max(seen, 1), 4) if params is not None else None,
             "tuned": {"threads_per_line": {str(sd): sorted(set(t for t, d in zip(threads, drawn) if d["side"] == sd))
                                            for sd in sides}, "grid_blocks": grid},
             "setup_seconds": round(setup_s, 2), "tune_seconds": round(tune_s, 2),
@@ -349,7 +349,7 @@ def main():
             "value": round(hvalue, 3), "unit": "Mcell-stage updates/s", "vs_plain": round(hvalue / value, 3),
             "ms_per_step": round(helapsed * 1e3 / max(hdone, 1), 4),
             "linear_face_fraction": round(lin / max(seen, 1), 4),
-            "overflow_face_fraction": round(over / max(seen, 1), 4),
+            "weno5_face_fraction": round(over / max(seen, 1), 4),
             "rel_l1_rho_vs_plain_at_end": float(np.abs(rho_h - rho_p).sum() / np.abs(rho_p).sum()),
             "end_time_steps": {"plain": steps_p, "hybrid": steps_h},
             "end_time_seconds": {"plain": round(el_p, 3), "hybrid": round(el_h, 3)},

@@ -97,6 +97,21 @@ int artra_hip_march(ArtraHipSolver *solver, Time *time, int max_steps, int *step
 #define ARTRA_NOALLOC_MAX_STEPS 1024
 int artra_hip_march_noalloc(ArtraHipSolver *solver, Time *time, int max_steps, int *steps_done, Real *dts);
 
+/* IntegrateSurfaceForce (solid_dynamics.c:50-171) on the device-resident
+ * level TO: for every body with state <= 0, the pressure force, the wall
+ * shear of a no-slip body in a viscous flow and their torque about the
+ * body centre, into space->geo.poly[n].Fp, Fv, Tt.  A reduction over the
+ * bodies' layer-2 ghost nodes on the device (in the reference's node
+ * order); only the per-body sums cross to the host. */
+int artra_hip_surface_force(ArtraHipSolver *solver);
+
+/* WriteData for one probe kind (data_probe.c:22-197; kind PROPT point,
+ * PROLN line, PROCV body-conformal curve, PROFC surface force): the same
+ * files, in the current directory, as the reference writes, from device
+ * gathers of the probed nodes only (the field stays on the device).  The
+ * surface-force file holds the values of the last artra_hip_surface_force. */
+int artra_hip_write_probe_data(ArtraHipSolver *solver, const Time *time, int kind);
+
 /* Copy a time level between the device and the node array of space. */
 int artra_hip_download(ArtraHipSolver *solver, int tn);
 int artra_hip_upload(ArtraHipSolver *solver, int tn);
@@ -235,8 +250,8 @@ int artra_hip_set_indicator(ArtraHipSolver *solver, const float *params, float l
 int artra_hip_set_indicator_options(ArtraHipSolver *solver, int every_stage, float smooth_variation);
 /* Face tallies per classification (once per directional operator):
  * counts[0] faces computed with the linear flux, counts[1] faces
- * classified, counts[2] faces of lines whose troubled faces overflowed the
- * line's queue and which therefore ran plain WENO5 on every face. */
+ * classified, counts[2] faces computed with the full WENO5 flux
+ * (counts[0] + counts[2] = counts[1]). */
 int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[3], int reset);
 
 /* Train in place on the device (Adam, weighted logistic loss of

@@ -186,24 +186,23 @@ def test_all_troubled_routing_is_plain_weno5(model, tmp_path):
     name = _flux_model_variant(tmp_path, model)
     print("hybrid all-WENO shockvortex64", model, flush=True)
     U0, d0, _ = _evolve(name, 4)
-    U1, d1, (lin, seen, over) = _evolve(name, 4, _params(3), threshold=-1e30, gate=0.0)
+    U1, d1, (lin, seen, weno) = _evolve(name, 4, _params(3), threshold=-1e30, gate=0.0)
     assert lin == 0 and seen > 0
-    assert over == seen                 # 65 faces per line, more than the 16-face queue: every line overflows
+    assert weno == seen
     np.testing.assert_array_equal(d1, d0)
     assert np.array_equal(U1, U0)
 
 
-def test_queued_faces_are_plain_weno5(tmp_path):
-    """Lines of 12 interior nodes have 13 faces, within the 16-face queue,
-    so with every face troubled each face goes through the queue's per-field
-    WENO5 items (no line overflows to the plain flux): a 2-D Riemann problem
-    must then match the plain sweeps bit for bit."""
+def test_all_troubled_short_lines_are_plain_weno5(tmp_path):
+    """Lines of 12 interior nodes (13 faces, one partly filled wavefront per
+    line) with every face troubled: a 2-D Riemann problem matches the plain
+    sweeps bit for bit."""
     from artracfd_amd import indicator
     d = indicator.riemann_case(str(tmp_path / "riemann12"), np.random.default_rng(7), n=12, steps=30)
-    print("hybrid all-WENO through the queue, riemann 12x12", flush=True)
+    print("hybrid all-WENO on 13-face lines, riemann 12x12", flush=True)
     U0, d0, _ = _evolve(d, 30)
-    U1, d1, (lin, seen, over) = _evolve(d, 30, _params(3), threshold=-1e30, gate=0.0)
-    assert lin == 0 and seen > 0 and over == 0
+    U1, d1, (lin, seen, weno) = _evolve(d, 30, _params(3), threshold=-1e30, gate=0.0)
+    assert lin == 0 and seen > 0 and weno == seen
     np.testing.assert_array_equal(d1, d0)
     assert np.array_equal(U1, U0)
 
@@ -219,8 +218,8 @@ def test_all_linear_on_smooth_data_is_close_to_weno5(tmp_path):
     Ui = init.interior(init.field())
     init.close()
     Uw, dw, _ = _evolve(d, 20)
-    Ul, dl, (lin, seen, over) = _evolve(d, 20, _params(3), threshold=1e30)
-    assert lin == seen and seen > 0 and over == 0
+    Ul, dl, (lin, seen, weno) = _evolve(d, 20, _params(3), threshold=1e30)
+    assert lin == seen and seen > 0 and weno == 0
     np.testing.assert_allclose(dl, dw, rtol=1e-8)    # the speeds differ at the level the fluxes do
     change = np.abs(Uw[0] - Ui[0]).max()
     assert change > 1e-5
@@ -329,8 +328,8 @@ def test_all_linear_routing_equals_weno5_on_linear_data(tmp_path, numerical):
     U0 = c.interior(c.field())
     c.close()
     Uw, dw, _ = _evolve(d, 1)
-    Ul, dl, (lin, seen, over) = _evolve(d, 1, _params(3), threshold=1e30, gate=0.0)
-    assert lin == seen and seen > 0 and over == 0
+    Ul, dl, (lin, seen, weno) = _evolve(d, 1, _params(3), threshold=1e30, gate=0.0)
+    assert lin == seen and seen > 0 and weno == 0
     mid = slice(24, -24)
     change = np.abs(Uw[0, ..., mid] - U0[0, ..., mid]).max()
     assert change > 1e-4, change
@@ -354,7 +353,7 @@ def test_trained_indicator_at_256_to_end_time():
     rp, dp, _ = _evolve_density("shockvortex256")
     rh, dh, (lin, seen, over) = _evolve_density("shockvortex256", params, thr)
     l1 = float(np.abs(rh - rp).sum() / np.abs(rp).sum())
-    print("steps", len(dp), len(dh), "rel L1 rho", l1, "linear", lin / seen, "overflowed", over / seen, flush=True)
+    print("steps", len(dp), len(dh), "rel L1 rho", l1, "linear", lin / seen, "weno5", over / seen, flush=True)
     assert len(dh) == len(dp) and len(dp) > 100
     assert l1 <= 1e-3
     assert lin / seen >= 0.9
