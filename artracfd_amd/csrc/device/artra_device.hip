@@ -63,9 +63,10 @@ constexpr int kListBlocks = 768;    // persistent blocks of rk_line_list (3 per 
 constexpr int kMarchChunk = ARTRA_DEV_MARCH_CHUNK;   // steps per device-stepped march call (the record's length)
 
 // bytes of a device march's scratch with a record of cap steps (see MarchScratch)
+constexpr int kMarchClock = ARTRA_DEV_MARCH_CLOCK;     // doubles of a device march's clock
 constexpr size_t MarchScratchBytes(int cap)
 {
-    return (size_t)(4 + 6 + 2 + cap) * sizeof(double);
+    return (size_t)(4 + kMarchClock + 2 + cap) * sizeof(double);
 }
 
 // Scratch of the allocation-free march (artra_dev_march_noalloc): speed
@@ -73,7 +74,7 @@ constexpr size_t MarchScratchBytes(int cap)
 // as MarchScratch.  Every GPU the library runs on holds its own copy until
 // the process ends; calls that overlap on one GPU share it and race.
 constexpr int kNoAllocSteps = ARTRA_DEV_NOALLOC_STEPS;
-static __device__ __align__(256) double scratch[4 + 6 + 2 + kNoAllocSteps];
+static __device__ __align__(256) double scratch[4 + kMarchClock + 2 + kNoAllocSteps];
 static_assert(sizeof scratch == MarchScratchBytes(kNoAllocSteps), "scratch layout");
 
 constexpr int kMaxDevices = 64;
@@ -533,6 +534,41 @@ __global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__res
 // With HYB the learned indicator routes each face: smooth faces take the
 // linear fifth-order upwind flux, troubled ones the full WENO5 flux.
 constexpr int kLineRules = 12;
+constexpr int kRefillPerNode = 4;   // ghosts a line's node refills between stages, at most
+
+// The ghosts a node feeds, kRefillPerNode line positions packed two to a
+// register (16 bits each, 0xffff for none; line positions are below 2^16).
+struct Feeds {
+    unsigned w[kRefillPerNode / 2];
+};
+
+// (every slot is written through a compile-time index, so the set stays in
+// registers)
+__device__ __forceinline__ Feeds FeedsOf(const int *rules, int nrules, int node)
+{
+    static_assert(kRefillPerNode == 4, "two packed registers");
+    unsigned w0 = 0xffffffffu, w1 = 0xffffffffu;
+    int nf = 0;
+    for (int e = 0; e < nrules; ++e) {
+        if (rules[kLineRules + e] != node || nf >= kRefillPerNode) continue;
+        const unsigned pos = (unsigned)rules[e];
+        const unsigned sh = 16u * (nf & 1);
+        const unsigned mask = ~(0xffffu << sh);
+        if (nf < 2) w0 = (w0 & mask) | (pos << sh);
+        else w1 = (w1 & mask) | (pos << sh);
+        ++nf;
+    }
+    Feeds f;
+    f.w[0] = w0;
+    f.w[1] = w1;
+    return f;
+}
+
+__device__ __forceinline__ int FeedAt(const Feeds &f, int e)
+{
+    const unsigned v = (f.w[e >> 1] >> (16u * (e & 1))) & 0xffffu;
+    return (v == 0xffffu) ? -1 : (int)v;
+}
 // The face tallies of the indicator are spread over slots on separate
 // 128-byte lines (block b adds into slot b mod kCountSlots): one counter
 // pair taking an atomic from every block serialises tens of thousands of
@@ -613,18 +649,20 @@ __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool hyb)
 // and total enthalpies, or the model's average) and the closed-form local
 // Lax-Friedrichs flux (the characteristic form for Steger-Warming).  The
 // one definition every hybrid kernel uses, so that a face's flux does not
-// depend on the kernel that computed it.
-template <int S>
+// depend on the kernel that computed it.  MEAN compiles only the default
+// model (arithmetic mean, local Lax-Friedrichs), whose registers fit more
+// wavefronts; otherwise the model is read at run time.
+template <int S, bool MEAN = false>
 __device__ __forceinline__ void LinearFaceFlux(const FluxModel &mdl, const double *st, int f, double Fx[5])
 {
     double Sg[5], Dl[5];
-    {
-        double w[6][5];
 #pragma unroll
-        for (int j = 0; j < 6; ++j)
+    for (int c = 0; c < 5; ++c) {
+        // one component's six stencil values at a time (fewer live registers)
+        double w[6];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) w[j][c] = st[(f - 2 + j) * 5 + c];
-        artra::linear_sum_diff(w, Sg, Dl);
+        for (int j = 0; j < 6; ++j) w[j] = st[(f - 2 + j) * 5 + c];
+        artra::linear_sum_diff_1(w, Sg[c], Dl[c]);
     }
     double UL[5], UR[5];
 #pragma unroll
@@ -633,7 +671,7 @@ __device__ __forceinline__ void LinearFaceFlux(const FluxModel &mdl, const doubl
         UR[c] = st[(f + 1) * 5 + c];
     }
     artra::FaceState fs;
-    if (!mdl.roe && !mdl.sw) {
+    if (MEAN || (!mdl.roe && !mdl.sw)) {
         double nL[4], nR[4];
         artra::node_velocity_enthalpy(mdl.gamma, UL, nL);
         artra::node_velocity_enthalpy(mdl.gamma, UR, nR);
@@ -729,14 +767,26 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         fl[k] = own[k] && la.fluid[base + (long)i * la.stride];
     }
     __syncthreads();
+    // between-stage refill duties: the ghosts this thread's nodes feed, and
+    // (threads tt < nrules) one constant-state rule
+    Feeds feed[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) feed[k] = FeedsOf(rules, la.nrules, la.lo + tt + T * k);
+    const int constRule = (tt < la.nrules && rules[kLineRules + tt] < 0) ? tt : -1;
     const int M = (F + T - 1) / T;
     const double r = Uniform(la.dt ? *la.dt * la.dd : la.r);
     if (la.dt && *la.dt == 0.0) write = false;      // a batched case past its end time
+    // With HYB the faces are dealt to the wavefronts rotated by a whole
+    // wavefront per line: the troubled faces of a shock crossing every line
+    // at the same place then fall on a different wavefront (and SIMD) from
+    // line to line instead of loading one SIMD with all of them.
+    const int QT = M * T;
+    const int rot = HYB ? (int)(line % (QT / 64)) * 64 : 0;
     if (HYB) {
         // classification (every thread runs the same trip count)
         unsigned long long troubledFaces = 0, faces = 0;
         for (int m = 0; m < M; ++m) {
-            const int q = tt + T * m;
+            const int q = (tt + T * m + rot) % QT;
             const bool valid = q < F;
             const int f = la.lo - 1 + (valid ? q : F - 1);
             const bool troubled = ClassifyFace(hy, nodeRP, n, f, valid, frags, bias);
@@ -755,8 +805,11 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     }
     for (int s = 0; s < la.stages; ++s) {
         for (int m = 0; m < M; ++m) {
-            const int q = tt + T * m;
-            if (q >= F) break;
+            const int q = HYB ? (tt + T * m + rot) % QT : tt + T * m;
+            if (q >= F) {
+                if (HYB) continue;
+                break;
+            }
             const int f = la.lo - 1 + q;   // face between f and f + 1
             double Fx[5];
             if (HYB && !tflag[q]) {
@@ -775,39 +828,41 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
         }
         __syncthreads();
-        // node updates (fluid_dynamics.c:243-274); each thread reads and writes only its own nodes
+        // node updates (fluid_dynamics.c:243-274); each thread reads and
+        // writes only its own nodes and, before another stage, refills the
+        // ghosts they feed (the line's between-stage fill)
         const double a = la.a[s], b = la.b[s];
+        const bool refill = s + 1 < la.stages;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (!fl[k]) continue;
             const int i = la.lo + tt + T * k;
             const int q = i - la.lo;
+            double v[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
                 const double FL = fb[q * 5 + c], FR = fb[(q + 1) * 5 + c];
                 const double phi = FAST ? FL - FR : ((FL - FR) + 0.0) - 0.0;
                 const double Un = st[i * 5 + c];
                 const double Uo = (s == 0) ? Un : u0[q * 5 + c];
-                st[i * 5 + c] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
+                v[c] = FAST ? fma(a, Uo, b * fma(r, phi, Un)) : a * Uo + b * (Un + r * phi);
+                st[i * 5 + c] = v[c];
             }
-        }
-        __syncthreads();
-        if (s + 1 < la.stages) {
-            // refill the line's ghost ends from the stage's interior
-            for (int e = tt; e < la.nrules; e += T) {
-                const int d = rules[e], q = rules[kLineRules + e];
-                double v[5];
-                if (q >= 0) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) v[c] = st[q * 5 + c];
-                } else {
-                    Conservative(la.gamma, la.given[-1 - q], v);
-                }
+            for (int e = 0; e < kRefillPerNode; ++e) {
+                const int d = FeedAt(feed[k], e);
+                if (!refill || d < 0) continue;
 #pragma unroll
                 for (int c = 0; c < 5; ++c) st[d * 5 + c] = v[c];
             }
-            __syncthreads();
         }
+        if (refill && constRule >= 0) {
+            double v[5];
+            Conservative(la.gamma, la.given[-1 - rules[kLineRules + constRule]], v);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = v[c];
+        }
+        __syncthreads();
     }
     if (!write) return;
 #pragma unroll
@@ -853,11 +908,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : F
 // no WENO5 code and no network fragments past the classification, so it
 // holds fewer registers and runs more lines per compute unit.  Its faces
 // are those LineBody computes for the same line, bit for bit.
-template <int S>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void rk_lean(LineArgs la, FluxModel mdl,
-                                                                                      LineHybrid hy, int *defer,
-                                                                                      int *ndefer,
-                                                                                      unsigned long long *deferTotal)
+template <int S, bool MEAN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 4))) void rk_lean(
+    LineArgs la, FluxModel mdl, LineHybrid hy, int *defer, int *ndefer, unsigned long long *deferTotal)
 {
     if (la.dt && *la.dt == 0.0) return;             // a device-stepped march past its end time
     extern __shared__ double lds[];
@@ -917,44 +970,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void r
 #pragma unroll
     for (int c = 0; c < 5; ++c) U0[c] = st[(own ? i : la.lo) * 5 + c];
     const double r = Uniform(la.dt ? *la.dt * la.dd : la.r);
+    // between-stage refill duties (as in LineBody)
+    const Feeds feed = FeedsOf(rules, la.nrules, i);
+    const int constRule = (tt < la.nrules && rules[kLineRules + tt] < 0) ? tt : -1;
     for (int s = 0; s < la.stages; ++s) {
         for (int m = 0; m < M; ++m) {
             const int q = tt + T * m;
             if (q >= F) break;
             double Fx[5];
-            LinearFaceFlux<S>(mdl, st, la.lo - 1 + q, Fx);
+            LinearFaceFlux<S, MEAN>(mdl, st, la.lo - 1 + q, Fx);
 #pragma unroll
             for (int c = 0; c < 5; ++c) fb[q * 5 + c] = Fx[c];
         }
         __syncthreads();
-        // node update (fluid_dynamics.c:243-274)
+        // node update (fluid_dynamics.c:243-274) and, before another
+        // stage, the refill of the ghosts this node feeds
+        const bool refill = s + 1 < la.stages;
         if (fl) {
             const double a = la.a[s], b = la.b[s];
             const int q = i - la.lo;
+            double v[5];
 #pragma unroll
             for (int c = 0; c < 5; ++c) {
                 const double phi = fb[q * 5 + c] - fb[(q + 1) * 5 + c];
                 const double Un = st[i * 5 + c];
                 const double Uo = (s == 0) ? Un : U0[c];
-                st[i * 5 + c] = fma(a, Uo, b * fma(r, phi, Un));
+                v[c] = fma(a, Uo, b * fma(r, phi, Un));
+                st[i * 5 + c] = v[c];
             }
-        }
-        __syncthreads();
-        if (s + 1 < la.stages) {
-            for (int e = tt; e < la.nrules; e += T) {
-                const int d = rules[e], q = rules[kLineRules + e];
-                double v[5];
-                if (q >= 0) {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) v[c] = st[q * 5 + c];
-                } else {
-                    Conservative(la.gamma, la.given[-1 - q], v);
-                }
+            for (int e = 0; e < kRefillPerNode; ++e) {
+                const int d = FeedAt(feed, e);
+                if (!refill || d < 0) continue;
 #pragma unroll
                 for (int c = 0; c < 5; ++c) st[d * 5 + c] = v[c];
             }
-            __syncthreads();
         }
+        if (refill && constRule >= 0) {
+            double v[5];
+            Conservative(la.gamma, la.given[-1 - rules[kLineRules + constRule]], v);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = v[c];
+        }
+        __syncthreads();
     }
     if (fl) {
 #pragma unroll
@@ -1321,7 +1379,7 @@ __global__ __launch_bounds__(256) void cfl_batch(const CflChunk *chunks, int nch
     }
 }
 
-constexpr int kClock = 6;           // doubles per case clock: now, end, rcInt, tmInt, stepC, stepN
+constexpr int kClock = 6;           // doubles per batch case clock: now, end, rcInt, tmInt, stepC, stepN
 
 // One step of the solver loop's clock (solve.c:59-78): dt = CFL min_s(d_s /
 // vmax_s), rectified to the output instants and the end time.  ck = {now,
@@ -1377,10 +1435,27 @@ struct StepGeometry {
 
 // Time step of a single case on the device: step[0] = dt, step[1] = dt / 2
 // (the operators' step), hist[0] = dt (the march's record of its steps).
+// The clock carries the output cadence of the solver loop (solve.c:57-62,
+// 91-104): clock[7 + n] accumulates the steps of data kind n, clock[12 + n]
+// is its interval; after a step at which some output is due (its
+// accumulated time reaches the interval, or the loop ends) clock[6] is set
+// and later steps leave the clock as it is, so that the host writes the
+// output of exactly that state.
 __global__ void dt_single(StepGeometry sg, const unsigned long long *vmax, double *clock, double *step, double *hist)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const double h = AdvanceClock(sg.d, sg.cfl, vmax, clock);
+    double h = 0.0;
+    if (clock[6] == 0.0) {
+        h = AdvanceClock(sg.d, sg.cfl, vmax, clock);
+        if (h > 0.0) {
+            bool due = false;
+            for (int n = 0; n < 5; ++n) {
+                clock[7 + n] = clock[7 + n] + h;
+                if (clock[7 + n] >= clock[12 + n] || clock[0] == clock[1] || clock[4] == clock[5]) due = true;
+            }
+            if (due) clock[6] = 1.0;
+        }
+    }
     step[0] = h;
     step[1] = 0.5 * h;
     hist[0] = h;
@@ -1738,6 +1813,24 @@ void LaunchLineS(ArtraDevice *dev, const LineShape &sh, const LineArgs &la, cons
     }
 }
 
+// The hybrid's lean pass over every line of axis S, then the full hybrid
+// body over the lines it deferred (the list's length stays on the device).
+template <int S>
+void LaunchLean(ArtraDevice *dev, const LineShape &sh, const LineArgs &la, const LineHybrid &hy, size_t leanLds,
+                bool mean)
+{
+    const unsigned listGrid = (unsigned)std::min<long>(sh.lines, kListBlocks);
+    if (mean) {
+        hipLaunchKernelGGL((rk_lean<S, true>), dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
+                           hy, dev->deferList, dev->deferCount, dev->deferTotal + S);
+    } else {
+        hipLaunchKernelGGL((rk_lean<S, false>), dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la,
+                           dev->mdl, hy, dev->deferList, dev->deferCount, dev->deferTotal + S);
+    }
+    hipLaunchKernelGGL(rk_line_list<S>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
+                       dev->deferList, dev->deferCount);
+}
+
 // All RK stages of the directional operator along p in one launch; the
 // result lands in TO, whose full boundary fill follows.
 int LineSweep(ArtraDevice *dev, double dt, int p)
@@ -1798,23 +1891,10 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
         // lean pass over every line, then the full hybrid kernel over the
         // lines it deferred (the list's length stays on the device)
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, dev->stream, dev->deferCount, 1L, 0);
-        const unsigned listGrid = (unsigned)std::min<long>(sh.lines, kListBlocks);
-        if (p == 0) {
-            hipLaunchKernelGGL(rk_lean<0>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
-                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
-            hipLaunchKernelGGL(rk_line_list<0>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
-                               dev->deferList, dev->deferCount);
-        } else if (p == 1) {
-            hipLaunchKernelGGL(rk_lean<1>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
-                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
-            hipLaunchKernelGGL(rk_line_list<1>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
-                               dev->deferList, dev->deferCount);
-        } else {
-            hipLaunchKernelGGL(rk_lean<2>, dim3((unsigned)sh.lines), dim3(sh.T), leanLds, dev->stream, la, dev->mdl,
-                               hy, dev->deferList, dev->deferCount, dev->deferTotal + p);
-            hipLaunchKernelGGL(rk_line_list<2>, dim3(listGrid), dim3(sh.T), sh.lds, dev->stream, la, dev->mdl, hy,
-                               dev->deferList, dev->deferCount);
-        }
+        const bool mean = !dev->mdl.roe && !dev->mdl.sw;
+        if (p == 0) LaunchLean<0>(dev, sh, la, hy, leanLds, mean);
+        else if (p == 1) LaunchLean<1>(dev, sh, la, hy, leanLds, mean);
+        else LaunchLean<2>(dev, sh, la, hy, leanLds, mean);
         dev->leanLines[p] += sh.lines;
     } else if (p == 0) {
         LaunchLineS<0>(dev, sh, la, hy);
@@ -1882,6 +1962,15 @@ void DeriveLineRules(ArtraDevice *dev, const ArtraBcGather *gt)
             }
         }
         if (!ok || (int)ref.size() > kLineRules) continue;
+        // copy rules read interior nodes, each feeding at most kRefillPerNode
+        // ghosts (the owner of a node refills them when it updates it)
+        for (const auto &r : ref) {
+            if (r.second < 0) continue;
+            int feeds = 0;
+            for (const auto &o : ref) feeds += (o.second == r.second);
+            ok = ok && r.second >= g.lo[p] && r.second < g.hi[p] && feeds <= kRefillPerNode;
+        }
+        if (!ok) continue;
         for (int pos = g.lo[p] - half; pos < g.hi[p] + half; ++pos) {
             if (pos >= g.lo[p] && pos < g.hi[p]) continue;
             bool hit = pos >= 0 && pos < ext[p];
@@ -2014,8 +2103,8 @@ int EnqueueSpeeds(ArtraDevice *dev, const double floor[3], unsigned long long *v
 }
 
 // Device memory of a device-stepped march: the speed maxima (3 x u64, one
-// spare), the clock (kClock), the step (dt, dt / 2) and the record of up to
-// cap steps, all 8-byte words.
+// spare), the clock (kMarchClock), the step (dt, dt / 2) and the record of
+// up to cap steps, all 8-byte words.
 struct MarchScratch {
     unsigned long long *vmax;
     double *clock;
@@ -2029,7 +2118,7 @@ MarchScratch MarchScratchAt(void *base, int cap)
     MarchScratch m;
     m.vmax = (unsigned long long *)base;
     m.clock = (double *)base + 4;
-    m.step = m.clock + kClock;
+    m.step = m.clock + kMarchClock;
     m.hist = m.step + 2;
     m.cap = cap;
     return m;
@@ -2040,14 +2129,14 @@ MarchScratch MarchScratchAt(void *base, int cap)
 // step from device memory, with no host round trip in between.  clock
 // (in/out) is the solver loop's clock, dts[i] the steps taken (0 once the
 // clock has reached its end), *advanced the steps that moved the clock.
-int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kClock], const double floor[3], double cfl,
+int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kMarchClock], const double floor[3], double cfl,
                int steps, double *dts, int *advanced)
 {
     *advanced = 0;
     if (steps <= 0) return 0;
     if (steps > ms.cap) return Fail("a device march takes at most %d steps per call (%d asked)", ms.cap, steps);
     const double step0 = clock[4];
-    HIP_TRY(hipMemcpyAsync(ms.clock, clock, kClock * sizeof(double), hipMemcpyHostToDevice, dev->stream));
+    HIP_TRY(hipMemcpyAsync(ms.clock, clock, kMarchClock * sizeof(double), hipMemcpyHostToDevice, dev->stream));
     StepGeometry sg;
     for (int s = 0; s < 3; ++s) sg.d[s] = dev->par.d[s];
     sg.cfl = cfl;
@@ -2062,7 +2151,7 @@ int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kClock], c
     }
     dev->stepDev = nullptr;
     if (rc) return 1;
-    HIP_TRY(hipMemcpyAsync(clock, ms.clock, kClock * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+    HIP_TRY(hipMemcpyAsync(clock, ms.clock, kMarchClock * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipMemcpyAsync(dts, ms.hist, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     *advanced = (int)(clock[4] - step0);
@@ -2405,13 +2494,13 @@ int artra_dev_gather_primitive(ArtraDevice *dev, const int *nodes, int count, do
     return 0;
 }
 
-int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[3], double cfl, int steps,
+int artra_dev_march_steps(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[3], double cfl, int steps,
                           double *dts, int *advanced)
 {
     return MarchSteps(dev, MarchScratchAt(dev->marchBuf, kMarchChunk), clock, floor, cfl, steps, dts, advanced);
 }
 
-int artra_dev_march_noalloc(ArtraDevice *dev, double clock[6], const double floor[3], double cfl, int steps,
+int artra_dev_march_noalloc(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[3], double cfl, int steps,
                             double *dts, int *advanced)
 {
     *advanced = 0;

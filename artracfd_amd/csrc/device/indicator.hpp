@@ -273,15 +273,23 @@ __device__ __forceinline__ void node_velocity_enthalpy(double gamma, const doubl
 }
 
 // Sum and difference of the two linear combinations, sum c+ U + sum c- U and
-// sum c+ U - sum c- U, from the six-node stencil in symmetric form.
+// sum c+ U - sum c- U, of one component's six stencil values in symmetric
+// form.
+__host__ __device__ __forceinline__ void linear_sum_diff_1(const double w[6], double &Sg, double &Dl)
+{
+    const double a05 = w[0] + w[5], a14 = w[1] + w[4], a23 = w[2] + w[3];
+    const double d05 = w[0] - w[5], d14 = w[1] - w[4], d23 = w[2] - w[3];
+    Sg = fma(2.0, a05, fma(-16.0, a14, 74.0 * a23)) * (1.0 / 60.0);
+    Dl = fma(2.0, d05, fma(-10.0, d14, 20.0 * d23)) * (1.0 / 60.0);
+}
+
+// The same for every component of the six-node stencil st[node][component].
 __host__ __device__ __forceinline__ void linear_sum_diff(const double st[6][5], double Sg[5], double Dl[5])
 {
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        const double a05 = st[0][c] + st[5][c], a14 = st[1][c] + st[4][c], a23 = st[2][c] + st[3][c];
-        const double d05 = st[0][c] - st[5][c], d14 = st[1][c] - st[4][c], d23 = st[2][c] - st[3][c];
-        Sg[c] = fma(2.0, a05, fma(-16.0, a14, 74.0 * a23)) * (1.0 / 60.0);
-        Dl[c] = fma(2.0, d05, fma(-10.0, d14, 20.0 * d23)) * (1.0 / 60.0);
+        const double w[6] = {st[0][c], st[1][c], st[2][c], st[3][c], st[4][c], st[5][c]};
+        linear_sum_diff_1(w, Sg[c], Dl[c]);
     }
 }
 

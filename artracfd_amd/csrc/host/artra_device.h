@@ -93,18 +93,23 @@ int artra_dev_max_speed(ArtraDevice *dev, const double floor[DIMS], double vmax[
 /* one EvolveFluidDynamics step of size dt */
 int artra_dev_evolve(ArtraDevice *dev, double dt);
 /* up to ARTRA_DEV_MARCH_CHUNK whole time steps with dt computed on the
- * device (CFL maxima folded with the floors, the solver loop's clock
- * clock = {now, end, rcInt, tmInt, stepC, stepN}, EvolveFluidDynamics),
- * no host round trip between steps; clock is updated, dts[i] receives each
- * step's dt (0 once the clock stopped), *advanced the steps that moved it */
+ * device (CFL maxima folded with the floors, the solver loop's clock,
+ * EvolveFluidDynamics), no host round trip between steps.  clock =
+ * {now, end, rcInt, tmInt, stepC, stepN, halt, rcData[5], dtData[5]}: the
+ * loop's time, output-instant and step counters and its output cadence
+ * (solve.c:57-104); the march stops moving the clock after the first step
+ * at which an output falls due (halt becomes 1) or the loop ends.  clock
+ * is updated, dts[i] receives each step's dt (0 once the clock stopped),
+ * *advanced the steps that moved it */
 #define ARTRA_DEV_MARCH_CHUNK 64
-int artra_dev_march_steps(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
-                          double *dts, int *advanced);
+#define ARTRA_DEV_MARCH_CLOCK 17
+int artra_dev_march_steps(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[DIMS],
+                          double cfl, int steps, double *dts, int *advanced);
 /* the same with the device scratch in a static per-GPU array (no device
  * allocation at all); at most ARTRA_DEV_NOALLOC_STEPS steps per call */
 #define ARTRA_DEV_NOALLOC_STEPS ARTRA_NOALLOC_MAX_STEPS
-int artra_dev_march_noalloc(ArtraDevice *dev, double clock[6], const double floor[DIMS], double cfl, int steps,
-                            double *dts, int *advanced);
+int artra_dev_march_noalloc(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[DIMS],
+                            double cfl, int steps, double *dts, int *advanced);
 int artra_dev_sync(ArtraDevice *dev);
 /* surface force: the table once, then per call the per-body sums of the
  * pressure force, wall shear and torque over its nodes in table order and

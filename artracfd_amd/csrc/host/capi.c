@@ -250,6 +250,19 @@ int artra_hip_evolve(ArtraHipSolver *s, Real dt)
     return artra_dev_evolve(s->dev, dt) ? DevError() : 0;
 }
 
+/* The device march's clock for the plain time loop: no output cadence
+ * (the intervals are infinite, so the march only stops at the loop's end). */
+static void MarchClock(const Time *time, Real rcInt, Real tmInt, double clock[ARTRA_DEV_MARCH_CLOCK])
+{
+    const double base[6] = {time->now, time->end, rcInt, tmInt, (double)time->stepC, (double)time->stepN};
+    for (int q = 0; q < 6; ++q) clock[q] = base[q];
+    clock[6] = 0.0;
+    for (int n = 0; n < NPROBE; ++n) {
+        clock[7 + n] = 0.0;
+        clock[12 + n] = INFINITY;
+    }
+}
+
 /* Steps the device may take in one call without passing the end time: the
  * remaining time over the last step, less a margin of two steps for a
  * growing dt (a step past the end on the device is a no-op anyway). */
@@ -280,7 +293,8 @@ int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_don
      * rectification), in chunks that the host checks between */
     while (time->now < time->end && time->stepC < time->stepN && done < max_steps) {
         const int k = ChunkSteps(time, lastDt, max_steps - done);
-        double clock[6] = {time->now, time->end, s->rcInt, tmInt, (double)time->stepC, (double)time->stepN};
+        double clock[ARTRA_DEV_MARCH_CLOCK];
+        MarchClock(time, s->rcInt, tmInt, clock);
         int adv = 0;
         if (artra_dev_march_steps(s->dev, clock, floor, time->numCFL, k, rec, &adv)) return DevError();
         time->now = clock[0];
@@ -312,7 +326,8 @@ int artra_hip_march_noalloc(ArtraHipSolver *s, Time *time, int max_steps, int *s
     if (time->stepN - time->stepC < k) k = time->stepN - time->stepC;
     /* every step in one device call: steps past the end time are no-ops */
     double rec[ARTRA_NOALLOC_MAX_STEPS];
-    double clock[6] = {time->now, time->end, s->rcInt, tmInt, (double)time->stepC, (double)time->stepN};
+    double clock[ARTRA_DEV_MARCH_CLOCK];
+    MarchClock(time, s->rcInt, tmInt, clock);
     int adv = 0;
     if (artra_dev_march_noalloc(s->dev, clock, floor, time->numCFL, k, rec, &adv)) return DevError();
     time->now = clock[0];
@@ -719,5 +734,19 @@ int artra_case_clock(const ArtraCase *c, double *now, int *step)
     *now = c->time.now;
     *step = c->time.stepC;
     return 0;
+}
+
+int artra_case_forces(const ArtraCase *c, Real *forces)
+{
+    const Geometry *geo = &c->space.geo;
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *p = geo->poly + n;
+        for (int d = 0; d < DIMS; ++d) {
+            forces[9 * n + d] = p->Fp[d];
+            forces[9 * n + 3 + d] = p->Fv[d];
+            forces[9 * n + 6 + d] = p->Tt[d];
+        }
+    }
+    return geo->totN;
 }
 

@@ -9,6 +9,7 @@
  */
 #define _POSIX_C_SOURCE 200809L
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -368,4 +369,772 @@ int artra_hip_write_probe_data(ArtraHipSolver *s, const Time *time, int kind)
     default:
         return artra_set_error(1, "artra_hip_write_probe_data: kind %d is not a probe (0..3)", kind);
     }
+}
+
+/* ------------------------------------------------------------ field output */
+/* The field files are formatted from one download of level TO; the
+ * "%.6g" text of each array is formatted in parallel chunks (the bulk of
+ * the writer's time) and written in order. */
+enum { PV_RHO, PV_U, PV_V, PV_W, PV_P, PV_T, PV_DID, PV_FID, PV_LID, PV_GST, PV_VEL, PV_POINTS };
+
+static void NodeValues(const ArtraHipSolver *s, const double *U, int k, int j, int i, int what, double out[3])
+{
+    const Space *space = s->space;
+    const Partition *part = &space->part;
+    const long N = s->N;
+    const int idx = artra_index(k, j, i, part->n[Y], part->n[X]);
+    const double U0 = U[idx], U1 = U[N + idx], U2 = U[2 * N + idx], U3 = U[3 * N + idx], U4 = U[4 * N + idx];
+    const Node *nd = space->node + idx;
+    switch (what) {
+    case PV_RHO: out[0] = U0; break;
+    case PV_U: out[0] = U1 / U0; break;
+    case PV_V: out[0] = U2 / U0; break;
+    case PV_W: out[0] = U3 / U0; break;
+    case PV_P: out[0] = (U4 - 0.5 * (U1 * U1 + U2 * U2 + U3 * U3) / U0) * (s->model->gamma - 1.0); break;
+    case PV_T: out[0] = (U4 - 0.5 * (U1 * U1 + U2 * U2 + U3 * U3) / U0) / (U0 * s->model->cv); break;
+    case PV_DID: out[0] = nd->did; break;
+    case PV_FID: out[0] = nd->fid; break;
+    case PV_LID: out[0] = nd->lid; break;
+    case PV_GST: out[0] = nd->gst; break;
+    case PV_VEL:
+        out[0] = U1 / U0;
+        out[1] = U2 / U0;
+        out[2] = U3 / U0;
+        break;
+    default:
+        NodePoint(part, k, j, i, out);
+        break;
+    }
+}
+
+/* Every node of the output range in (k, j, i) order as "%.6g " text (three
+ * values per node for the vector and the points). */
+static int WriteRangeValues(FILE *fp, const ArtraHipSolver *s, const double *U, int range, int what)
+{
+    const Partition *part = &s->space->part;
+    const int x0 = part->ns[range][X][MIN], nx = part->ns[range][X][MAX] - x0;
+    const int y0 = part->ns[range][Y][MIN], ny = part->ns[range][Y][MAX] - y0;
+    const int z0 = part->ns[range][Z][MIN], nz = part->ns[range][Z][MAX] - z0;
+    const long total = (long)nx * ny * nz;
+    const int per = (what >= PV_VEL) ? 3 : 1;
+    const long chunk = 1L << 15;
+    const long nchunk = (total + chunk - 1) / chunk;
+    char **text = calloc((size_t)(nchunk > 0 ? nchunk : 1), sizeof(char *));
+    size_t *len = calloc((size_t)(nchunk > 0 ? nchunk : 1), sizeof(size_t));
+    if (!text || !len) {
+        free(text);
+        free(len);
+        return artra_set_error(1, "out of host memory");
+    }
+    int failed = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < nchunk; ++c) {
+        const long lo = c * chunk, hi = (lo + chunk < total) ? lo + chunk : total;
+        const size_t cap = (size_t)(hi - lo) * per * 16 + 1;
+        char *buf = malloc(cap);
+        if (!buf) {
+            failed = 1;
+            continue;
+        }
+        size_t at = 0;
+        for (long t = lo; t < hi; ++t) {
+            const int i = x0 + (int)(t % nx), j = y0 + (int)((t / nx) % ny), k = z0 + (int)(t / ((long)nx * ny));
+            double v[3];
+            NodeValues(s, U, k, j, i, what, v);
+            for (int q = 0; q < per; ++q) at += (size_t)snprintf(buf + at, cap - at, "%.6g ", v[q]);
+        }
+        text[c] = buf;
+        len[c] = at;
+    }
+    for (long c = 0; c < nchunk; ++c) {
+        if (!failed && text[c]) fwrite(text[c], 1, len[c], fp);
+        free(text[c]);
+    }
+    free(text);
+    free(len);
+    return failed ? artra_set_error(1, "out of host memory") : 0;
+}
+
+/* The line of the collection file where the next data set is appended:
+ * the first line reading "</Collection>" once normalised
+ * (commons.c:167-182). */
+static void SeekLine(FILE *fp, const char *line)
+{
+    char str[256];
+    fpos_t pos;
+    fgetpos(fp, &pos);
+    while (fgets(str, sizeof str, fp)) {
+        artra_parse_line(str);
+        if (0 == strcmp(str, line)) {
+            fsetpos(fp, &pos);
+            break;
+        }
+        fgetpos(fp, &pos);
+    }
+}
+
+/* The transient collection file of one data set and its per-output case
+ * file (paraview_writer.c:56-97). */
+static int PvCaseFiles(const Time *time, const char *rname, const char *bname, const char *fext)
+{
+    char name[160];
+    FILE *fp;
+    if (0 == time->stepC) {
+        snprintf(name, sizeof name, "%s.pvd", rname);
+        if (!(fp = OpenFile(name, "w"))) return 1;
+        fprintf(fp, "<?xml version=\"1.0\"?>\n");
+        fprintf(fp, "<VTKFile type=\"Collection\" version=\"1.0\" byte_order=\"%s\">\n", "LittleEndian");
+        fprintf(fp, "  <Collection>\n");
+        fprintf(fp, "  </Collection>\n");
+        fprintf(fp, "</VTKFile>\n");
+        fclose(fp);
+    }
+    snprintf(name, sizeof name, "%s.pvd", bname);
+    if (!(fp = OpenFile(name, "w"))) return 1;
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"Collection\" version=\"1.0\" byte_order=\"%s\">\n", "LittleEndian");
+    fprintf(fp, "  <Collection>\n");
+    fprintf(fp, "    <DataSet timestep=\"%.6g\" group=\"\" part=\"0\"\n", time->now);
+    fprintf(fp, "             file=\"%s%s\"/>\n", bname, fext);
+    fprintf(fp, "  </Collection>\n");
+    fprintf(fp, "</VTKFile>\n");
+    fprintf(fp, "<!--\n");
+    fprintf(fp, "  Time %.6g\n", time->now);
+    fprintf(fp, "  Step %d\n", time->stepC);
+    fprintf(fp, "-->\n");
+    fclose(fp);
+    snprintf(name, sizeof name, "%s.pvd", rname);
+    if (!(fp = OpenFile(name, "r+"))) return 1;
+    SeekLine(fp, "</Collection>");
+    fprintf(fp, "    <DataSet timestep=\"%.6g\" group=\"\" part=\"0\"\n", time->now);
+    fprintf(fp, "             file=\"%s%s\"/>\n", bname, fext);
+    fprintf(fp, "  </Collection>\n");
+    fprintf(fp, "</VTKFile>\n");
+    fclose(fp);
+    return 0;
+}
+
+/* WriteStructuredDataParaview (paraview_writer.c:31-197): the output range
+ * (ghost layers of the physical boundary included) as a VTK structured
+ * grid with ten scalars, the velocity vector and the node coordinates. */
+static int PvStructured(ArtraHipSolver *s, const Time *time)
+{
+    static const char *sca[10] = {"rho", "u", "v", "w", "p", "T", "did", "fid", "lid", "gst"};
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "field", time->dataC);
+    if (PvCaseFiles(time, "field", bname, ".vts")) return 1;
+    if (artra_dev_download(s->dev, TO, s->staging)) return artra_dev_failed();
+    const Partition *part = &s->space->part;
+    const int ne[DIMS] = {part->ns[PIO][X][MAX] - part->ns[PIO][X][MIN] - 1,
+                          part->ns[PIO][Y][MAX] - part->ns[PIO][Y][MIN] - 1,
+                          part->ns[PIO][Z][MAX] - part->ns[PIO][Z][MIN] - 1};
+    snprintf(name, sizeof name, "%s%s", bname, ".vts");
+    FILE *fp = OpenFile(name, "w");
+    if (!fp) return 1;
+    int rc = 0;
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"StructuredGrid\" version=\"1.0\" byte_order=\"%s\">\n", "LittleEndian");
+    fprintf(fp, "  <StructuredGrid WholeExtent=\"%d %d %d %d %d %d\">\n", 0, ne[X], 0, ne[Y], 0, ne[Z]);
+    fprintf(fp, "    <Piece Extent=\"%d %d %d %d %d %d\">\n", 0, ne[X], 0, ne[Y], 0, ne[Z]);
+    fprintf(fp, "      <PointData>\n");
+    for (int q = 0; q < 10 && !rc; ++q) {
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"%s\" format=\"ascii\">\n", "Float32", sca[q]);
+        fprintf(fp, "          ");
+        rc = WriteRangeValues(fp, s, s->staging, PIO, q);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (!rc) {
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"%s\" NumberOfComponents=\"3\" format=\"ascii\">\n", "Float32",
+                "Vel");
+        fprintf(fp, "          ");
+        rc = WriteRangeValues(fp, s, s->staging, PIO, PV_VEL);
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    if (!rc) {
+        fprintf(fp, "      </PointData>\n");
+        fprintf(fp, "      <CellData>\n");
+        fprintf(fp, "      </CellData>\n");
+        fprintf(fp, "      <Points>\n");
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"points\" NumberOfComponents=\"3\" format=\"ascii\">\n",
+                "Float32");
+        fprintf(fp, "          ");
+        rc = WriteRangeValues(fp, s, s->staging, PIO, PV_POINTS);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "      </Points>\n");
+        fprintf(fp, "    </Piece>\n");
+        fprintf(fp, "  </StructuredGrid>\n");
+        fprintf(fp, "</VTKFile>\n");
+    }
+    fclose(fp);
+    return rc;
+}
+
+/* WritePolyStateData (data_stream.c:131-152). */
+static void PolyState(FILE *fp, int pm, int pn, const Geometry *geo)
+{
+    for (int n = pm; n < pn; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        fprintf(fp, "  %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %d\n",
+                poly->O[X], poly->O[Y], poly->O[Z], poly->r, poly->V[TO][X], poly->V[TO][Y], poly->V[TO][Z],
+                poly->W[TO][X], poly->W[TO][Y], poly->W[TO][Z], poly->rho, poly->T, poly->cf, poly->area,
+                poly->volume, poly->mid);
+        fprintf(fp, "  %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g, %.6g\n",
+                poly->at[TO][X], poly->at[TO][Y], poly->at[TO][Z], poly->ar[TO][X], poly->ar[TO][Y], poly->ar[TO][Z],
+                poly->at[TN][X], poly->at[TN][Y], poly->at[TN][Z], poly->g[X], poly->g[Y], poly->g[Z],
+                poly->ar[TN][X], poly->ar[TN][Y], poly->ar[TN][Z], poly->to);
+    }
+}
+
+/* WritePolyDataParaview for the analytical spheres (paraview_writer.c:216-303). */
+static int PvSpheres(const Time *time, const Geometry *geo)
+{
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "geo_sph", time->dataC);
+    if (PvCaseFiles(time, "geo_sph", bname, ".vtp")) return 1;
+    snprintf(name, sizeof name, "%s%s", bname, ".vtp");
+    FILE *fp = OpenFile(name, "w");
+    if (!fp) return 1;
+    const int pm = 0, pn = geo->sphN;
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"%s\">\n", "LittleEndian");
+    fprintf(fp, "  <PolyData>\n");
+    fprintf(fp, "    <Piece NumberOfPoints=\"%d\" NumberOfVerts=\"1\" NumberOfPolys=\"0\">\n", pn - pm);
+    fprintf(fp, "      <PointData>\n");
+    static const char *sca[2] = {"r", "did"};
+    for (int q = 0; q < 2; ++q) {
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"%s\" format=\"ascii\">\n", "Float32", sca[q]);
+        fprintf(fp, "          ");
+        for (int n = pm; n < pn; ++n) fprintf(fp, "%.6g ", (0 == q) ? geo->poly[n].r : (Real)(n + 1));
+        fprintf(fp, "\n        </DataArray>\n");
+    }
+    fprintf(fp, "        <DataArray type=\"%s\" Name=\"%s\" NumberOfComponents=\"3\" format=\"ascii\">\n", "Float32",
+            "Vel");
+    fprintf(fp, "          ");
+    for (int n = pm; n < pn; ++n) {
+        const Real *V = geo->poly[n].V[TO];
+        fprintf(fp, "%.6g %.6g %.6g ", V[X], V[Y], V[Z]);
+    }
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "      </PointData>\n");
+    fprintf(fp, "      <CellData>\n");
+    fprintf(fp, "      </CellData>\n");
+    fprintf(fp, "      <Points>\n");
+    fprintf(fp, "        <DataArray type=\"%s\" Name=\"points\" NumberOfComponents=\"3\" format=\"ascii\">\n", "Float32");
+    fprintf(fp, "          ");
+    for (int n = pm; n < pn; ++n) {
+        const Real *O = geo->poly[n].O;
+        fprintf(fp, "%.6g %.6g %.6g ", O[X], O[Y], O[Z]);
+    }
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "      </Points>\n");
+    fprintf(fp, "      <Verts>\n");
+    fprintf(fp, "        <DataArray type=\"%s\" Name=\"connectivity\" format=\"ascii\">\n", "Int32");
+    fprintf(fp, "          ");
+    for (int n = pm; n < pn; ++n) fprintf(fp, "%d ", n);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "        <DataArray type=\"%s\" Name=\"offsets\" format=\"ascii\">\n", "Int32");
+    fprintf(fp, "          ");
+    fprintf(fp, "%d ", pn - pm);
+    fprintf(fp, "\n        </DataArray>\n");
+    fprintf(fp, "      </Verts>\n");
+    fprintf(fp, "      <Polys>\n");
+    fprintf(fp, "      </Polys>\n");
+    fprintf(fp, "    </Piece>\n");
+    fprintf(fp, "  </PolyData>\n");
+    fprintf(fp, "</VTKFile>\n");
+    fprintf(fp, "<!--\n");
+    PolyState(fp, pm, pn, geo);
+    fprintf(fp, "-->\n");
+    fclose(fp);
+    return 0;
+}
+
+/* WritePolyDataParaview for the triangulated bodies (paraview_writer.c:304-398). */
+static int PvPolygons(const Time *time, const Geometry *geo)
+{
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "geo_stl", time->dataC);
+    if (PvCaseFiles(time, "geo_stl", bname, ".vtp")) return 1;
+    snprintf(name, sizeof name, "%s%s", bname, ".vtp");
+    FILE *fp = OpenFile(name, "w");
+    if (!fp) return 1;
+    const int pm = geo->sphN, pn = geo->totN;
+    fprintf(fp, "<?xml version=\"1.0\"?>\n");
+    fprintf(fp, "<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"%s\">\n", "LittleEndian");
+    fprintf(fp, "  <PolyData>\n");
+    for (int m = pm; m < pn; ++m) {
+        const Polyhedron *poly = geo->poly + m;
+        fprintf(fp, "    <Piece NumberOfPoints=\"%d\" NumberOfVerts=\"0\" NumberOfPolys=\"%d\">\n", poly->vertN,
+                poly->faceN);
+        fprintf(fp, "      <!--\n");
+        fprintf(fp, "        vertN = %d\n", poly->vertN);
+        fprintf(fp, "        edgeN = %d\n", poly->edgeN);
+        fprintf(fp, "        faceN = %d\n", poly->faceN);
+        fprintf(fp, "      -->\n");
+        fprintf(fp, "      <PointData>\n");
+        fprintf(fp, "      </PointData>\n");
+        fprintf(fp, "      <CellData>\n");
+        fprintf(fp, "      </CellData>\n");
+        fprintf(fp, "      <Points>\n");
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"points\" NumberOfComponents=\"3\" format=\"ascii\">\n",
+                "Float32");
+        fprintf(fp, "          ");
+        for (int n = 0; n < poly->vertN; ++n) fprintf(fp, "%.6g %.6g %.6g ", poly->v[n][X], poly->v[n][Y], poly->v[n][Z]);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "      </Points>\n");
+        fprintf(fp, "      <Verts>\n");
+        fprintf(fp, "      </Verts>\n");
+        fprintf(fp, "      <Polys>\n");
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"connectivity\" format=\"ascii\">\n", "Int32");
+        fprintf(fp, "          ");
+        for (int n = 0; n < poly->faceN; ++n) fprintf(fp, "%d %d %d ", poly->f[n][0], poly->f[n][1], poly->f[n][2]);
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "        <DataArray type=\"%s\" Name=\"offsets\" format=\"ascii\">\n", "Int32");
+        fprintf(fp, "          ");
+        for (int n = 0; n < poly->faceN; ++n) fprintf(fp, "%d ", 3 * (n + 1));
+        fprintf(fp, "\n        </DataArray>\n");
+        fprintf(fp, "      </Polys>\n");
+        fprintf(fp, "    </Piece>\n");
+    }
+    fprintf(fp, "  </PolyData>\n");
+    fprintf(fp, "</VTKFile>\n");
+    fprintf(fp, "<!--\n");
+    PolyState(fp, pm, pn, geo);
+    fprintf(fp, "-->\n");
+    fclose(fp);
+    return 0;
+}
+
+/* WriteStateData (data_stream.c:115-125). */
+static int StateLog(const Time *time)
+{
+    FILE *fp = OpenFile("artracfd.log", "w");
+    if (!fp) return 1;
+    fprintf(fp, "#------------------------------------------------------------------------------\n");
+    fprintf(fp, "%d  # completion checkpoint\n", (time->now == time->end) || (time->stepC == time->stepN));
+    fprintf(fp, "%d  # data checkpoint\n", time->dataC);
+    fprintf(fp, "#------------------------------------------------------------------------------\n");
+    fclose(fp);
+    return 0;
+}
+
+/* ------------------------------------------------------------ EnSight Gold */
+/* The EnSight writer's files (ensight_writer.c:34-452): C-binary geometry
+ * and variable files with FP32 values and 80-byte string records, and the
+ * text case files.  A string record is the writer's 80-byte buffer after a
+ * strncpy (zero-padded) or an snprintf (which leaves the bytes past its
+ * terminator as they were), so the buffer persists through a file as the
+ * reference's does. */
+typedef struct {
+    char str[80];
+    FILE *fp;
+    int failed;
+} EnFile;
+
+static void EnCopy(EnFile *f, const char *text)
+{
+    strncpy(f->str, text, sizeof f->str);
+    f->failed |= 1 != fwrite(f->str, sizeof f->str, 1, f->fp);
+}
+
+static void EnPrint(EnFile *f, const char *fmt, int a, int b, int c, int nargs)
+{
+    if (1 == nargs) snprintf(f->str, sizeof f->str, fmt, a);
+    else snprintf(f->str, sizeof f->str, fmt, a, b, c);
+    f->failed |= 1 != fwrite(f->str, sizeof f->str, 1, f->fp);
+}
+
+static void EnInt(EnFile *f, int v)
+{
+    f->failed |= 1 != fwrite(&v, sizeof v, 1, f->fp);
+}
+
+static void EnReal(EnFile *f, double v)
+{
+    const float x = (float)v;
+    f->failed |= 1 != fwrite(&x, sizeof x, 1, f->fp);
+}
+
+static int EnOpen(EnFile *f, const char *name)
+{
+    memset(f->str, 0, sizeof f->str);
+    f->failed = 0;
+    f->fp = OpenFile(name, "wb");
+    return f->fp ? 0 : 1;
+}
+
+static int EnClose(EnFile *f, const char *name)
+{
+    const int bad = (0 != fclose(f->fp)) || f->failed;
+    return bad ? artra_set_error(1, "writing %s failed", name) : 0;
+}
+
+static void EnHeader(EnFile *f)
+{
+    EnCopy(f, "C Binary");
+    EnCopy(f, "Ensight Geometry File");
+    EnCopy(f, "Written by ArtraCFD");
+    EnCopy(f, "node id off");
+    EnCopy(f, "element id off");
+}
+
+/* The transient case file (at the first output) and the per-output case
+ * file of one data set (ensight_writer.c:61-134). */
+static int EnCaseFiles(const Time *time, const char *rname, const char *bname, const char *gtag, const char *vtag,
+                       const char *const *sca, int scaN, const char *const *vec, int vecN)
+{
+    char name[160];
+    FILE *fp;
+    if (0 == time->stepC) {
+        snprintf(name, sizeof name, "%s.case", rname);
+        if (!(fp = OpenFile(name, "w"))) return 1;
+        fprintf(fp, "FORMAT\n");
+        fprintf(fp, "type: ensight gold\n");
+        fprintf(fp, "\n");
+        fprintf(fp, "GEOMETRY\n");
+        fprintf(fp, "model: %s%s.geo\n", rname, gtag);
+        fprintf(fp, "\n");
+        fprintf(fp, "VARIABLE\n");
+        for (int n = 0; n < scaN; ++n) fprintf(fp, "scalar per node:  1  %3s  %s%s.%s\n", sca[n], rname, vtag, sca[n]);
+        for (int n = 0; n < vecN; ++n) fprintf(fp, "vector per node:  1  %3s  %s%s.%s\n", vec[n], rname, vtag, vec[n]);
+        fprintf(fp, "\n");
+        fprintf(fp, "TIME\n");
+        fprintf(fp, "time set: 1\n");
+        fprintf(fp, "number of steps:          0          \n");
+        fprintf(fp, "filename start number:    0\n");
+        fprintf(fp, "filename increment:       1\n");
+        fprintf(fp, "time values:  ");
+        fclose(fp);
+    }
+    snprintf(name, sizeof name, "%s.case", bname);
+    if (!(fp = OpenFile(name, "w"))) return 1;
+    fprintf(fp, "FORMAT\n");
+    fprintf(fp, "type: ensight gold\n");
+    fprintf(fp, "\n");
+    fprintf(fp, "GEOMETRY\n");
+    if ('\0' == *gtag) fprintf(fp, "model: %s.geo\n", rname);
+    else fprintf(fp, "model: %s.geo\n", bname);
+    fprintf(fp, "\n");
+    fprintf(fp, "VARIABLE\n");
+    fprintf(fp, "constant per case:  Time  %.6g\n", time->now);
+    fprintf(fp, "constant per case:  Step  %d\n", time->stepC);
+    for (int n = 0; n < scaN; ++n) fprintf(fp, "scalar per node:     %3s  %s.%s\n", sca[n], bname, sca[n]);
+    for (int n = 0; n < vecN; ++n) fprintf(fp, "vector per node:     %3s  %s.%s\n", vec[n], bname, vec[n]);
+    fprintf(fp, "\n");
+    fclose(fp);
+    snprintf(name, sizeof name, "%s.case", rname);
+    if (!(fp = OpenFile(name, "r+"))) return 1;
+    {
+        char str[256];
+        while (fgets(str, sizeof str, fp)) {
+            artra_parse_line(str);
+            if (0 == strcmp(str, "time set: 1")) {
+                fseek(fp, 0, SEEK_CUR);
+                break;
+            }
+        }
+    }
+    fprintf(fp, "number of steps:          %d", time->dataC + 1);
+    fseek(fp, 0, SEEK_END);
+    if (0 == time->dataC % 5) fprintf(fp, "\n");
+    fprintf(fp, "%.6g ", time->now);
+    fclose(fp);
+    return 0;
+}
+
+/* WriteStructuredDataEnsight (ensight_writer.c:34-246): the output range as
+ * one structured block; the geometry file at the first output only. */
+static int EnStructured(ArtraHipSolver *s, const Time *time)
+{
+    static const char *sca[7] = {"rho", "u", "v", "w", "p", "T", "did"};
+    static const char *vec[1] = {"Vel"};
+    const Partition *part = &s->space->part;
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "field", time->dataC);
+    EnFile f;
+    if (0 == time->stepC) {
+        if (EnCaseFiles(time, "field", bname, "", "*****", sca, 7, vec, 1)) return 1;
+        snprintf(name, sizeof name, "%s.geo", "field");
+        if (EnOpen(&f, name)) return 1;
+        EnHeader(&f);
+        EnCopy(&f, "part");
+        EnInt(&f, 1);
+        EnPrint(&f, "part %d", PIO, 0, 0, 1);
+        EnCopy(&f, "block");
+        int ne[DIMS];
+        for (int d = 0; d < DIMS; ++d) ne[d] = part->ns[PIO][d][MAX] - part->ns[PIO][d][MIN];
+        for (int d = 0; d < DIMS; ++d) EnInt(&f, ne[d]);
+        for (int d = 0; d < DIMS; ++d) {
+            for (int k = part->ns[PIO][Z][MIN]; k < part->ns[PIO][Z][MAX]; ++k)
+                for (int j = part->ns[PIO][Y][MIN]; j < part->ns[PIO][Y][MAX]; ++j)
+                    for (int i = part->ns[PIO][X][MIN]; i < part->ns[PIO][X][MAX]; ++i) {
+                        const int ijk[DIMS] = {i, j, k};
+                        EnReal(&f, artra_map_point(ijk[d], part->domain[d][MIN], part->d[d], part->ng[d]));
+                    }
+        }
+        if (EnClose(&f, name)) return 1;
+    } else if (EnCaseFiles(time, "field", bname, "", "*****", sca, 7, vec, 1)) {
+        return 1;
+    }
+    if (artra_dev_download(s->dev, TO, s->staging)) return artra_dev_failed();
+    for (int q = 0; q < 8; ++q) {
+        snprintf(name, sizeof name, "%s.%s", bname, (q < 7) ? sca[q] : vec[0]);
+        if (EnOpen(&f, name)) return 1;
+        EnCopy(&f, (q < 7) ? "scalar variable" : "vector variable");
+        EnCopy(&f, "part");
+        EnInt(&f, 1);
+        EnCopy(&f, "block");
+        for (int n = 0; n < ((q < 7) ? 1 : 3); ++n) {
+            for (int k = part->ns[PIO][Z][MIN]; k < part->ns[PIO][Z][MAX]; ++k)
+                for (int j = part->ns[PIO][Y][MIN]; j < part->ns[PIO][Y][MAX]; ++j)
+                    for (int i = part->ns[PIO][X][MIN]; i < part->ns[PIO][X][MAX]; ++i) {
+                        double v[3];
+                        NodeValues(s, s->staging, k, j, i, (q < 7) ? q : PV_VEL, v);
+                        EnReal(&f, (q < 7) ? v[0] : v[n]);
+                    }
+        }
+        if (EnClose(&f, name)) return 1;
+    }
+    return 0;
+}
+
+static int EnPolyState(const char *bname, int pm, int pn, const Geometry *geo)
+{
+    char name[128];
+    snprintf(name, sizeof name, "%s.state", bname);
+    FILE *fp = OpenFile(name, "w");
+    if (!fp) return 1;
+    PolyState(fp, pm, pn, geo);
+    fclose(fp);
+    return 0;
+}
+
+/* WritePolyDataEnsight for the analytical spheres (ensight_writer.c:258-344,
+ * 396-442). */
+static int EnSpheres(const Time *time, const Geometry *geo)
+{
+    static const char *sca[2] = {"r", "did"};
+    static const char *vec[1] = {"Vel"};
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "geo_sph", time->dataC);
+    if (EnCaseFiles(time, "geo_sph", bname, "*****", "*****", sca, 2, vec, 1)) return 1;
+    const int pm = 0, pn = geo->sphN;
+    EnFile f;
+    snprintf(name, sizeof name, "%s.geo", bname);
+    if (EnOpen(&f, name)) return 1;
+    EnHeader(&f);
+    EnCopy(&f, "part");
+    EnInt(&f, 1);
+    EnPrint(&f, "%d", pn - pm, 0, 0, 1);
+    EnCopy(&f, "coordinates");
+    EnInt(&f, pn - pm);
+    for (int d = 0; d < DIMS; ++d)
+        for (int n = pm; n < pn; ++n) EnReal(&f, geo->poly[n].O[d]);
+    EnCopy(&f, "point");
+    EnInt(&f, pn - pm);
+    for (int n = pm, m = 1; n < pn; ++n, ++m) EnInt(&f, m);
+    if (EnClose(&f, name)) return 1;
+    for (int q = 0; q < 3; ++q) {
+        snprintf(name, sizeof name, "%s.%s", bname, (q < 2) ? sca[q] : vec[0]);
+        if (EnOpen(&f, name)) return 1;
+        EnCopy(&f, (q < 2) ? "scalar variable" : "vector variable");
+        EnCopy(&f, "part");
+        EnInt(&f, 1);
+        EnCopy(&f, "coordinates");
+        if (q < 2) {
+            for (int n = pm; n < pn; ++n) EnReal(&f, (0 == q) ? geo->poly[n].r : (Real)(n + 1));
+        } else {
+            for (int d = 0; d < DIMS; ++d)
+                for (int n = pm; n < pn; ++n) EnReal(&f, geo->poly[n].V[TO][d]);
+        }
+        if (EnClose(&f, name)) return 1;
+    }
+    return EnPolyState(bname, pm, pn, geo);
+}
+
+/* WritePolyDataEnsight for the triangulated bodies (ensight_writer.c:345-395). */
+static int EnPolygons(const Time *time, const Geometry *geo)
+{
+    char bname[96], name[128];
+    snprintf(bname, sizeof bname, "%s%05d", "geo_stl", time->dataC);
+    if (EnCaseFiles(time, "geo_stl", bname, "*****", "*****", NULL, 0, NULL, 0)) return 1;
+    const int pm = geo->sphN, pn = geo->totN;
+    EnFile f;
+    snprintf(name, sizeof name, "%s.geo", bname);
+    if (EnOpen(&f, name)) return 1;
+    EnHeader(&f);
+    for (int p = pm, pnum = 1; p < pn; ++p, ++pnum) {
+        const Polyhedron *poly = geo->poly + p;
+        EnCopy(&f, "part");
+        EnInt(&f, pnum);
+        EnPrint(&f, "%d %d %d", poly->vertN, poly->edgeN, poly->faceN, 3);
+        EnCopy(&f, "coordinates");
+        EnInt(&f, poly->vertN);
+        for (int d = 0; d < DIMS; ++d)
+            for (int n = 0; n < poly->vertN; ++n) EnReal(&f, poly->v[n][d]);
+        EnCopy(&f, "tria3");
+        EnInt(&f, poly->faceN);
+        for (int n = 0; n < poly->faceN; ++n)
+            for (int q = 0; q < POLYN; ++q) EnInt(&f, poly->f[n][q] + 1);
+    }
+    if (EnClose(&f, name)) return 1;
+    return EnPolyState(bname, pm, pn, geo);
+}
+
+int artra_hip_write_space_data(ArtraHipSolver *s, const Time *time)
+{
+    const Geometry *geo = &s->space->geo;
+    if (0 == time->dataStreamer) {
+        if (PvStructured(s, time)) return 1;
+        if (0 != geo->totN) {
+            if (0 != geo->sphN && PvSpheres(time, geo)) return 1;
+            if (0 != geo->stlN && PvPolygons(time, geo)) return 1;
+        }
+    } else if (1 == time->dataStreamer) {
+        if (EnStructured(s, time)) return 1;
+        if (0 != geo->totN) {
+            if (0 != geo->sphN && EnSpheres(time, geo)) return 1;
+            if (0 != geo->stlN && EnPolygons(time, geo)) return 1;
+        }
+    } else {
+        return artra_set_error(2, "data streamer %d: 0 (ParaView) or 1 (EnSight)", time->dataStreamer);
+    }
+    return StateLog(time);
+}
+
+/* ----------------------------------------------------------- the solver loop */
+static int WriteDataKind(ArtraHipSolver *s, const Time *time, int n)
+{
+    return (PROSD == n) ? artra_hip_write_space_data(s, time) : artra_hip_write_probe_data(s, time, n);
+}
+
+/* Solve (solve.c:38-46) after the case's initialisation: the initial
+ * outputs of InitializeComputeDomain (initialization.c:51-55) unless the
+ * case restarts, then EvolveSolution (solve.c:48-107) with the time steps on
+ * the device and the outputs at the loop's cadence.  The device march stops
+ * at the first step where an output falls due, so every output is written
+ * from exactly the state the reference writes it from. */
+int artra_hip_solve(ArtraHipSolver *s, Time *time)
+{
+    if (0 == time->restart) {
+        if (WriteDataKind(s, time, PROPT) || WriteDataKind(s, time, PROFC) || WriteDataKind(s, time, PROSD)) return 1;
+    }
+    if (0.0 >= time->end - time->now) return 0;
+    Real dtData[NPROBE], rcData[NPROBE];
+    for (int n = 0; n < NPROBE; ++n) {
+        dtData[n] = time->end / (Real)(time->dataW[n]);
+        rcData[n] = 0.0;
+    }
+    const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : dtData[PROSD];
+    Real rcInt = 0.0, lastDt = 0.0;
+    double floor[DIMS];
+    {
+        const Geometry *geo = &s->space->geo;
+        floor[X] = floor[Y] = floor[Z] = 0.0;
+        for (int n = 0; n < geo->totN; ++n) {
+            const Polyhedron *p = geo->poly + n;
+            const double V[DIMS] = {fabs(p->V[TO][X]) + fmax(fabs(p->W[TO][Y]), fabs(p->W[TO][Z])) * p->r,
+                                    fabs(p->V[TO][Y]) + fmax(fabs(p->W[TO][Z]), fabs(p->W[TO][X])) * p->r,
+                                    fabs(p->V[TO][Z]) + fmax(fabs(p->W[TO][X]), fabs(p->W[TO][Y])) * p->r};
+            for (int d = 0; d < DIMS; ++d) {
+                if (floor[d] < V[d]) floor[d] = V[d];
+            }
+        }
+    }
+    double rec[ARTRA_DEV_MARCH_CHUNK];
+    while (time->now < time->end && time->stepC < time->stepN) {
+        /* steps up to the next output on the last step's dt (the march
+         * stops by itself at the output; this only bounds the steps that
+         * would be enqueued past it) */
+        int k = ARTRA_DEV_MARCH_CHUNK;
+        if (time->stepN - time->stepC < k) k = time->stepN - time->stepC;
+        if (lastDt > 0.0) {
+            double ahead = (time->end - time->now) / lastDt;
+            for (int n = 0; n < NPROBE; ++n) {
+                const double to = (dtData[n] - rcData[n]) / lastDt;
+                if (to < ahead) ahead = to;
+            }
+            if (ahead + 1.0 < (double)k) k = (ahead >= 0.0) ? (int)ahead + 1 : 1;
+        } else {
+            k = 1;
+        }
+        double clock[ARTRA_DEV_MARCH_CLOCK] = {time->now, time->end, rcInt, tmInt, (double)time->stepC,
+                                               (double)time->stepN, 0.0};
+        for (int n = 0; n < NPROBE; ++n) {
+            clock[7 + n] = rcData[n];
+            clock[12 + n] = dtData[n];
+        }
+        int adv = 0;
+        if (artra_dev_march_steps(s->dev, clock, floor, time->numCFL, k, rec, &adv)) return artra_dev_failed();
+        time->now = clock[0];
+        rcInt = clock[2];
+        time->stepC += adv;
+        for (int n = 0; n < NPROBE; ++n) rcData[n] = clock[7 + n];
+        if (adv > 0) lastDt = rec[adv - 1];
+        if (0.0 != clock[6]) {
+            /* the outputs due after this step (solve.c:91-104) */
+            for (int n = 0; n < NPROBE; ++n) {
+                if (rcData[n] >= dtData[n] || time->now == time->end || time->stepC == time->stepN) {
+                    if (PROFC == n && artra_hip_surface_force(s)) return 1;
+                    if (PROSD == n) ++time->dataC;
+                    if (WriteDataKind(s, time, n)) return 1;
+                    rcData[n] = 0.0;
+                }
+            }
+        } else if (0 == adv) {
+            break;
+        }
+    }
+    s->rcInt = rcInt;
+    return artra_hip_synchronize(s);
+}
+
+/* ------------------------------------------------------------- FP64 restart */
+/* Restart file: "ARTRARS1", int32 n[3], ng[3], stepC, dataC, double now,
+ * rcInt, then level TO of every node (ghosts included) as FP64
+ * component-major [5][N].  Unlike the reference's restart from its
+ * "%.6g" field text (paraview_reader.c), a run continued from it is the
+ * uninterrupted run bit for bit. */
+static const char kRestartMagic[8] = {'A', 'R', 'T', 'R', 'A', 'R', 'S', '1'};
+
+int artra_hip_write_restart(ArtraHipSolver *s, const Time *time, const char *path)
+{
+    const Partition *part = &s->space->part;
+    if (artra_dev_download(s->dev, TO, s->staging)) return artra_dev_failed();
+    FILE *fp = OpenFile(path, "wb");
+    if (!fp) return 1;
+    const int hdr[8] = {part->n[X], part->n[Y], part->n[Z], part->ng[X], part->ng[Y], part->ng[Z], time->stepC,
+                        time->dataC};
+    const double clk[2] = {time->now, s->rcInt};
+    int ok = fwrite(kRestartMagic, 1, 8, fp) == 8 && fwrite(hdr, sizeof hdr, 1, fp) == 1 &&
+             fwrite(clk, sizeof clk, 1, fp) == 1 &&
+             fwrite(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N;
+    ok = (0 == fclose(fp)) && ok;
+    return ok ? 0 : artra_set_error(1, "writing the restart file %s failed", path);
+}
+
+int artra_hip_read_restart(ArtraHipSolver *s, Time *time, const char *path)
+{
+    const Partition *part = &s->space->part;
+    FILE *fp = OpenFile(path, "rb");
+    if (!fp) return 1;
+    char magic[8];
+    int hdr[8];
+    double clk[2];
+    int ok = fread(magic, 1, 8, fp) == 8 && 0 == memcmp(magic, kRestartMagic, 8) && fread(hdr, sizeof hdr, 1, fp) == 1 &&
+             fread(clk, sizeof clk, 1, fp) == 1;
+    for (int d = 0; ok && d < DIMS; ++d) ok = hdr[d] == part->n[d] && hdr[DIMS + d] == part->ng[d];
+    ok = ok && fread(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N;
+    fclose(fp);
+    if (!ok) return artra_set_error(1, "%s is not a restart file of this grid", path);
+    if (artra_dev_upload(s->dev, TO, s->staging)) return artra_dev_failed();
+    Node *node = s->space->node;
+    for (long q = 0; q < s->N; ++q) {
+        for (int c = 0; c < DIMU; ++c) node[q].U[TO][c] = s->staging[c * s->N + q];
+    }
+    time->stepC = hdr[6];
+    time->dataC = hdr[7];
+    time->now = clk[0];
+    s->rcInt = clk[1];
+    return 0;
 }

@@ -58,6 +58,10 @@ def library():
         "artra_hip_march_noalloc": (ctypes.c_int, [vp, vp, ctypes.c_int, ip, dp]),
         "artra_hip_surface_force": (ctypes.c_int, [vp]),
         "artra_hip_write_probe_data": (ctypes.c_int, [vp, vp, ctypes.c_int]),
+        "artra_hip_write_space_data": (ctypes.c_int, [vp, vp]),
+        "artra_hip_solve": (ctypes.c_int, [vp, vp]),
+        "artra_hip_write_restart": (ctypes.c_int, [vp, vp, ctypes.c_char_p]),
+        "artra_hip_read_restart": (ctypes.c_int, [vp, vp, ctypes.c_char_p]),
         "artra_hip_download": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_upload": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_read_field": (ctypes.c_int, [vp, ctypes.c_int, dp]),
@@ -83,6 +87,7 @@ def library():
         "artra_case_set_field": (ctypes.c_int, [vp, ctypes.c_int, dp]),
         "artra_case_get_flags": (ctypes.c_int, [vp, ip]),
         "artra_case_clock": (ctypes.c_int, [vp, dp, ip]),
+        "artra_case_forces": (ctypes.c_int, [vp, dp]),
         "artra_calc_expression": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, dp]),
         "artra_hip_face_count": (ctypes.c_int, [vp, ctypes.c_int, lp]),
@@ -222,6 +227,13 @@ class Case:
         library().artra_case_clock(self._h, ctypes.byref(now), ctypes.byref(step))
         return now.value, step.value
 
+    def forces(self):
+        """Surface force of every body as [bodies][9] = Fp, Fv, Tt (the last
+        integration's values)."""
+        out = np.zeros((64, 9))
+        n = library().artra_case_forces(self._h, _dptr(out))
+        return out[:n].copy()
+
     def interior(self, arr):
         """Restrict a [..., nz, ny, nx] array to the interior range."""
         (x0, x1), (y0, y1), (z0, z1) = self.pin
@@ -299,6 +311,32 @@ class Solver:
 
     def synchronize(self):
         _check(library().artra_hip_synchronize(self._h))
+
+    def surface_force(self):
+        """IntegrateSurfaceForce on the device; the case's bodies hold the result."""
+        _check(library().artra_hip_surface_force(self._h))
+
+    def write_probe_data(self, kind):
+        """The probe files of one kind (0 point, 1 line, 2 curve, 3 surface
+        force) in the current directory."""
+        _check(library().artra_hip_write_probe_data(self._h, self.case.time, kind))
+
+    def write_space_data(self):
+        """The field, geometry and log files of the output in the current directory."""
+        _check(library().artra_hip_write_space_data(self._h, self.case.time))
+
+    def write_restart(self, path):
+        """FP64 restart file of level TO and the clock."""
+        _check(library().artra_hip_write_restart(self._h, self.case.time, os.fsencode(path)))
+
+    def read_restart(self, path):
+        """Continue from an FP64 restart file (field, clock, output recorder)."""
+        _check(library().artra_hip_read_restart(self._h, self.case.time, os.fsencode(path)))
+
+    def solve(self):
+        """The solver loop with its output cadence, writing every output
+        file into the current directory."""
+        _check(library().artra_hip_solve(self._h, self.case.time))
 
     def set_timing(self, on=True):
         _check(library().artra_hip_set_timing(self._h, 1 if on else 0))

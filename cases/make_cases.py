@@ -14,7 +14,7 @@ import struct
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def case_text(space, time, numerical, material, reference, init, bcs, regions=()):
+def case_text(space, time, numerical, material, reference, init, bcs, regions=(), probes=None):
     """Assemble an artracfd.case file.
 
     space: ((x0,y0,z0),(x1,y1,z1),(mx,my,mz)); time: (restart,end,cfl,stepN,dataW,streamer)
@@ -22,6 +22,9 @@ def case_text(space, time, numerical, material, reference, init, bcs, regions=()
     material: (mid, refMu, gState, (gx,gy,gz)); reference: (L, rho, V, T)
     init: five expressions; bcs: six entries (W,E,S,N,F,B), each a list of lines
     regions: (kind, lines) for plane/sphere/box/cylinder initializers
+    probes: (points, lines, curve, force, control) with points [(x,y,z)],
+      lines [((x0,y0,z0),(x1,y1,z1),samples)], curve/force counts (on when > 0)
+      and control the four writing frequencies (case_loader.c:185-222)
     """
     out = ["# case written by cases/make_cases.py"]
     (x0, y0, z0), (x1, y1, z1), (mx, my, mz) = space
@@ -37,31 +40,47 @@ def case_text(space, time, numerical, material, reference, init, bcs, regions=()
         out += [f"{name} boundary begin"] + list(lines) + [f"{name} boundary end"]
     for kind, lines in regions:
         out += [f"{kind} initialization begin"] + list(lines) + [f"{kind} initialization end"]
+    if probes is not None:
+        points, plines, curve, force, control = probes
+        out += ["probe count begin", str(len(points)), str(len(plines)), str(curve), str(force), "probe count end"]
+        out += ["probe control begin"] + [str(v) for v in control] + ["probe control end"]
+        if points:
+            out += ["point probe begin"] + [f"{x!r}, {y!r}, {z!r}" for (x, y, z) in points] + ["point probe end"]
+        if plines:
+            out.append("line probe begin")
+            for (a, b, m) in plines:
+                out += [f"{a[0]!r}, {a[1]!r}, {a[2]!r}", f"{b[0]!r}, {b[1]!r}, {b[2]!r}", str(m)]
+            out.append("line probe end")
     return "\n".join(out) + "\n"
 
 
 # Temperatures are nondimensional: with reference values (1, 1, 1, 1) the gas
 # constant is 1/(1.4 Ma^2) ~ 287, so T = p / (rho R) ~ 0.0035 for p ~ rho ~ 1.
-STATE_STATIONARY = ("0, 0, 0, {r}, 0, 0, 0, 0, 0, 0, 1e37, {T}, {cf}, 0, 0, 0",
+STATE_STATIONARY = ("0, 0, 0, {r}, 0, 0, 0, 0, 0, 0, {rho}, {T}, {cf}, 0, 0, 0",
                     "0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0")
 
 
 def geo_text(spheres=(), stls=()):
-    """spheres: (x, y, z, r, T, cf); stls: (file, r_hint, T, cf)."""
+    """spheres: (x, y, z, r, T, cf[, rho]); stls: (file, r_hint, T, cf[, rho]).  A body density
+    above 1e36 marks a stationary body without surface-force integration; a finite one (the body
+    still does not move while the phase interaction is off) has its surface force integrated
+    (initialization.c:235-256)."""
     out = ["count begin", str(len(spheres)), str(len(stls)), "count end"]
     if spheres:
         out.append("sphere state begin")
-        for (x, y, z, r, T, cf) in spheres:
-            out.append(f"{x}, {y}, {z}, {r}, 0, 0, 0, 0, 0, 0, 1e37, {T}, {cf}, 0, 0, 0")
+        for sp in spheres:
+            (x, y, z, r, T, cf), rho = sp[:6], (sp[6] if len(sp) > 6 else "1e37")
+            out.append(f"{x}, {y}, {z}, {r}, 0, 0, 0, 0, 0, 0, {rho}, {T}, {cf}, 0, 0, 0")
             out.append(STATE_STATIONARY[1])
         out.append("sphere state end")
     if stls:
         out.append("polyhedron geometry begin")
-        out += [f for (f, _, _, _) in stls]
+        out += [st[0] for st in stls]
         out.append("polyhedron geometry end")
         out.append("polyhedron state begin")
-        for (_, r, T, cf) in stls:
-            out.append(STATE_STATIONARY[0].format(r=r, T=T, cf=cf))
+        for st in stls:
+            (_, r, T, cf), rho = st[:4], (st[4] if len(st) > 4 else "1e37")
+            out.append(STATE_STATIONARY[0].format(r=r, T=T, cf=cf, rho=rho))
             out.append(STATE_STATIONARY[1])
         out.append("polyhedron state end")
         out.append("polyhedron transform begin")
@@ -178,6 +197,19 @@ def cases():
                                      [plane((shock_x, 0, 0), (-1, 0, 0), CYL_POST)]),
                       geo=geo_text(stls=[("cylinder.stl", 0.1, -1, 0)]),
                       stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
+    # 3b. the same with every probe kind of the solver (data_probe.c:22-197): two points, two lines,
+    # the body-conformal curve and the surface force, written every 10th of the run to t = 0.04, and the
+    # field written twice
+    c["cyl64_probes"] = dict(case=case_text(((0.5, 0.5, 0), (1.5, 1.5, 0.016), (64, 64, 4)), (0, 0.04, 0.6, 0, 2, 0),
+                                            DEFAULT_NUM, INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
+                                            [inflow(*CYL_POST), OUTFLOW, slip(), slip(), PERIODIC, PERIODIC],
+                                            [plane((0.8, 0, 0), (-1, 0, 0), CYL_POST)],
+                                            probes=([(0.62, 1.0, 0.008), (1.3, 1.17, 0.0)],
+                                                    [((0.55, 1.0, 0.008), (1.45, 1.0, 0.008), 40),
+                                                     ((1.0, 0.5, 0.0), (1.0, 1.5, 0.016), 7)],
+                                                    1, 1, (10, 10, 10, 10))),
+                             geo=geo_text(stls=[("cylinder.stl", 0.1, -1, 0, 2.7)]),
+                             stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
     # 4. viscous flow past a no-slip isothermal sphere (analytical body), every IB layer reconstructed
     c["sphere20"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (20, 20, 20)),
                                         (0, 1e9, 0.6, 20, 1, 0), (1, 1, 0, 0, 0, 0, 0),
@@ -186,6 +218,31 @@ def cases():
                                         [inflow(1, 0.3, 0, 0, 1), OUTFLOW, noslip(0.004), slip(), OUTFLOW,
                                          OUTFLOW]),
                          geo=geo_text(spheres=[(0.05, 0.0, 0.0, 0.2, 0.0038, 1)]))
+    # 4b. the same with probes: the surface force then has its viscous (no-slip) part
+    c["sphere20_probes"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (20, 20, 20)),
+                                               (0, 0.15, 0.6, 0, 2, 0), (1, 1, 0, 0, 0, 0, 0),
+                                               (0, 0.05, 0, (0, 0, 0)), (1, 1, 1, 1),
+                                               ("1", "0.3", "0.1*sin(3*y)", "0", "1+0.1*cos(4*z)"),
+                                               [inflow(1, 0.3, 0, 0, 1), OUTFLOW, noslip(0.004), slip(), OUTFLOW,
+                                                OUTFLOW],
+                                               probes=([(0.3, 0.1, -0.05)], [((-0.5, 0.0, 0.0), (0.5, 0.0, 0.0), 30)],
+                                                       1, 1, (5, 5, 5, 5))),
+                                geo=geo_text(spheres=[(0.05, 0.0, 0.0, 0.2, 0.0038, 1, 2.7)]))
+    # 4c. the cylinder and the sphere again with the EnSight Gold writer (ensight_writer.c:34-452): binary
+    # field, geometry and body files and the transient case files
+    c["cyl64_ensight"] = dict(case=case_text(((0.5, 0.5, 0), (1.5, 1.5, 0.016), (64, 64, 4)), (0, 0.04, 0.6, 0, 2, 1),
+                                             DEFAULT_NUM, INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
+                                             [inflow(*CYL_POST), OUTFLOW, slip(), slip(), PERIODIC, PERIODIC],
+                                             [plane((0.8, 0, 0), (-1, 0, 0), CYL_POST)]),
+                              geo=geo_text(stls=[("cylinder.stl", 0.1, -1, 0, 2.7)]),
+                              stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
+    c["sphere20_ensight"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (20, 20, 20)),
+                                                (0, 0.15, 0.6, 0, 6, 1), (1, 1, 0, 0, 0, 0, 0),
+                                                (0, 0.05, 0, (0, 0, 0)), (1, 1, 1, 1),
+                                                ("1", "0.3", "0.1*sin(3*y)", "0", "1+0.1*cos(4*z)"),
+                                                [inflow(1, 0.3, 0, 0, 1), OUTFLOW, noslip(0.004), slip(), OUTFLOW,
+                                                 OUTFLOW]),
+                                 geo=geo_text(spheres=[(0.05, 0.0, 0.0, 0.2, 0.0038, 1, 2.7)]))
     # 5. scheme variants on a small 2D Riemann-like problem (10 steps each)
     quad = [box((-1, -1, -1), (0.5, 0.5, 1), (0.5, 0.1, -0.2, 0, 0.4)),
             sphere((0.7, 0.3, 0.008), 0.15, (2.0, 0, 0, 0, 3.0))]

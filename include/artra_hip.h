@@ -112,6 +112,30 @@ int artra_hip_surface_force(ArtraHipSolver *solver);
  * surface-force file holds the values of the last artra_hip_surface_force. */
 int artra_hip_write_probe_data(ArtraHipSolver *solver, const Time *time, int kind);
 
+/* WriteData(PROSD) (data_stream.c:76-125): the field file of the output
+ * range, the bodies' geometry files and artracfd.log, in the current
+ * directory, from one download of level TO, as the reference's writer of
+ * the case's data streamer makes them: 0, ParaView (paraview_writer.c:
+ * 31-398: the same names, collection files, layout and "%.6g" text,
+ * formatted in parallel); 1, EnSight Gold (ensight_writer.c:34-452: the
+ * transient and per-output case files, C-binary FP32 geometry and variable
+ * files, body state files).  Another streamer is refused with code 2. */
+int artra_hip_write_space_data(ArtraHipSolver *solver, const Time *time);
+
+/* Solve (solve.c:38-107) on the device after the case's initialisation: the
+ * initial outputs unless the case restarts, then the time loop with its
+ * output cadence (probes, surface force, field files) written exactly as
+ * the reference writes them, from the device-resident field. */
+int artra_hip_solve(ArtraHipSolver *solver, Time *time);
+
+/* FP64 restart: level TO of every node (ghosts included), the clock
+ * (now, stepC, dataC) and the output-instant recorder, exactly; a run
+ * continued from it is the uninterrupted run bit for bit (the reference
+ * restarts from its "%.6g" field text, paraview_reader.c).  Reading
+ * checks the grid size, uploads the field and sets time and solver. */
+int artra_hip_write_restart(ArtraHipSolver *solver, const Time *time, const char *path);
+int artra_hip_read_restart(ArtraHipSolver *solver, Time *time, const char *path);
+
 /* Copy a time level between the device and the node array of space. */
 int artra_hip_download(ArtraHipSolver *solver, int tn);
 int artra_hip_upload(ArtraHipSolver *solver, int tn);
@@ -285,6 +309,9 @@ int artra_case_set_field(ArtraCase *c, int tn, const Real *field);
 int artra_case_get_flags(const ArtraCase *c, int *flags);
 /* current time and step count of the case's clock */
 int artra_case_clock(const ArtraCase *c, double *now, int *step);
+/* Surface force of every body as the last integration left it:
+ * forces[n][9] = Fp[3], Fv[3], Tt[3]; returns the number of bodies. */
+int artra_case_forces(const ArtraCase *c, Real *forces);
 /* Evaluate an initial-condition expression (t = 0); returns 0 on success. */
 int artra_calc_expression(const char *text, Real x, Real y, Real z, Real *ans);
 

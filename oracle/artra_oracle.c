@@ -20,6 +20,7 @@
  *   oracle_treat_boundary         boundary_treatment.c:28-192
  *   immersed-boundary fill        immersed_boundary.c:270-487,
  *                                 computational_geometry.c:449-783
+ *   oracle_integrate_surface_force solid_dynamics.c:50-171
  *
  * Pinning: every function is compared with the compiled reference
  * (tests/golden fixtures made by oracle/ref_harness.c) and with closed-form
@@ -910,4 +911,86 @@ int oracle_march(Time *time, Space *space, const Model *model, int max_steps, do
         ++done;
     }
     return done;
+}
+
+/* ------------------------------------------------------ surface force */
+/* IntegrateSurfaceForce (solid_dynamics.c:50-171) on level TO: per body with
+ * state <= 0, the pressure force, the wall shear of a no-slip body in a
+ * viscous flow and their torque about the body centre, summed over the
+ * body's layer-2 ghost nodes in (k, j, i) order inside its bounding box,
+ * scaled by -area / (interfacial layer-2 nodes); an equilibrium pressure
+ * (variance below FLT_EPSILON^2 of the squared mean) gives zero. */
+void oracle_integrate_surface_force(Space *space, const Model *model)
+{
+    const Partition *part = &space->part;
+    const Node *node = space->node;
+    Geometry *geo = &space->geo;
+    const double percent = FLT_EPSILON * FLT_EPSILON;
+    for (int n = 0; n < geo->totN; ++n) {
+        Polyhedron *poly = geo->poly + n;
+        if (0 < poly->state) continue;
+        for (int s = 0; s < DIMS; ++s) poly->Fp[s] = poly->Fv[s] = poly->Tt[s] = 0.0;
+        double fvar[3] = {0.0, 0.0, 0.0};
+        int lidN = 0, gstN = 0;
+        int box[DIMS][LIMIT];
+        for (int s = 0; s < DIMS; ++s) {
+            const int lo = part->ns[PIN][s][MIN], hi = part->ns[PIN][s][MAX];
+            for (int e = 0; e < LIMIT; ++e) {
+                int q = MapNodeOf(poly->box[s][e], part->domain[s][MIN], part->dd[s], part->ng[s]);
+                q = (q < lo) ? lo : q;
+                q = (q > hi - 1) ? hi - 1 : q;
+                box[s][e] = q + e;
+            }
+        }
+        for (int k = box[Z][MIN]; k < box[Z][MAX]; ++k) {
+            for (int j = box[Y][MIN]; j < box[Y][MAX]; ++j) {
+                for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
+                    const Node *nd = node + IDX(k, j, i, part->n[Y], part->n[X]);
+                    if (2 == nd->lid && n + 1 == nd->did) ++lidN;
+                    if (2 != nd->gst || n + 1 != nd->did) continue;
+                    ++gstN;
+                    const double pG[3] = {MapPointOf(i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                                          MapPointOf(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                                          MapPointOf(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+                    double pO[3], pI[3], N[3], Uo[DIMUo], Fp[3], Fv[3], Fs[3], Tt[3];
+                    GeometricData(pG, nd->fid, poly, pO, pI, N);
+                    const double r[3] = {pO[0] - poly->O[0], pO[1] - poly->O[1], pO[2] - poly->O[2]};
+                    MapPrimitive(model->gamma, model->gasR, nd->U[TO], Uo);
+                    for (int s = 0; s < 3; ++s) Fp[s] = Uo[4] * N[s];
+                    if (1 == gstN) fvar[0] = Uo[4];
+                    fvar[1] = fvar[1] + Uo[4] - fvar[0];
+                    fvar[2] = fvar[2] + (Uo[4] - fvar[0]) * (Uo[4] - fvar[0]);
+                    if (0.0 < model->refMu && 0.0 < poly->cf) {
+                        const double Ts = Uo[5] * model->refT;
+                        const double mu = model->refMu * (1.458e-6 * pow(Ts, 1.5) / (Ts + 110.4));
+                        double V[3];
+                        Cross3(poly->W[TO], r, V);
+                        for (int s = 0; s < 3; ++s) V[s] = Uo[1 + s] - (poly->V[TO][s] + V[s]);
+                        const double Vn = Dot3(V, N);
+                        const double dist = sqrt(Dist2(pG, pO));
+                        for (int s = 0; s < 3; ++s) Fv[s] = mu * (V[s] - Vn * N[s]) / dist;
+                    } else {
+                        Fv[0] = Fv[1] = Fv[2] = 0.0;
+                    }
+                    for (int s = 0; s < 3; ++s) Fs[s] = Fp[s] + Fv[s];
+                    Cross3(r, Fs, Tt);
+                    for (int s = 0; s < 3; ++s) {
+                        poly->Fp[s] = poly->Fp[s] + Fp[s];
+                        poly->Fv[s] = poly->Fv[s] + Fv[s];
+                        poly->Tt[s] = poly->Tt[s] + Tt[s];
+                    }
+                }
+            }
+        }
+        if (0 == lidN || 0 == gstN) continue;
+        double ds = poly->area / lidN;
+        fvar[2] = (fvar[2] - fvar[1] * fvar[1] / gstN) / gstN;
+        fvar[1] = fvar[1] / gstN + fvar[0];
+        if (percent * fvar[1] * fvar[1] > fvar[2]) ds = 0.0;
+        for (int s = 0; s < 3; ++s) {
+            poly->Fp[s] = -poly->Fp[s] * ds;
+            poly->Fv[s] = -poly->Fv[s] * ds;
+            poly->Tt[s] = -poly->Tt[s] * ds;
+        }
+    }
 }

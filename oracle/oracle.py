@@ -48,6 +48,7 @@ def library():
         lib.oracle_march.restype = ctypes.c_int
         lib.oracle_map_primitive.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
         lib.oracle_map_conservative.argtypes = [ctypes.c_double, dp, dp]
+        lib.oracle_integrate_surface_force.argtypes = [vp, vp]
         lib.oracle_point_triangle_distance.argtypes = [dp] * 5
         lib.oracle_point_triangle_distance.restype = ctypes.c_double
         _lib = lib
@@ -87,6 +88,13 @@ def evolve(case, dt):
 
 def compute_dt(case):
     return library().oracle_compute_time_step(case.time, case.space, case.model)
+
+
+def surface_force(case):
+    """IntegrateSurfaceForce on the case's level TO; returns [bodies][9]
+    (Fp, Fv, Tt) as the case's bodies then hold them."""
+    library().oracle_integrate_surface_force(case.space, case.model)
+    return case.forces()
 
 
 def march(case, max_steps=2 ** 31 - 1):

@@ -11,7 +11,9 @@
  *     EvolveFluidDynamics (fluid_dynamics.c:55-116) under the dt rectification
  *     of EvolveSolution (solve.c:65-78), without the field output,
  *   - writes full-precision FP64 dumps of the padded node array, and
- *   - writes per-function input/output vectors for the flux building blocks.
+ *   - writes per-function input/output vectors for the flux building blocks,
+ *   - or (mode "solve") runs the reference's whole Solve with all of its
+ *     output files, for the probe, surface-force and writer fixtures.
  *
  * Built with -DARTRA_DROPIN it is instead linked against the reference
  * objects minus the ones libartra_hip replaces (see oracle/Makefile,
@@ -32,6 +34,7 @@
 
 #include "commons.h"
 #include "preprocess.h"
+#include "solve.h"
 #include "initialization.h"
 #include "fluid_dynamics.h"
 #include "boundary_treatment.h"
@@ -160,6 +163,47 @@ static int RunCase(int argc, char **argv)
                 fluid += (0 == space.node[IndexNode(k, j, i, part->n[Y], part->n[X])].did);
     printf("{\"steps\": %d, \"now\": %.17g, \"seconds\": %.6f, \"fluid_nodes\": %ld}\n",
            steps, time.now, elapsed, fluid);
+    return 0;
+}
+
+/*
+ * The reference's whole solver on the case in the working directory:
+ * Preprocess and Solve (solve.c:38-46), i.e. initialisation and
+ * EvolveSolution with every output it makes (probes, surface force, field
+ * and geometry files, artracfd.log) written into the working directory.
+ * Afterwards the final state and each body's surface force (Fp, Fv, Tt: 9
+ * doubles per body, as the last IntegrateSurfaceForce of the loop left
+ * them) are dumped.
+ */
+static int SolveCase(int argc, char **argv)
+{
+    if (argc < 3) Die("usage: solve <out_prefix>");
+    const char *prefix = argv[2];
+    char fname[1024];
+    Time time = {0};
+    Space space = {0};
+    Model model = {0};
+    space.part.proc[X] = space.part.proc[Y] = space.part.proc[Z] = 1;
+    space.part.procN = 1;
+    FILE *out = stdout;
+    stdout = fopen("/dev/null", "w");
+    Preprocess(&time, &space, &model);
+    Solve(&time, &space, &model);
+    snprintf(fname, sizeof fname, "%s_final.bin", prefix);
+    DumpState(fname, &space, time.stepC, time.now, 0.0);
+    snprintf(fname, sizeof fname, "%s_force.bin", prefix);
+    FILE *fp = fopen(fname, "wb");
+    if (!fp) Die("cannot open force file");
+    for (int n = 0; n < space.geo.totN; ++n) {
+        const Polyhedron *poly = space.geo.poly + n;
+        fwrite(poly->Fp, sizeof(double), DIMS, fp);
+        fwrite(poly->Fv, sizeof(double), DIMS, fp);
+        fwrite(poly->Tt, sizeof(double), DIMS, fp);
+    }
+    fclose(fp);
+    fclose(stdout);
+    stdout = out;
+    printf("{\"steps\": %d, \"now\": %.17g, \"bodies\": %d}\n", time.stepC, time.now, space.geo.totN);
     return 0;
 }
 
@@ -329,8 +373,9 @@ static int Layout(void)
 
 int main(int argc, char **argv)
 {
-    if (argc < 2) Die("usage: artra_ref run|vectors|layout ...");
+    if (argc < 2) Die("usage: artra_ref run|solve|vectors|layout ...");
     if (0 == strcmp(argv[1], "run")) return RunCase(argc, argv);
+    if (0 == strcmp(argv[1], "solve")) return SolveCase(argc, argv);
 #ifndef ARTRA_DROPIN
     if (0 == strcmp(argv[1], "vectors")) return Vectors(argc, argv);
 #endif

@@ -12,6 +12,10 @@ compressed numpy archives:
   vectors.npz  WENO5/WENO3, face fluxes on a random grid, primitive maps,
                point-triangle distances
   abi_layout.txt  sizeof/offsetof of the drop-in structures (commons.h)
+  solve_<case>.npz  the reference's whole Solve on a case with probes: the
+               text of every probe and log file it writes, the SHA-256 and
+               size of its field and geometry files, the final surface
+               force of each body (Fp, Fv, Tt) and the final interior field
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -82,6 +86,48 @@ def make_vectors(work, seed=20261202, count=400):
     print("vectors", count, nf)
 
 
+SOLVE_CASES = ("cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight")
+SOLVE_TEXT = (".csv", ".log", ".pvd", ".case", ".state")   # stored as text; other outputs (.vts, .vtp, EnSight binaries) by hash
+
+
+def make_solve(name, work):
+    """Every output file of the reference's Solve on the case (run in a copy
+    of the case directory), with the inputs and the harness's dumps left
+    out."""
+    import hashlib
+    import json
+    import shutil
+    import subprocess
+    case_dir = os.path.join(ROOT, "cases", name)
+    d = os.path.join(work, "solve_" + name)
+    shutil.copytree(case_dir, d)
+    inputs = set(os.listdir(d))
+    out = subprocess.run([oracle.REF, "solve", "out"], cwd=d, check=True, capture_output=True, text=True)
+    info = json.loads(out.stdout.strip().splitlines()[-1])
+    names = sorted(f for f in os.listdir(d) if f not in inputs and not f.startswith("out_")
+                   and f != "artracfd.verify" and f != "geo_mass_property.csv")
+    texts, hashed = [], []
+    for f in names:
+        data = open(os.path.join(d, f), "rb").read()
+        (texts if f.endswith(SOLVE_TEXT) else hashed).append((f, data))
+    blob = b"".join(data for _, data in texts)
+    offsets = np.cumsum([0] + [len(data) for _, data in texts])
+    hf, ff, Uf = oracle.read_dump(os.path.join(d, "out_final.bin"))
+    pin = pin_of(case_dir)
+    np.savez_compressed(os.path.join(HERE, f"solve_{name}.npz"),
+                        text_names=np.array([f for f, _ in texts], dtype="S64"),
+                        text_offsets=offsets.astype(np.int64),
+                        text_blob=np.frombuffer(blob, dtype=np.uint8),
+                        hash_names=np.array([f for f, _ in hashed], dtype="S64"),
+                        hash_sha256=np.array([hashlib.sha256(data).hexdigest() for _, data in hashed], dtype="S64"),
+                        hash_sizes=np.array([len(data) for _, data in hashed], dtype=np.int64),
+                        hash_lines=np.array([data.count(b"\n") for _, data in hashed], dtype=np.int64),
+                        force=np.fromfile(os.path.join(d, "out_force.bin")).reshape(-1, 9),
+                        steps=info["steps"], now=info["now"],
+                        U_final=np.ascontiguousarray(interior(Uf, pin)))
+    print(name, info, len(texts), "text files", len(hashed), "hashed files")
+
+
 def make_layout():
     """sizeof/offsetof of the drop-in structures as the reference's own
     commons.h lays them out (the harness's layout dump)."""
@@ -102,11 +148,15 @@ def main(argv):
                 make_vectors(work)
             elif name == "layout":
                 make_layout()
+            elif name in SOLVE_CASES:
+                make_solve(name, work)
             else:
                 make_case(name, CASES[name], work)
         if not argv:
             make_vectors(work)
             make_layout()
+            for name in SOLVE_CASES:
+                make_solve(name, work)
 
 
 if __name__ == "__main__":
