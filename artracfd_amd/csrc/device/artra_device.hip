@@ -116,16 +116,30 @@ __device__ __forceinline__ int Index(const Grid &g, int i, int j, int k)
     return (k * g.ny + j) * g.nx + i;
 }
 
-__device__ __forceinline__ void LoadNode(const double *U, long N, int idx, double out[5])
+// The field is node-major: the five conserved components of a node are
+// contiguous (40 bytes), so a node is one or two cache lines whichever
+// direction a line runs.
+__device__ __forceinline__ double *NodeAt(double *U, long idx)
 {
+    return U + idx * 5;
+}
+
+__device__ __forceinline__ const double *NodeAt(const double *U, long idx)
+{
+    return U + idx * 5;
+}
+
+__device__ __forceinline__ void LoadNode(const double *U, long idx, double out[5])
+{
+    const double *p = NodeAt(U, idx);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) out[c] = U[c * N + idx];
+    for (int c = 0; c < 5; ++c) out[c] = p[c];
 }
 
 // Solution operator of one node (fluid_dynamics.c:243-274).  FAST evaluates
 // the same expression with fused multiply-adds.
 template <bool VISC, bool FAST>
-__device__ __forceinline__ void UpdateNode(const StageArgs &sa, long N, int idx, const double FL[5], const double FR[5],
+__device__ __forceinline__ void UpdateNode(const StageArgs &sa, long idx, const double FL[5], const double FR[5],
                                            const double FvL[5], const double FvR[5], const double Un[5])
 {
     double phi[5];
@@ -136,10 +150,9 @@ __device__ __forceinline__ void UpdateNode(const StageArgs &sa, long N, int idx,
     }
     if (sa.accumulate) {
         const double br = sa.b * sa.r;
+        double *um = NodeAt(sa.Utm, idx);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            sa.Utm[c * N + idx] = FAST ? fma(br, phi[c], sa.Utm[c * N + idx]) : sa.Utm[c * N + idx] + br * phi[c];
-        }
+        for (int c = 0; c < 5; ++c) um[c] = FAST ? fma(br, phi[c], um[c]) : um[c] + br * phi[c];
         return;
     }
     double Uo[5];
@@ -147,12 +160,13 @@ __device__ __forceinline__ void UpdateNode(const StageArgs &sa, long N, int idx,
 #pragma unroll
         for (int c = 0; c < 5; ++c) Uo[c] = Un[c];
     } else {
-        LoadNode(sa.Uto, N, idx, Uo);
+        LoadNode(sa.Uto, idx, Uo);
     }
+    double *um = NodeAt(sa.Utm, idx);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        if (FAST) sa.Utm[c * N + idx] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
-        else sa.Utm[c * N + idx] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
+        if (FAST) um[c] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
+        else um[c] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
     }
 }
 
@@ -179,14 +193,14 @@ __device__ __forceinline__ void LoadViscousNeighbours(const Grid &g, const doubl
     constexpr int TB = (S == 2) ? 1 : 2;
     const int stride[3] = {1, g.nx, g.nx * g.ny};
     const int e = stride[S], a = stride[TA], b = stride[TB];
-    LoadNode(U, g.N, idxP - a, nb[0]);
-    LoadNode(U, g.N, idxP + a, nb[1]);
-    LoadNode(U, g.N, idxP - b, nb[2]);
-    LoadNode(U, g.N, idxP + b, nb[3]);
-    LoadNode(U, g.N, idxP + e - a, nb[4]);
-    LoadNode(U, g.N, idxP + e + a, nb[5]);
-    LoadNode(U, g.N, idxP + e - b, nb[6]);
-    LoadNode(U, g.N, idxP + e + b, nb[7]);
+    LoadNode(U, idxP - a, nb[0]);
+    LoadNode(U, idxP + a, nb[1]);
+    LoadNode(U, idxP - b, nb[2]);
+    LoadNode(U, idxP + b, nb[3]);
+    LoadNode(U, idxP + e - a, nb[4]);
+    LoadNode(U, idxP + e + a, nb[5]);
+    LoadNode(U, idxP + e - b, nb[6]);
+    LoadNode(U, idxP + e + b, nb[7]);
 }
 
 // ------------------------------------------------------------------ sweeps
@@ -213,7 +227,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
         const int base = Index(g, i, j, k);
         double st[NS][5];
 #pragma unroll
-        for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, g.N, base + SL + m, st[m]);
+        for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, base + SL + m, st[m]);
         double F[5];
         ConvFlux<0, NS, FAST>(mdl, st, F);
 #pragma unroll
@@ -243,7 +257,7 @@ __global__ __launch_bounds__(kXBlock) void sweep_x(Grid g, StageArgs sa, FluxMod
                 FvL[c] = VISC ? sFv[VISC ? c : 0][threadIdx.x] : 0.0;
                 FvR[c] = VISC ? sFv[VISC ? c : 0][threadIdx.x + 1] : 0.0;
             }
-            UpdateNode<VISC, FAST>(sa, g.N, idx, FL, FR, FvL, FvR, Un);
+            UpdateNode<VISC, FAST>(sa, idx, FL, FR, FvL, FvR, Un);
         }
     }
 }
@@ -266,7 +280,7 @@ __global__ __launch_bounds__(kMarchX *kMarchT) __attribute__((amdgpu_waves_per_e
     // window for the face between s0 - 1 and s0
     double st[NS][5];
 #pragma unroll
-    for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, g.N, idx0 + (SL - 1 + m) * stride, st[m]);
+    for (int m = 0; m < NS; ++m) LoadNode(sa.Utn, idx0 + (SL - 1 + m) * stride, st[m]);
     double FL[5], FvL[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     ConvFlux<S, NS, FAST>(mdl, st, FL);
     if (VISC) {
@@ -281,7 +295,7 @@ __global__ __launch_bounds__(kMarchX *kMarchT) __attribute__((amdgpu_waves_per_e
 #pragma unroll
             for (int c = 0; c < 5; ++c) st[m][c] = st[m + 1][c];
         }
-        LoadNode(sa.Utn, g.N, idx + (SL + NS - 1) * stride, st[NS - 1]);
+        LoadNode(sa.Utn, idx + (SL + NS - 1) * stride, st[NS - 1]);
         double FR[5], FvR[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (VISC) {
             // the viscous flux first: its neighbour values are dead before
@@ -291,7 +305,7 @@ __global__ __launch_bounds__(kMarchX *kMarchT) __attribute__((amdgpu_waves_per_e
             ViscFlux<S, FAST>(mdl, sa.dd, st[K0], st[K0 + 1], nb, FvR);
         }
         ConvFlux<S, NS, FAST>(mdl, st, FR);
-        if (sa.fluid[idx]) UpdateNode<VISC, FAST>(sa, g.N, idx, FL, FR, FvL, FvR, st[K0]);
+        if (sa.fluid[idx]) UpdateNode<VISC, FAST>(sa, idx, FL, FR, FvL, FvR, st[K0]);
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             FL[c] = FR[c];
@@ -319,12 +333,12 @@ __global__ __launch_bounds__(256) void source_stage(Grid g, StageArgs sa, double
     const int idx = Index(g, i, j, k);
     if (!sa.fluid[idx]) return;
     double Un[5], Uo[5], phi[5];
-    LoadNode(sa.Utn, g.N, idx, Un);
+    LoadNode(sa.Utn, idx, Un);
     if (sa.toIsTn) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) Uo[c] = Un[c];
     } else {
-        LoadNode(sa.Uto, g.N, idx, Uo);
+        LoadNode(sa.Uto, idx, Uo);
     }
     const double V[3] = {Un[1] / Un[0], Un[2] / Un[0], Un[3] / Un[0]};
     const double fb[3] = {Un[0] * gx, Un[0] * gy, Un[0] * gz};
@@ -333,10 +347,11 @@ __global__ __launch_bounds__(256) void source_stage(Grid g, StageArgs sa, double
     phi[2] = fb[1];
     phi[3] = fb[2];
     phi[4] = FAST ? fma(fb[0], V[0], fma(fb[1], V[1], fb[2] * V[2])) : fb[0] * V[0] + fb[1] * V[1] + fb[2] * V[2];
+    double *um = NodeAt(sa.Utm, idx);
 #pragma unroll
     for (int c = 0; c < 5; ++c) {
-        if (FAST) sa.Utm[c * g.N + idx] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
-        else sa.Utm[c * g.N + idx] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
+        if (FAST) um[c] = fma(sa.a, Uo[c], sa.b * fma(sa.r, phi[c], Un[c]));
+        else um[c] = sa.a * Uo[c] + sa.b * (Un[c] + sa.r * phi[c]);
     }
 }
 
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(256) void sample_faces(Grid g, const double *U, Flu
     const int base = Index(g, c[0], c[1], c[2]);
     double st[6][5];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) LoadNode(U, g.N, base + (m - 2) * stride, st[m]);
+    for (int m = 0; m < 6; ++m) LoadNode(U, base + (m - 2) * stride, st[m]);
     float x[artra::kFeat];
     artra::face_features<6>(mdl.gamma, st, x);
 #pragma unroll
@@ -420,10 +435,11 @@ __device__ __forceinline__ void Conservative(double gamma, const double Uo[6], d
     U[4] = 0.5 * Uo[0] * (Uo[1] * Uo[1] + Uo[2] * Uo[2] + Uo[3] * Uo[3]) + Uo[4] / (gamma - 1.0);
 }
 
-__device__ __forceinline__ void StoreNode(double *U, long N, int idx, const double v[5])
+__device__ __forceinline__ void StoreNode(double *U, long idx, const double v[5])
 {
+    double *p = NodeAt(U, idx);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) U[c * N + idx] = v[c];
+    for (int c = 0; c < 5; ++c) p[c] = v[c];
 }
 
 __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
@@ -447,9 +463,9 @@ __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
     if (r != 0) {
         if (fc.type == 2 || fc.type == 3) {            // slip / no-slip wall: mirror about the wall
             double UO[5], UI[5], UoO[6], UoI[6], UoG[6];
-            LoadNode(ba.U, g.N, Index(g, i - r * N[0], j - r * N[1], k - r * N[2]), UO);
+            LoadNode(ba.U, Index(g, i - r * N[0], j - r * N[1], k - r * N[2]), UO);
             Primitive(ba.gamma, ba.gasR, UO, UoO);
-            LoadNode(ba.U, g.N, Index(g, i - 2 * r * N[0], j - 2 * r * N[1], k - 2 * r * N[2]), UI);
+            LoadNode(ba.U, Index(g, i - 2 * r * N[0], j - 2 * r * N[1], k - 2 * r * N[2]), UI);
             Primitive(ba.gamma, ba.gasR, UI, UoI);
             UoG[1] = UoO[1] + UoO[1] - UoI[1];
             UoG[2] = UoO[2] + UoO[2] - UoI[2];
@@ -459,11 +475,11 @@ __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
             UoG[0] = UoG[4] / (UoG[5] * ba.gasR);
             Conservative(ba.gamma, UoG, out);
         } else if (fc.type == 4) {                     // periodic: copy from one period inward
-            LoadNode(ba.U, g.N, Index(g, i - ba.m[0] * N[0], j - ba.m[1] * N[1], k - ba.m[2] * N[2]), out);
+            LoadNode(ba.U, Index(g, i - ba.m[0] * N[0], j - ba.m[1] * N[1], k - ba.m[2] * N[2]), out);
         } else {                                       // copy the inward neighbour
-            LoadNode(ba.U, g.N, Index(g, i - N[0], j - N[1], k - N[2]), out);
+            LoadNode(ba.U, Index(g, i - N[0], j - N[1], k - N[2]), out);
         }
-        StoreNode(ba.U, g.N, idx, out);
+        StoreNode(ba.U, idx, out);
         return;
     }
     switch (fc.type) {
@@ -471,12 +487,12 @@ __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
             Conservative(ba.gamma, fc.given, out);
             break;
         case 1:                                        // outflow: zero gradient
-            LoadNode(ba.U, g.N, Index(g, i - N[0], j - N[1], k - N[2]), out);
+            LoadNode(ba.U, Index(g, i - N[0], j - N[1], k - N[2]), out);
             break;
         case 2:
         case 3: {                                      // walls
             double Uh[5], Uoh[6], UoO[6];
-            LoadNode(ba.U, g.N, Index(g, i - N[0], j - N[1], k - N[2]), Uh);
+            LoadNode(ba.U, Index(g, i - N[0], j - N[1], k - N[2]), Uh);
             Primitive(ba.gamma, ba.gasR, Uh, Uoh);
             if (fc.type == 2) {
                 UoO[1] = (!N[0]) * Uoh[1];
@@ -496,14 +512,13 @@ __global__ __launch_bounds__(256) void bc_group(BcArgs ba, BcGroup grp)
         default:
             return;
     }
-    StoreNode(ba.U, g.N, idx, out);
+    StoreNode(ba.U, idx, out);
 }
 
 // Resolved boundary fill: each entry copies one interior node or writes the
 // inflow state of a face (boundary_treatment.c:66-185 replayed on the host).
 struct GatherArgs {
     double *U;
-    long N;
     double gamma;
     double given[7][6];
 };
@@ -516,11 +531,11 @@ __global__ __launch_bounds__(256) void bc_gather(GatherArgs ga, const int *__res
     const int d = dst[t], q = src[t];
     double out[5];
     if (q >= 0) {
-        LoadNode(ga.U, ga.N, q, out);
+        LoadNode(ga.U, q, out);
     } else {
         Conservative(ga.gamma, ga.given[-3 - q], out);
     }
-    StoreNode(ga.U, ga.N, d, out);
+    StoreNode(ga.U, d, out);
 }
 
 // ------------------------------------------------- fused directional sweep
@@ -580,7 +595,6 @@ constexpr size_t kCountBytes = (size_t)kCountSlots * kCountStride * sizeof(unsig
 struct LineArgs {
     double *U;                      // level TO: read at the start, written at the end
     const uint8_t *fluid;
-    long N;
     int n;                          // nodes along the line, ghosts included
     int lo, hi;                     // interior range along the line
     int n1;                         // interior extent of the first transverse axis
@@ -745,8 +759,9 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
     for (int p = tt; p < n; p += T) {
         double v[5];
+        LoadNode(la.U, base + (long)p * la.stride, v);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
+        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c];
         if (p >= la.lo && p < la.hi) {
 #pragma unroll
             for (int c = 0; c < 5; ++c) u0[(p - la.lo) * 5 + c] = v[c];
@@ -870,7 +885,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         if (!fl[k]) continue;
         const int i = la.lo + tt + T * k;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[i * 5 + c];
+        for (int c = 0; c < 5; ++c) NodeAt(la.U, base + (long)i * la.stride)[c] = st[i * 5 + c];
     }
 }
 
@@ -930,8 +945,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
     for (int e = tt; e < 3 * artra::kHidden; e += T) bias[e] = hy.bias[e];
     for (int p = tt; p < n; p += T) {
         double v[5];
+        LoadNode(la.U, base + (long)p * la.stride, v);
 #pragma unroll
-        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c] = la.U[c * la.N + base + (long)p * la.stride];
+        for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c];
         artra::node_rho_p(mdl.gamma, v, nodeRP[p], nodeRP[n + p]);
     }
     for (int e = tt; e < la.nrules; e += T) {
@@ -1016,7 +1032,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
     }
     if (fl) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) la.U[c * la.N + base + (long)i * la.stride] = st[i * 5 + c];
+        for (int c = 0; c < 5; ++c) NodeAt(la.U, base + (long)i * la.stride)[c] = st[i * 5 + c];
     }
 }
 
@@ -1122,7 +1138,7 @@ struct GhostDev {
     double Vs[3], N[3], Ta[3], Tb[3];
 };
 
-__global__ __launch_bounds__(128) void ib_layer(double *U, long N, const GhostDev *ghosts, int begin, int end,
+__global__ __launch_bounds__(128) void ib_layer(double *U, const GhostDev *ghosts, int begin, int end,
                                                 const int *snode, const double *sw, double gamma, double gasR)
 {
     const int gidx = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1132,7 +1148,7 @@ __global__ __launch_bounds__(128) void ib_layer(double *U, long N, const GhostDe
     double wsum = 0.0;
     for (int q = gh.first; q < gh.first + gh.count; ++q) {
         double Uh[5], Uoh[6];
-        LoadNode(U, N, snode[q], Uh);
+        LoadNode(U, snode[q], Uh);
         Primitive(gamma, gasR, Uh, Uoh);
         const double w = sw[q];
 #pragma unroll
@@ -1177,7 +1193,7 @@ __global__ __launch_bounds__(128) void ib_layer(double *U, long N, const GhostDe
     UoG[0] = UoG[4] / (UoG[5] * gasR);
     double out[5];
     Conservative(gamma, UoG, out);
-    StoreNode(U, N, gh.node, out);
+    StoreNode(U, gh.node, out);
 }
 
 // ----------------------------------------------------------- surface force
@@ -1185,7 +1201,7 @@ __global__ __launch_bounds__(128) void ib_layer(double *U, long N, const GhostDe
 // mu (V_rel - (V_rel . N) N) / |pG - pO| of a no-slip body in a viscous
 // flow, and their torque r x (Fp + Fv) (solid_dynamics.c:110-141), in the
 // reference's operation order; terms[t] = Fp[3], Fv[3], Tt[3], p.
-__global__ __launch_bounds__(256) void force_terms(const double *U, long N, const ArtraForceNode *nodes, int count,
+__global__ __launch_bounds__(256) void force_terms(const double *U, const ArtraForceNode *nodes, int count,
                                                    const ArtraForceBody *bodies, double gamma, double gasR,
                                                    double refMu, double refT, double *terms)
 {
@@ -1194,7 +1210,7 @@ __global__ __launch_bounds__(256) void force_terms(const double *U, long N, cons
     const ArtraForceNode e = nodes[t];
     const ArtraForceBody &b = bodies[e.body];
     double Un[5], Uo[6], Fp[3], Fv[3], Fs[3], Tt[3];
-    LoadNode(U, N, e.node, Un);
+    LoadNode(U, e.node, Un);
     Primitive(gamma, gasR, Un, Uo);
 #pragma unroll
     for (int s = 0; s < 3; ++s) Fp[s] = Uo[4] * e.N[s];
@@ -1255,13 +1271,13 @@ __global__ void force_sum(const ArtraForceBody *bodies, int nbody, const double 
 }
 
 // Primitive variables of level TO at a list of nodes (the probes).
-__global__ __launch_bounds__(256) void gather_primitive(const double *U, long N, const int *nodes, int count,
+__global__ __launch_bounds__(256) void gather_primitive(const double *U, const int *nodes, int count,
                                                         double gamma, double gasR, double *prim)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     double Un[5], Uo[6];
-    LoadNode(U, N, nodes[t], Un);
+    LoadNode(U, nodes[t], Un);
     Primitive(gamma, gasR, Un, Uo);
 #pragma unroll
     for (int c = 0; c < 6; ++c) prim[(size_t)t * 6 + c] = Uo[c];
@@ -1281,7 +1297,7 @@ __global__ __launch_bounds__(256) void cfl_max(const double *U, const uint8_t *f
         const int idx = Index(g, i, j, k);
         if (!fluid[idx]) continue;
         double Un[5], Uo[6];
-        LoadNode(U, g.N, idx, Un);
+        LoadNode(U, idx, Un);
         Primitive(gamma, gasR, Un, Uo);
         const double c = sqrt(gamma * gasR * Uo[5]);
 #pragma unroll
@@ -1347,7 +1363,7 @@ __global__ __launch_bounds__(256) void cfl_batch(const CflChunk *chunks, int nch
             const int idx = Index(g, i, j, k);
             if (!bc.fluid[idx]) continue;
             double Un[5], Uo[6];
-            LoadNode(bc.U, g.N, idx, Un);
+            LoadNode(bc.U, idx, Un);
             Primitive(bc.gamma, bc.gasR, Un, Uo);
             const double c = sqrt(bc.gamma * bc.gasR * Uo[5]);
 #pragma unroll
@@ -1463,7 +1479,6 @@ __global__ void dt_single(StepGeometry sg, const unsigned long long *vmax, doubl
 
 struct GatherCase {
     double *U;                      // level TO
-    long N;
     double gamma;
     double given[7][6];
 };
@@ -1479,11 +1494,11 @@ __global__ __launch_bounds__(256) void gather_batch(const GatherCase *cases, con
     const int d = dst[t], q = src[t];
     double out[5];
     if (q >= 0) {
-        LoadNode(gc.U, gc.N, q, out);
+        LoadNode(gc.U, q, out);
     } else {
         Conservative(gc.gamma, gc.given[-3 - q], out);
     }
-    StoreNode(gc.U, gc.N, d, out);
+    StoreNode(gc.U, d, out);
 }
 
 __global__ void fill_i32(int *p, long n, int v)
@@ -1493,6 +1508,25 @@ __global__ void fill_i32(int *p, long n, int v)
 }
 
 // Plain streaming copy: the reference point for HBM counters and bandwidth.
+// Field exchange between the component-major host layout and the
+// node-major device layout (one thread per node).
+__global__ __launch_bounds__(256) void to_node_major(const double *__restrict__ soa, double *__restrict__ aos, long N)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) aos[t * 5 + c] = soa[c * N + t];
+}
+
+__global__ __launch_bounds__(256) void to_component_major(const double *__restrict__ aos, double *__restrict__ soa,
+                                                          long N)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) soa[c * N + t] = aos[t * 5 + c];
+}
+
 __global__ __launch_bounds__(256) void stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, long n)
 {
     for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
@@ -1519,7 +1553,8 @@ __global__ void init_speeds(unsigned long long *vmax, unsigned long long x, unsi
 struct ArtraDevice {
     ArtraDevParams par;
     Grid g;
-    double *U = nullptr;            // [3][5][N]
+    double *U = nullptr;            // [3][N][5]: three levels, node-major
+    double *exchange = nullptr;     // [5][N]: host-layout staging of one level
     uint8_t *fluid = nullptr;
     unsigned long long *vmax = nullptr;
     double *marchBuf = nullptr;     // device march: speeds, clock, step, step record (kMarchChunk)
@@ -1709,7 +1744,7 @@ int ImmersedFill(ArtraDevice *dev, int level)
         if (e <= b) continue;
         int slot;
         if (Begin(dev, ARTRA_T_IB, &slot)) return 1;
-        hipLaunchKernelGGL(ib_layer, dim3((e - b + 127) / 128), dim3(128), 0, dev->stream, U, dev->g.N, dev->ghosts, b,
+        hipLaunchKernelGGL(ib_layer, dim3((e - b + 127) / 128), dim3(128), 0, dev->stream, U, dev->ghosts, b,
                            e, dev->snode, dev->sw, dev->par.gamma, dev->par.gasR);
         HIP_TRY(hipGetLastError());
         if (End(dev, slot)) return 1;
@@ -1727,7 +1762,6 @@ int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
         if (count <= 0) return 0;
         GatherArgs ga;
         ga.U = U;
-        ga.N = dev->g.N;
         ga.gamma = dev->par.gamma;
         for (int p = 0; p < 7; ++p)
             for (int q = 0; q < 6; ++q) ga.given[p][q] = dev->par.varBC[p][q];
@@ -1843,7 +1877,6 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
     LineArgs la{};
     la.U = Level(dev, 0);
     la.fluid = dev->fluid;
-    la.N = g.N;
     la.n = ext[p];
     la.lo = g.lo[p];
     la.hi = g.hi[p];
@@ -2245,6 +2278,7 @@ void artra_dev_destroy(ArtraDevice *dev)
     if (dev->stream) hipStreamSynchronize(dev->stream);
     for (hipEvent_t ev : dev->evPool) hipEventDestroy(ev);
     hipFree(dev->U);
+    hipFree(dev->exchange);
     hipFree(dev->fluid);
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
@@ -2379,18 +2413,35 @@ int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *t)
     return 0;
 }
 
+// The host hands fields over component-major ([5][N], the layout of the C
+// interface); the device keeps them node-major.  The exchange goes through
+// a device buffer of one level, allocated at the first transfer.
+static int ExchangeBuffer(ArtraDevice *dev)
+{
+    if (!dev->exchange) HIP_TRY(hipMalloc(&dev->exchange, (size_t)5 * dev->g.N * sizeof(double)));
+    return 0;
+}
+
 int artra_dev_upload(ArtraDevice *dev, int level, const double *soa)
 {
-    HIP_TRY(hipMemcpyAsync(Level(dev, level), soa, (size_t)5 * dev->g.N * sizeof(double), hipMemcpyHostToDevice,
-                           dev->stream));
+    if (ExchangeBuffer(dev)) return 1;
+    const long N = dev->g.N;
+    HIP_TRY(hipMemcpyAsync(dev->exchange, soa, (size_t)5 * N * sizeof(double), hipMemcpyHostToDevice, dev->stream));
+    hipLaunchKernelGGL(to_node_major, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, dev->stream, dev->exchange,
+                       Level(dev, level), N);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(dev->stream));
     return 0;
 }
 
 int artra_dev_download(ArtraDevice *dev, int level, double *soa)
 {
-    HIP_TRY(hipMemcpyAsync(soa, Level(dev, level), (size_t)5 * dev->g.N * sizeof(double), hipMemcpyDeviceToHost,
-                           dev->stream));
+    if (ExchangeBuffer(dev)) return 1;
+    const long N = dev->g.N;
+    hipLaunchKernelGGL(to_component_major, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, dev->stream,
+                       Level(dev, level), dev->exchange, N);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(soa, dev->exchange, (size_t)5 * N * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     return 0;
 }
@@ -2455,7 +2506,7 @@ int artra_dev_surface_force(ArtraDevice *dev, double *sums)
     if (dev->forceBodyCount <= 0) return 0;
     if (dev->forceCount > 0) {
         hipLaunchKernelGGL(force_terms, dim3((dev->forceCount + 255) / 256), dim3(256), 0, dev->stream,
-                           Level(dev, 0), dev->g.N, dev->forceNodes, dev->forceCount, dev->forceBodies,
+                           Level(dev, 0), dev->forceNodes, dev->forceCount, dev->forceBodies,
                            dev->par.gamma, dev->par.gasR, dev->par.refMu, dev->par.refT, dev->forceTerms);
         HIP_TRY(hipGetLastError());
     }
@@ -2485,7 +2536,7 @@ int artra_dev_gather_primitive(ArtraDevice *dev, const int *nodes, int count, do
         if (nodes[t] < 0 || nodes[t] >= dev->g.N) return Fail("probe node %d outside the grid", nodes[t]);
     }
     HIP_TRY(hipMemcpyAsync(dev->probeNodes, nodes, (size_t)count * sizeof(int), hipMemcpyHostToDevice, dev->stream));
-    hipLaunchKernelGGL(gather_primitive, dim3((count + 255) / 256), dim3(256), 0, dev->stream, Level(dev, 0), dev->g.N,
+    hipLaunchKernelGGL(gather_primitive, dim3((count + 255) / 256), dim3(256), 0, dev->stream, Level(dev, 0),
                        dev->probeNodes, count, dev->par.gamma, dev->par.gasR, dev->probePrim);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(prim, dev->probePrim, (size_t)count * 6 * sizeof(double), hipMemcpyDeviceToHost,
@@ -2931,7 +2982,6 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
             LineArgs la{};
             la.U = Level(dv, 0);
             la.fluid = dv->fluid;
-            la.N = g.N;
             la.n = ext[p];
             la.lo = g.lo[p];
             la.hi = g.hi[p];
@@ -3002,7 +3052,6 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
         for (long st = 0; st < total; st += per) chunks.push_back(CflChunk{c, 0, st, std::min(per, total - st)});
         GatherCase &gc = gcs[c];
         gc.U = Level(dv, 0);
-        gc.N = g.N;
         gc.gamma = dv->par.gamma;
         for (int q = 0; q < 7; ++q)
             for (int v = 0; v < 6; ++v) gc.given[q][v] = dv->par.varBC[q][v];
