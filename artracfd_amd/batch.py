@@ -78,11 +78,11 @@ class Batch:
         _check(_bind().artra_hip_batch_set_indicator(self._h, _fptr(p), float(logit_threshold)))
 
     def counts(self, reset=False):
-        """(faces computed with the linear flux, faces classified, faces of
-        overflowed lines), as Solver.hybrid_counts."""
-        c = (ctypes.c_ulonglong * 3)()
+        """(faces on the linear flux, faces classified, faces on the full
+        WENO5 flux, faces flagged troubled), as Solver.hybrid_counts."""
+        c = (ctypes.c_ulonglong * 4)()
         _check(_bind().artra_hip_batch_counts(self._h, c, 1 if reset else 0))
-        return int(c[0]), int(c[1]), int(c[2])
+        return tuple(int(v) for v in c)
 
     def march(self, steps, graph=True):
         """steps lockstep time steps of every case; returns the device milliseconds."""

@@ -590,11 +590,11 @@ __device__ __forceinline__ int FeedAt(const Feeds &f, int e)
 // atomics per launch on one L2 line.
 constexpr int kCountSlots = 256;
 constexpr int kCountStride = 16;                 // unsigned long long per slot
+constexpr int kTallies = ARTRA_HYBRID_TALLIES;   // faces linear, classified, WENO5, flagged troubled
 constexpr size_t kCountBytes = (size_t)kCountSlots * kCountStride * sizeof(unsigned long long);
 
 struct LineArgs {
     double *U;                      // level TO: read at the start, written at the end
-    const uint8_t *fluid;
     int n;                          // nodes along the line, ghosts included
     int lo, hi;                     // interior range along the line
     int n1;                         // interior extent of the first transverse axis
@@ -640,21 +640,57 @@ __device__ __forceinline__ double Uniform(double x)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// LDS of one line.  Every line: the states [n][5], the face fluxes [F][5],
-// the operator's initial state of the interior nodes [F - 1][5] and the fill
-// rules [2][kLineRules].  With the indicator (hyb) also the FP32 density and
-// pressure of every node [2][n] (the network's features) and one troubled
-// flag per face [F].
-__host__ __device__ __forceinline__ size_t LinePlainBytes(int n, int F)
+// LDS of one line: the states [n][5] and a work area holding the face
+// fluxes [F][5], the operator's initial state of the interior nodes
+// [F - 1][5] and the fill rules with their constant states (kRuleBytes);
+// with the indicator (hyb) also the FP32 density and pressure of every node
+// [2][n] (the network's features).
+__host__ __device__ __forceinline__ size_t LineStateBytes(int n)
 {
-    return ((size_t)(5 * n + 5 * F + 5 * (F - 1)) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) + 15) /
-           16 * 16;
+    return ((size_t)5 * n * sizeof(double) + 15) / 16 * 16;
+}
+
+// The fill rules in LDS: destinations and sources [2][kLineRules] (ints),
+// then the conservative state of every constant-state rule [kLineRules][5].
+constexpr size_t kRuleBytes = (size_t)2 * kLineRules * sizeof(int) + (size_t)kLineRules * 5 * sizeof(double);
+
+// The rules and the constant states of one line's fill into LDS (threads tt
+// of T); the caller's barrier publishes them.
+__device__ __forceinline__ void LoadRules(const LineArgs &la, int *rules, int tt, int T)
+{
+    double *cst = (double *)(rules + 2 * kLineRules);
+    for (int e = tt; e < la.nrules; e += T) {
+        rules[e] = la.rdst[e];
+        rules[kLineRules + e] = la.rsrc[e];
+        if (la.rsrc[e] < 0) {
+            double v[5];
+            Conservative(la.gamma, la.given[-1 - la.rsrc[e]], v);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) cst[e * 5 + c] = v[c];
+        }
+    }
+}
+
+__host__ __device__ __forceinline__ size_t LineWorkPlainBytes(int F)
+{
+    return ((size_t)(5 * F + 5 * (F - 1)) * sizeof(double) + kRuleBytes + 15) / 16 * 16;
+}
+
+__host__ __device__ __forceinline__ size_t LineWorkBytes(int n, int F, bool hyb)
+{
+    return LineWorkPlainBytes(F) + (hyb ? ((size_t)2 * n * sizeof(float) + 15) / 16 * 16 : 0);
 }
 
 __host__ __device__ __forceinline__ size_t LineLdsBytes(int n, int F, bool hyb)
 {
-    const size_t extra = (size_t)2 * n * sizeof(float) + (size_t)F;
-    return LinePlainBytes(n, F) + (hyb ? (extra + 15) / 16 * 16 : 0);
+    return LineStateBytes(n) + LineWorkBytes(n, F, hyb);
+}
+
+// The line's node index of interior position (line % n1, line / n1) across S.
+__device__ __forceinline__ long LineBase(const LineArgs &la, long line)
+{
+    const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
+    return (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
 }
 
 // The linear fifth-order face flux of the learned hybrid at the face between
@@ -737,11 +773,12 @@ __device__ __forceinline__ bool ClassifyFace(const LineHybrid &hy, const float *
 // With HYB, the indicator classifies the faces once per operator (at its
 // first RK stage, before the stage loop: the solution moves less than a
 // cell within one operator, and the network's fragments are then dead in
-// the flux passes) and every stage computes the linear flux on the smooth
-// faces and the full WENO5 flux on the troubled ones.  A wavefront with no
-// troubled face runs the linear flux only; one holding troubled faces runs
-// both branches for its lanes, so a line crossed by a shock pays full
-// WENO5 only on the wavefronts around the shock.
+// the flux passes) and the routing is per wavefront: a wavefront whose 64
+// faces are all smooth computes the linear flux, one holding a troubled
+// face computes the full WENO5 flux on all its faces (a branch the whole
+// wavefront takes, never both branches), so a line crossed by a shock pays
+// full WENO5 only on the wavefronts around the shock and no wavefront pays
+// more than the plain kernel.
 template <int S, int NS, bool FAST, bool HYB, int K>
 __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &mdl, const LineHybrid &hy, long line,
                                          int tt, int T, char *mem, bool write, unsigned long long *tally,
@@ -750,13 +787,13 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     constexpr int SL = (NS == 6) ? 2 : 1;           // stencil nodes left of the face's left node
     const int n = la.n, F = la.hi - la.lo + 1;
     double *st = (double *)mem;                     // [n][5]
-    double *fb = st + 5 * n;                        // [F][5]
+    char *work = mem + LineStateBytes(n);
+    double *fb = (double *)work;                    // [F][5]
     double *u0 = fb + 5 * F;                        // [F - 1][5]
     int *rules = (int *)(u0 + 5 * (F - 1));         // [2][kLineRules]: destinations, sources
-    float *nodeRP = (float *)(mem + LinePlainBytes(n, F));   // [2][n] (HYB)
-    uint8_t *tflag = (uint8_t *)(nodeRP + 2 * n);   // [F] (HYB)
-    const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
-    const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
+    const double *cst = (const double *)(rules + 2 * kLineRules);   // [kLineRules][5] constant states
+    float *nodeRP = (float *)(work + LineWorkPlainBytes(F));   // [2][n] (HYB)
+    const long base = LineBase(la, line);
     for (int p = tt; p < n; p += T) {
         double v[5];
         LoadNode(la.U, base + (long)p * la.stride, v);
@@ -770,17 +807,12 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     }
     // the line's fill rules into LDS: read per lane from the argument
     // block once here, where the field loads hide the latency
-    for (int e = tt; e < la.nrules; e += T) {
-        rules[e] = la.rdst[e];
-        rules[kLineRules + e] = la.rsrc[e];
-    }
-    bool own[K], fl[K];
+    LoadRules(la, rules, tt, T);
+    // the nodes this thread updates (every interior node of a fused line is
+    // a fluid node: the fused path runs only where all are)
+    bool own[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int i = la.lo + tt + T * k;
-        own[k] = i < la.hi;
-        fl[k] = own[k] && la.fluid[base + (long)i * la.stride];
-    }
+    for (int k = 0; k < K; ++k) own[k] = la.lo + tt + T * k < la.hi;
     __syncthreads();
     // between-stage refill duties: the ghosts this thread's nodes feed, and
     // (threads tt < nrules) one constant-state rule
@@ -797,26 +829,36 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
     // line to line instead of loading one SIMD with all of them.
     const int QT = M * T;
     const int rot = HYB ? (int)(line % (QT / 64)) * 64 : 0;
+    // bit m: this wavefront's faces of pass m take the full WENO5 flux
+    unsigned wenoPasses = HYB ? 0u : ~0u;
     if (HYB) {
         // classification (every thread runs the same trip count)
-        unsigned long long troubledFaces = 0, faces = 0;
+        unsigned long long wenoFaces = 0, faces = 0, flagged = 0;
         for (int m = 0; m < M; ++m) {
             const int q = (tt + T * m + rot) % QT;
             const bool valid = q < F;
             const int f = la.lo - 1 + (valid ? q : F - 1);
             const bool troubled = ClassifyFace(hy, nodeRP, n, f, valid, frags, bias);
-            if (valid) tflag[q] = troubled;
-            troubledFaces += __popcll(__ballot(troubled));
-            faces += __popcll(__ballot(valid));
+            const unsigned long long here = __popcll(__ballot(valid));
+            const unsigned long long marked = __popcll(__ballot(troubled));
+            if (marked) {
+                wenoPasses |= 1u << m;
+                wenoFaces += here;
+                flagged += marked;
+            }
+            faces += here;
         }
+        wenoPasses = __builtin_amdgcn_readfirstlane(wenoPasses);
         // tallies per wavefront: faces on the linear flux, faces classified,
-        // faces on the full WENO5 flux
+        // faces on the full WENO5 flux, faces the indicator flagged troubled
         if (__lane_id() == 0 && write) {
-            atomicAdd(&tally[0], faces - troubledFaces);
+            atomicAdd(&tally[0], faces - wenoFaces);
             atomicAdd(&tally[1], faces);
-            if (troubledFaces) atomicAdd(&tally[2], troubledFaces);
+            if (wenoFaces) {
+                atomicAdd(&tally[2], wenoFaces);
+                atomicAdd(&tally[3], flagged);
+            }
         }
-        __syncthreads();
     }
     for (int s = 0; s < la.stages; ++s) {
         for (int m = 0; m < M; ++m) {
@@ -827,7 +869,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             }
             const int f = la.lo - 1 + q;   // face between f and f + 1
             double Fx[5];
-            if (HYB && !tflag[q]) {
+            if (HYB && !((wenoPasses >> m) & 1u)) {
                 LinearFaceFlux<S>(mdl, st, f, Fx);
             } else if (FAST && NS == 6) {
                 artra::weno_flux_line<S>(mdl, st, f, Fx);
@@ -850,7 +892,7 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
         const bool refill = s + 1 < la.stages;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (!fl[k]) continue;
+            if (!own[k]) continue;
             const int i = la.lo + tt + T * k;
             const int q = i - la.lo;
             double v[5];
@@ -872,17 +914,15 @@ __device__ __forceinline__ void LineBody(const LineArgs &la, const FluxModel &md
             }
         }
         if (refill && constRule >= 0) {
-            double v[5];
-            Conservative(la.gamma, la.given[-1 - rules[kLineRules + constRule]], v);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = v[c];
+            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = cst[constRule * 5 + c];
         }
         __syncthreads();
     }
     if (!write) return;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (!fl[k]) continue;
+        if (!own[k]) continue;
         const int i = la.lo + tt + T * k;
 #pragma unroll
         for (int c = 0; c < 5; ++c) NodeAt(la.U, base + (long)i * la.stride)[c] = st[i * 5 + c];
@@ -896,12 +936,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : F
 {
     if (la.dt && *la.dt == 0.0) return;             // a device-stepped march past its end time
     extern __shared__ double lds[];
-    __shared__ unsigned long long tally[3];
+    __shared__ unsigned long long tally[kTallies];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
     artra::MlpWave frags;
     if (HYB) {
         for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
-        if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
+        if (threadIdx.x < kTallies) tally[threadIdx.x] = 0ull;
         artra::mlp_wave_fragments(hy.W1, hy.W2, frags);
     }
     const long line = XcdLine(blockIdx.x, gridDim.x);
@@ -911,6 +951,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HYB ? 3 : F
         atomicAdd(slot, tally[0]);
         atomicAdd(slot + 1, tally[1]);
         atomicAdd(slot + 2, tally[2]);
+        atomicAdd(slot + 3, tally[3]);
     }
 }
 
@@ -935,8 +976,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
     const int n = la.n, F = la.hi - la.lo + 1;
     double *st = lds;                                   // [n][5]
     double *fb = st + 5 * n;                            // [F][5]
-    int *rules = (int *)(fb + 5 * F);                   // [2][kLineRules]
-    float *nodeRP = (float *)(rules + 2 * kLineRules);  // [2][n]
+    int *rules = (int *)(fb + 5 * F);                   // [2][kLineRules], constant states
+    const double *cst = (const double *)(rules + 2 * kLineRules);
+    float *nodeRP = (float *)((char *)rules + kRuleBytes);   // [2][n]
     const long line = XcdLine(blockIdx.x, gridDim.x);
     const int a1 = (int)(line % la.n1), a2 = (int)(line / la.n1);
     const long base = (long)(la.lo1 + a1) * la.stride1 + (long)(la.lo2 + a2) * la.stride2;
@@ -950,13 +992,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
         for (int c = 0; c < 5; ++c) st[p * 5 + c] = v[c];
         artra::node_rho_p(mdl.gamma, v, nodeRP[p], nodeRP[n + p]);
     }
-    for (int e = tt; e < la.nrules; e += T) {
-        rules[e] = la.rdst[e];
-        rules[kLineRules + e] = la.rsrc[e];
-    }
+    LoadRules(la, rules, tt, T);
     const int i = la.lo + tt;                       // this thread's node
     const bool own = i < la.hi;
-    const bool fl = own && la.fluid[base + (long)i * la.stride];
     if (tt == 0) troubledAny = 0;
     __syncthreads();
     // classification: the faces q = tt and q = tt + T (same trip count on
@@ -1002,7 +1040,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
         // node update (fluid_dynamics.c:243-274) and, before another
         // stage, the refill of the ghosts this node feeds
         const bool refill = s + 1 < la.stages;
-        if (fl) {
+        if (own) {
             const double a = la.a[s], b = la.b[s];
             const int q = i - la.lo;
             double v[5];
@@ -1023,25 +1061,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MEAN ? 6 : 
             }
         }
         if (refill && constRule >= 0) {
-            double v[5];
-            Conservative(la.gamma, la.given[-1 - rules[kLineRules + constRule]], v);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = v[c];
+            for (int c = 0; c < 5; ++c) st[rules[constRule] * 5 + c] = cst[constRule * 5 + c];
         }
         __syncthreads();
     }
-    if (fl) {
+    if (own) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) NodeAt(la.U, base + (long)i * la.stride)[c] = st[i * 5 + c];
     }
 }
 
-// LDS of rk_lean: states [n][5], face fluxes [F][5], fill rules, FP32
-// density and pressure [2][n].
+// LDS of rk_lean: states [n][5], face fluxes [F][5], fill rules and
+// constant states, FP32 density and pressure [2][n].
 __host__ __device__ __forceinline__ size_t LeanLdsBytes(int n, int F)
 {
-    return (size_t)(5 * n + 5 * F) * sizeof(double) + (size_t)2 * kLineRules * sizeof(int) +
-           (size_t)2 * n * sizeof(float);
+    return (size_t)(5 * n + 5 * F) * sizeof(double) + kRuleBytes + (size_t)2 * n * sizeof(float);
 }
 
 // The full hybrid kernel over a list of lines (those rk_lean deferred):
@@ -1057,10 +1092,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
     const int total = *count;
     if ((int)blockIdx.x >= total) return;
     extern __shared__ double lds[];
-    __shared__ unsigned long long tally[3];
+    __shared__ unsigned long long tally[kTallies];
     __shared__ float bias[3 * artra::kHidden];
     for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
-    if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
+    if (threadIdx.x < kTallies) tally[threadIdx.x] = 0ull;
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
         // the network's fragments are loaded per line (cache hits) and die
         // after its classification; an opaque memory effect keeps the loads
@@ -1078,6 +1113,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         atomicAdd(slot, tally[0]);
         atomicAdd(slot + 1, tally[1]);
         atomicAdd(slot + 2, tally[2]);
+        atomicAdd(slot + 3, tally[3]);
     }
 }
 
@@ -1100,11 +1136,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
                                                                                          FluxModel mdl, LineHybrid hy)
 {
     extern __shared__ double lds[];
-    __shared__ unsigned long long tally[3];
+    __shared__ unsigned long long tally[kTallies];
     __shared__ float bias[HYB ? 3 * artra::kHidden : 1];
     if (HYB) {
         for (int e = threadIdx.x; e < 3 * artra::kHidden; e += blockDim.x) bias[e] = hy.bias[e];
-        if (threadIdx.x < 3) tally[threadIdx.x] = 0ull;
+        if (threadIdx.x < kTallies) tally[threadIdx.x] = 0ull;
     }
     for (long w = blockIdx.x; w < ntiles; w += gridDim.x) {
         const BatchTile bt = tiles[w];
@@ -1127,6 +1163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void r
         atomicAdd(slot, tally[0]);
         atomicAdd(slot + 1, tally[1]);
         atomicAdd(slot + 2, tally[2]);
+        atomicAdd(slot + 3, tally[3]);
     }
 }
 
@@ -1876,7 +1913,6 @@ int LineSweep(ArtraDevice *dev, double dt, int p)
     const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
     LineArgs la{};
     la.U = Level(dev, 0);
-    la.fluid = dev->fluid;
     la.n = ext[p];
     la.lo = g.lo[p];
     la.hi = g.hi[p];
@@ -2723,21 +2759,21 @@ int artra_dev_fused_axes(ArtraDevice *dev)
 }
 
 // Sum of the tally slots (and their reset).
-static int SumCounts(unsigned long long *slots, unsigned long long counts[3], int reset)
+static int SumCounts(unsigned long long *slots, unsigned long long counts[kTallies], int reset)
 {
     std::vector<unsigned long long> h((size_t)kCountSlots * kCountStride);
     HIP_TRY(hipMemcpy(h.data(), slots, kCountBytes, hipMemcpyDeviceToHost));
-    counts[0] = counts[1] = counts[2] = 0;
+    for (int v = 0; v < kTallies; ++v) counts[v] = 0;
     for (int q = 0; q < kCountSlots; ++q)
-        for (int v = 0; v < 3; ++v) counts[v] += h[(size_t)q * kCountStride + v];
+        for (int v = 0; v < kTallies; ++v) counts[v] += h[(size_t)q * kCountStride + v];
     if (reset) HIP_TRY(hipMemset(slots, 0, kCountBytes));
     return 0;
 }
 
-int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[3], int reset)
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[kTallies], int reset)
 {
     if (!dev->hybCounts) {
-        counts[0] = counts[1] = counts[2] = 0;
+        for (int v = 0; v < kTallies; ++v) counts[v] = 0;
         return 0;
     }
     HIP_TRY(hipStreamSynchronize(dev->stream));
@@ -2981,7 +3017,6 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
             const int t1 = (p == 0) ? 1 : 0, t2 = (p == 2) ? 1 : 2;
             LineArgs la{};
             la.U = Level(dv, 0);
-            la.fluid = dv->fluid;
             la.n = ext[p];
             la.lo = g.lo[p];
             la.hi = g.hi[p];
@@ -3155,9 +3190,9 @@ int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float thre
     return 0;
 }
 
-int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[3], int reset)
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[kTallies], int reset)
 {
-    counts[0] = counts[1] = counts[2] = 0;
+    for (int v = 0; v < kTallies; ++v) counts[v] = 0;
     if (!b->counts) return 0;
     HIP_TRY(hipStreamSynchronize(b->stream));
     return SumCounts(b->counts, counts, reset);

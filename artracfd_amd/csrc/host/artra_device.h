@@ -142,7 +142,7 @@ int artra_dev_set_exact(ArtraDevice *dev, int exact);
 int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float threshold);
 /* faces computed with the linear flux, faces classified, and faces of lines
  * that overflowed their queue (plain WENO5) since the last reset */
-int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[3], int reset);
+int artra_dev_hybrid_counts(ArtraDevice *dev, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset);
 int artra_dev_set_fused(ArtraDevice *dev, int fused);
 int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation);
 int artra_dev_fused_axes(ArtraDevice *dev);
@@ -166,7 +166,7 @@ int artra_dev_batch_create(ArtraDevice *const *devs, int count, const double *cl
                            ArtraBatch **out);
 void artra_dev_batch_destroy(ArtraBatch *b);
 int artra_dev_batch_set_indicator(ArtraBatch *b, const float *params, float threshold);
-int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[3], int reset);
+int artra_dev_batch_counts(ArtraBatch *b, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset);
 int artra_dev_batch_march(ArtraBatch *b, int steps, int use_graph, double *ms);
 int artra_dev_batch_clock(ArtraBatch *b, double *clock);
 int artra_dev_batch_set_clock(ArtraBatch *b, const double *clock);

@@ -422,7 +422,7 @@ int artra_hip_set_indicator(ArtraHipSolver *s, const float *params, float logit_
     return artra_dev_set_indicator(s->dev, params, logit_threshold) ? DevError() : 0;
 }
 
-int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[3], int reset)
+int artra_hip_hybrid_counts(ArtraHipSolver *s, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset)
 {
     return artra_dev_hybrid_counts(s->dev, counts, reset) ? DevError() : 0;
 }
@@ -701,7 +701,7 @@ int artra_hip_batch_set_indicator(ArtraHipBatch *hb, const float *params, float 
     return artra_dev_batch_set_indicator(hb->b, params, logit_threshold) ? DevError() : 0;
 }
 
-int artra_hip_batch_counts(ArtraHipBatch *hb, unsigned long long counts[3], int reset)
+int artra_hip_batch_counts(ArtraHipBatch *hb, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset)
 {
     return artra_dev_batch_counts(hb->b, counts, reset) ? DevError() : 0;
 }

@@ -183,7 +183,7 @@ def evaluate(case_directory, params, thresholds, max_steps=1 << 20):
     rho0, steps0, el0, _ = run(None, 0.0)
     out = []
     for thr in thresholds:
-        rho, steps, el, (lin, seen, _) = run(params, thr)
+        rho, steps, el, (lin, seen, _, _) = run(params, thr)
         out.append({"threshold": float(thr), "rel_l1_rho": float(np.abs(rho - rho0).sum() / np.abs(rho0).sum()),
                     "linear_fraction": lin / max(seen, 1), "steps": steps, "plain_steps": steps0,
                     "seconds": el, "plain_seconds": el0})

@@ -377,11 +377,12 @@ class Solver:
         _check(library().artra_hip_set_indicator(self._h, _fptr(p), float(logit_threshold)))
 
     def hybrid_counts(self, reset=False):
-        """(faces computed with the linear flux, faces classified, faces of
-        lines that overflowed their queue and ran plain WENO5)."""
-        c = (ctypes.c_ulonglong * 3)()
+        """(faces computed with the linear flux, faces classified, faces
+        computed with the full WENO5 flux, faces the indicator flagged
+        troubled); the routing is per wavefront of 64 faces."""
+        c = (ctypes.c_ulonglong * 4)()
         _check(library().artra_hip_hybrid_counts(self._h, c, 1 if reset else 0))
-        return int(c[0]), int(c[1]), int(c[2])
+        return tuple(int(v) for v in c)
 
     def close(self):
         if self._h:

@@ -172,7 +172,8 @@ def batched_both(with_indicator=True):
     if with_indicator:
         hyb = batched(use_indicator=True)
         plain["hybrid"] = {k: hyb[k] for k in ("value", "serial_value", "wall_ms", "linear_face_fraction",
-                                                "weno5_face_fraction", "tuned", "serial_ms_per_case")}
+                                                "weno5_face_fraction", "flagged_face_fraction", "tuned",
+                                                "serial_ms_per_case")}
         plain["hybrid"]["vs_plain"] = round(hyb["value"] / plain["value"], 3)
     return plain
 
@@ -207,7 +208,7 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
         t2 = time.perf_counter()
         dev_ms = b.march(steps, graph=True)
         wall_s = time.perf_counter() - t2
-        lin, seen, over = b.counts()
+        lin, seen, over, flagged = b.counts()
         b.close()
         for s in solvers:
             s.close()
@@ -240,6 +241,7 @@ def batched(seed=2026, count=64, steps=100, use_indicator=True):
                                    "p99": round(float(np.percentile(pc, 99)), 3), "sum": round(float(pc.sum()), 3)},
             "linear_face_fraction": round(lin / max(seen, 1), 4) if params is not None else None,
             "weno5_face_fraction": round(over / max(seen, 1), 4) if params is not None else None,
+            "flagged_face_fraction": round(flagged / max(seen, 1), 4) if params is not None else None,
             "tuned": {"threads_per_line": {str(sd): sorted(set(t for t, d in zip(threads, drawn) if d["side"] == sd))
                                            for sd in sides}, "grid_blocks": grid},
             "setup_seconds": round(setup_s, 2), "tune_seconds": round(tune_s, 2),
@@ -335,7 +337,7 @@ def main():
         case = Case(case_dir)
         solver = Solver(case)
         solver.set_indicator(params, thr)
-        hdone, helapsed, htm, (lin, seen, over) = timed(solver, args.warmup, args.steps)
+        hdone, helapsed, htm, (lin, seen, over, flagged) = timed(solver, args.warmup, args.steps)
         solver.close()
         case.close()
         hvalue = cells * 18 * hdone / helapsed / 1e6
@@ -350,6 +352,7 @@ def main():
             "ms_per_step": round(helapsed * 1e3 / max(hdone, 1), 4),
             "linear_face_fraction": round(lin / max(seen, 1), 4),
             "weno5_face_fraction": round(over / max(seen, 1), 4),
+            "flagged_face_fraction": round(flagged / max(seen, 1), 4),
             "rel_l1_rho_vs_plain_at_end": float(np.abs(rho_h - rho_p).sum() / np.abs(rho_p).sum()),
             "end_time_steps": {"plain": steps_p, "hybrid": steps_h},
             "end_time_seconds": {"plain": round(el_p, 3), "hybrid": round(el_h, 3)},

@@ -61,6 +61,9 @@ void ComputeGeometricField(Space *space, const Model *model);
 
 /* ----------------------------------------------------- 2. resident API */
 
+/* Face tallies of the learned hybrid (artra_hip_hybrid_counts). */
+#define ARTRA_HYBRID_TALLIES 4
+
 typedef struct ArtraHipSolver ArtraHipSolver;
 
 /* Upload level TO of space (all nodes) and build the boundary tables.  The
@@ -222,7 +225,7 @@ int artra_hip_batch_tune(ArtraHipBatch *batch, int reps, int *threads, int *grid
 int artra_hip_batch_set_threads(ArtraHipBatch *batch, const int *threads);
 /* Learned indicator for every case (NULL: plain WENO5). */
 int artra_hip_batch_set_indicator(ArtraHipBatch *batch, const float *params, float logit_threshold);
-int artra_hip_batch_counts(ArtraHipBatch *batch, unsigned long long counts[3], int reset);
+int artra_hip_batch_counts(ArtraHipBatch *batch, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset);
 /* steps lockstep time steps; use_graph replays one captured step; *ms the
  * device time of the steps (may be NULL).  Each case starts from its Time
  * (now, stepC) and solver clock as they stand, and its Time advances. */
@@ -275,8 +278,10 @@ int artra_hip_set_indicator_options(ArtraHipSolver *solver, int every_stage, flo
 /* Face tallies per classification (once per directional operator):
  * counts[0] faces computed with the linear flux, counts[1] faces
  * classified, counts[2] faces computed with the full WENO5 flux
- * (counts[0] + counts[2] = counts[1]). */
-int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[3], int reset);
+ * (counts[0] + counts[2] = counts[1]; the routing is per wavefront of 64
+ * faces, which takes WENO5 when any of its faces is troubled), counts[3]
+ * faces the indicator flagged troubled (at most counts[2]). */
+int artra_hip_hybrid_counts(ArtraHipSolver *solver, unsigned long long counts[ARTRA_HYBRID_TALLIES], int reset);
 
 /* Train in place on the device (Adam, weighted logistic loss of
  * label > tau); loss_history[epoch] receives the last batch loss per epoch

@@ -94,7 +94,7 @@ def test_batch_with_indicator_matches_single_case_runs(tmp_path):
     b = Batch(solvers)
     b.set_indicator(params, 0.5)
     b.march(3)
-    lin, seen, weno = b.counts()
+    lin, seen, weno, _ = b.counts()
     assert 0 < lin < seen and lin + weno == seen
     for q, (c, s) in enumerate(zip(cases, solvers)):
         assert np.array_equal(c.interior(s.field()), ref[q][0]), q
@@ -142,7 +142,7 @@ def test_batch_with_trained_indicator_matches_single_runs_for_every_tile_shape(t
     b.set_threads([T] * len(dirs))
     b.counts(reset=True)
     b.march(5)
-    lin, seen, weno = b.counts()
+    lin, seen, weno, _ = b.counts()
     print("linear", lin / seen, "weno5", weno / seen)
     assert 0 < weno < seen and 0 < lin and lin + weno == seen
     for q, (c, s) in enumerate(zip(cases, solvers)):
