@@ -133,6 +133,75 @@ def cpu_baseline(steps=10):
     return out
 
 
+def reference_rate(name, steps):
+    """The reference's own solver (oracle/_ref/artra_ref, compiled from its
+    sources in the build container) on cases/<name> for `steps` full time
+    steps, single thread: (Mcell-stage updates/s, seconds, steps), or None
+    where the harness is absent."""
+    import shutil
+    import subprocess
+    import tempfile
+    ref = os.path.join(ROOT, "oracle", "_ref", "artra_ref")
+    if not os.path.exists(ref):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        work = os.path.join(tmp, "case")
+        shutil.copytree(os.path.join(ROOT, "cases", name), work)
+        run = subprocess.run([ref, "run", str(steps), "out"], cwd=work, check=True, capture_output=True, text=True,
+                             timeout=600)
+        info = json.loads(run.stdout.strip().splitlines()[-1])
+    return info["fluid_nodes"] * 18 * info["steps"] / info["seconds"] / 1e6, info["seconds"], info["steps"]
+
+
+# The viscous and immersed-boundary workloads timed beside the headline:
+# (case, timed steps, CPU sample case, CPU sample steps, dominant kernel family)
+SIDE_WORKLOADS = (
+    ("tgv128", 20, "tgv64", 4, "sweep_x/sweep_march with the viscous flux (stage-by-stage sweeps)"),
+    ("cylinder", 40, "cyl64", 100, "sweep_x/sweep_march with the immersed-boundary fill (stage-by-stage sweeps)"),
+)
+
+
+def side_workloads(warmup, with_cpu):
+    """Rate, kernel times and the sweep roofline of the viscous Taylor-Green
+    vortex (128^3) and the shock over the immersed cylinder (512x512x4),
+    with the reference's CPU rate on a smaller sample of each."""
+    from artracfd_amd import Case, Solver
+    out = {}
+    for name, steps, cpu_case, cpu_steps, kernel in SIDE_WORKLOADS:
+        case_dir = os.path.join(ROOT, "cases", name)
+        case = Case(case_dir)
+        cells = fluid_nodes(case)
+        solver = Solver(case)
+        done, elapsed, tm, _ = timed(solver, warmup, steps)
+        solver.close()
+        case.close()
+        sweep_ms, sweep_n = tm["sweep"]
+        stage_b = stage_bytes(1)
+        # every sweep launch of these workloads is one RK stage of one axis
+        achieved = cells * stage_b * (18.0 * done) / (sweep_ms / 1e3) / 1e9
+        busy = {k: v[0] for k, v in tm.items()}
+        entry = {
+            "workload": describe(case_dir, name, cells), "value": round(cells * 18 * done / elapsed / 1e6, 3),
+            "unit": "Mcell-stage updates/s", "steps": done, "ms_per_step": round(elapsed * 1e3 / max(done, 1), 4),
+            "kernel_ms": {k: round(v, 3) for k, v in busy.items()},
+            "kernel_launches": {k: v[1] for k, v in tm.items()},
+            "dominant": max(busy, key=busy.get),
+            "roofline": {"kernel": kernel, "bound": "hbm", "algorithmic_bytes_per_cell_stage": round(stage_b, 1),
+                         "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "avg_launch_ms": round(sweep_ms / max(sweep_n, 1), 4), "launches": sweep_n},
+        }
+        if with_cpu:
+            r = reference_rate(cpu_case, cpu_steps)
+            if r is not None:
+                entry["cpu_baseline"] = {"value": round(r[0], 4), "unit": "Mcell-stage updates/s", "cores": 1,
+                                         "kind": "reference",
+                                         "sample": f"{cpu_case}, {r[2]} full time steps, {r[1]:.1f} s, the "
+                                                   f"reference's solver compiled from its own sources, single thread"}
+        out[name] = entry
+    return out
+
+
 def timed(solver, warmup, steps):
     solver.march(warmup)
     solver.synchronize()
@@ -261,6 +330,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-case batched workload")
     ap.add_argument("--batched-only", action="store_true", help="run only the 64-case batched workload")
     ap.add_argument("--batched-plain", action="store_true", help="batched workload without the indicator run")
+    ap.add_argument("--no-side", action="store_true", help="skip the viscous TGV and immersed-cylinder workloads")
     ap.add_argument("--no-end-time", action="store_true",
                     help="skip the plain and hybrid marches to the end time (profiling runs)")
     args = ap.parse_args()
@@ -360,6 +430,8 @@ def main():
         }
     if not args.no_batched and not args.unfused:
         out["batched"] = batched_both(not args.batched_plain)
+    if not args.no_side and not args.unfused:
+        out["viscous_and_immersed"] = side_workloads(args.warmup, not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
