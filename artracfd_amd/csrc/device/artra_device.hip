@@ -12,8 +12,9 @@
 //                  coalesced along x across the wavefront.
 //   bc_group       a group of TreatBoundary boxes whose nodes do not
 //                  interact (boundary_treatment.c:28-192).
-//   ib_layer       one ghost layer of the immersed-boundary fill from the
-//                  host-precomputed stencil table (immersed_boundary.c:270-487).
+//   ib_layer_wave  one ghost layer of the immersed-boundary fill from the
+//                  host-precomputed stencil table, a wavefront per ghost node
+//                  (immersed_boundary.c:270-487).
 //   cfl_max        per-axis maximum characteristic speed (solve.c:108-153).
 //
 // All FP64 arithmetic follows the reference's operation order; the file is
@@ -1175,23 +1176,13 @@ struct GhostDev {
     double Vs[3], N[3], Ta[3], Tb[3];
 };
 
-__global__ __launch_bounds__(128) void ib_layer(double *U, const GhostDev *ghosts, int begin, int end,
-                                                const int *snode, const double *sw, double gamma, double gasR)
+// The ghost node's state from the weighted sums of its stencil's primitive
+// states (Uo) and weights (wsum): for a first-layer ghost the boundary
+// point's state joins the stencil, then the state is reflected across the
+// wall (immersed_boundary.c:345-392).
+__device__ __forceinline__ void IbFinish(double *U, const GhostDev &gh, double Uo[6], double wsum, double gamma,
+                                         double gasR)
 {
-    const int gidx = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (gidx >= end) return;
-    const GhostDev gh = ghosts[gidx];
-    double Uo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double wsum = 0.0;
-    for (int q = gh.first; q < gh.first + gh.count; ++q) {
-        double Uh[5], Uoh[6];
-        LoadNode(U, snode[q], Uh);
-        Primitive(gamma, gasR, Uh, Uoh);
-        const double w = sw[q];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) Uo[c] = Uo[c] + Uoh[c] * w;
-        wsum = wsum + w;
-    }
     double UoG[6];
     if (gh.kind == 0) {
         // boundary point from the pre-estimate, then the boundary point joins the stencil
@@ -1231,6 +1222,50 @@ __global__ __launch_bounds__(128) void ib_layer(double *U, const GhostDev *ghost
     double out[5];
     Conservative(gamma, UoG, out);
     StoreNode(U, gh.node, out);
+}
+
+// Lane e's value of x (e the same on every lane).
+__device__ __forceinline__ double LaneValue(double x, int e)
+{
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)bits, e);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(bits >> 32), e);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// One wavefront per ghost node: the lanes load and convert the stencil
+// entries in parallel (the loop's memory latency), then the weighted sums
+// run in the stencil's order over the lanes' values, the same operations
+// as a loop over the stencil (immersed_boundary.c:393-440), so the ghost's
+// state is bit for bit the reference's.
+__global__ __launch_bounds__(256) void ib_layer_wave(double *U, const GhostDev *ghosts, int begin, int end,
+                                                     const int *snode, const double *sw, double gamma, double gasR)
+{
+    const int gidx = begin + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (gidx >= end) return;                        // whole wavefronts leave together
+    const int lane = __lane_id();
+    const GhostDev gh = ghosts[gidx];
+    double Uo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double wsum = 0.0;
+    const int last = gh.first + gh.count;
+    for (int base = gh.first; base < last; base += 64) {
+        const int q = base + lane;
+        double Uoh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w = 0.0;
+        if (q < last) {
+            double Uh[5];
+            LoadNode(U, snode[q], Uh);
+            Primitive(gamma, gasR, Uh, Uoh);
+            w = sw[q];
+        }
+        const int m = min(64, last - base);
+        for (int e = 0; e < m; ++e) {
+            const double we = LaneValue(w, e);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Uo[c] = Uo[c] + LaneValue(Uoh[c], e) * we;
+            wsum = wsum + we;
+        }
+    }
+    if (lane == 0) IbFinish(U, gh, Uo, wsum, gamma, gasR);
 }
 
 // ----------------------------------------------------------- surface force
@@ -1781,8 +1816,8 @@ int ImmersedFill(ArtraDevice *dev, int level)
         if (e <= b) continue;
         int slot;
         if (Begin(dev, ARTRA_T_IB, &slot)) return 1;
-        hipLaunchKernelGGL(ib_layer, dim3((e - b + 127) / 128), dim3(128), 0, dev->stream, U, dev->ghosts, b,
-                           e, dev->snode, dev->sw, dev->par.gamma, dev->par.gasR);
+        hipLaunchKernelGGL(ib_layer_wave, dim3((e - b + 3) / 4), dim3(256), 0, dev->stream, U, dev->ghosts, b, e,
+                           dev->snode, dev->sw, dev->par.gamma, dev->par.gasR);
         HIP_TRY(hipGetLastError());
         if (End(dev, slot)) return 1;
     }
