@@ -38,11 +38,12 @@ FP64_PEAK_TFLOPS = 78.6
 # Measured with rocprofv3 on this workload (profiles/rk_line_shockvortex256.md):
 # HBM bytes per cell-stage of the fused sweep kernel (FETCH_SIZE x 2.0 +
 # WRITE_SIZE, scale factors from a 1 GiB copy kernel) and FP64 flops per
-# cell-stage of the plain WENO5 flux and update (FMA = 2).
-TRAFFIC_B_PER_CELL_STAGE = 27.8
-FP64_FLOP_PER_CELL_STAGE = 1044.0
+# cell-stage of the plain WENO5 flux and update (FMA = 2, from the FP64
+# instruction counters).
+TRAFFIC_B_PER_CELL_STAGE = 25.9
+FP64_FLOP_PER_CELL_STAGE = 914.0
 # the stage-by-stage sweeps (profiles/sweep_shockvortex256.md)
-UNFUSED_TRAFFIC_B_PER_CELL_STAGE = 129.6
+UNFUSED_TRAFFIC_B_PER_CELL_STAGE = 112.3
 METRIC = "Mcell-stage updates/s (256³ WENO5-RK3) + learned-hybrid vs plain at ≤1e-3 L1"
 
 
@@ -154,10 +155,12 @@ def reference_rate(name, steps):
 
 
 # The viscous and immersed-boundary workloads timed beside the headline:
-# (case, timed steps, CPU sample case, CPU sample steps, dominant kernel family)
+# (case, timed steps, CPU sample case, CPU sample steps, dominant kernel family,
+# measured HBM bytes per cell-stage of its sweeps, profiles/sweep_tgv128_cylinder.md)
 SIDE_WORKLOADS = (
-    ("tgv128", 20, "tgv64", 4, "sweep_x/sweep_march with the viscous flux (stage-by-stage sweeps)"),
-    ("cylinder", 40, "cyl64", 100, "sweep_x/sweep_march with the immersed-boundary fill (stage-by-stage sweeps)"),
+    ("tgv128", 20, "tgv64", 4, "sweep_x/sweep_march with the viscous flux (stage-by-stage sweeps)", 176.6),
+    ("cylinder", 40, "cyl64", 100, "sweep_x/sweep_march with the immersed-boundary fill (stage-by-stage sweeps)",
+     126.0),
 )
 
 
@@ -167,7 +170,7 @@ def side_workloads(warmup, with_cpu):
     with the reference's CPU rate on a smaller sample of each."""
     from artracfd_amd import Case, Solver
     out = {}
-    for name, steps, cpu_case, cpu_steps, kernel in SIDE_WORKLOADS:
+    for name, steps, cpu_case, cpu_steps, kernel, traffic_b in SIDE_WORKLOADS:
         case_dir = os.path.join(ROOT, "cases", name)
         case = Case(case_dir)
         cells = fluid_nodes(case)
@@ -189,6 +192,7 @@ def side_workloads(warmup, with_cpu):
             "roofline": {"kernel": kernel, "bound": "hbm", "algorithmic_bytes_per_cell_stage": round(stage_b, 1),
                          "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic_bytes_per_cell_stage": traffic_b,
                          "avg_launch_ms": round(sweep_ms / max(sweep_n, 1), 4), "launches": sweep_n},
         }
         if with_cpu:
