@@ -19,7 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libartra_hip.so")
 ARCH = os.environ.get("ARTRA_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["case_io.c", "calculator.c", "setup.c", "geometry.c", "tables.c", "capi.c", "output.c"]
+HOST_SOURCES = ["case_io.c", "calculator.c", "setup.c", "geometry.c", "tables.c", "capi.c", "output.c", "motion.c"]
 DEVICE_SOURCES = ["artra_device.hip", "indicator.hip"]
 
 HOST_FLAGS = ["-O2", "-std=c11", "-fPIC", "-fopenmp", "-ffp-contract=off", "-fno-fast-math",

@@ -1176,6 +1176,16 @@ struct GhostDev {
     double Vs[3], N[3], Ta[3], Tb[3];
 };
 
+// A table of filled nodes on the device with its stencil entries; the
+// buffers grow as needed and are reused when the table is rebuilt.
+struct IbTable {
+    GhostDev *ghosts = nullptr;
+    int *snode = nullptr;
+    double *sw = nullptr;
+    int count = 0;
+    int ghostCap = 0, stencilCap = 0;
+};
+
 // The ghost node's state from the weighted sums of its stencil's primitive
 // states (Uo) and weights (wsum): for a first-layer ghost the boundary
 // point's state joins the stencil, then the state is reflected across the
@@ -1649,9 +1659,8 @@ struct ArtraDevice {
     int *gDst[4] = {nullptr, nullptr, nullptr, nullptr};
     int *gSrc[4] = {nullptr, nullptr, nullptr, nullptr};
     long gCount[4] = {0, 0, 0, 0};
-    GhostDev *ghosts = nullptr;
-    int *snode = nullptr;
-    double *sw = nullptr;
+    IbTable ib;                     // ghost nodes, grouped by layer
+    IbTable joins;                  // nodes a moving body has just uncovered
     std::vector<int> layerStart;
     FluxModel mdl{};
     int segment = 32;
@@ -1803,8 +1812,17 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
     return 0;
 }
 
-// mode 0: the whole external boundary; 1 + s: only the ghosts a sweep along
-// axis s reads (valid between the stages of an inviscid split sweep).
+// Entries [b, e) of a table, one wavefront each.
+int LaunchIb(ArtraDevice *dev, double *U, const IbTable &tb, int b, int e)
+{
+    int slot;
+    if (Begin(dev, ARTRA_T_IB, &slot)) return 1;
+    hipLaunchKernelGGL(ib_layer_wave, dim3((e - b + 3) / 4), dim3(256), 0, dev->stream, U, tb.ghosts, b, e, tb.snode,
+                       tb.sw, dev->par.gamma, dev->par.gasR);
+    HIP_TRY(hipGetLastError());
+    return End(dev, slot);
+}
+
 // The immersed-boundary ghost fill of one level, layer by layer
 // (immersed_boundary.c:270-340).
 int ImmersedFill(ArtraDevice *dev, int level)
@@ -1813,16 +1831,75 @@ int ImmersedFill(ArtraDevice *dev, int level)
     const int layers = (int)dev->layerStart.size() - 1;
     for (int r = 0; r < layers; ++r) {
         const int b = dev->layerStart[r], e = dev->layerStart[r + 1];
-        if (e <= b) continue;
-        int slot;
-        if (Begin(dev, ARTRA_T_IB, &slot)) return 1;
-        hipLaunchKernelGGL(ib_layer_wave, dim3((e - b + 3) / 4), dim3(256), 0, dev->stream, U, dev->ghosts, b, e,
-                           dev->snode, dev->sw, dev->par.gamma, dev->par.gasR);
-        HIP_TRY(hipGetLastError());
-        if (End(dev, slot)) return 1;
+        if (e > b && LaunchIb(dev, U, dev->ib, b, e)) return 1;
     }
     return 0;
 }
+
+// Replace a table's contents with a host table.  Kernels already enqueued
+// may still read the old contents, so the stream drains first.
+int UploadIbTable(ArtraDevice *dev, IbTable &tb, const ArtraGhostTable *t)
+{
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    tb.count = 0;
+    if (t->nghost > tb.ghostCap) {
+        const int cap = t->nghost + t->nghost / 4 + 64;
+        hipFree(tb.ghosts);
+        tb.ghosts = nullptr;
+        tb.ghostCap = 0;
+        HIP_TRY(hipMalloc(&tb.ghosts, (size_t)cap * sizeof(GhostDev)));
+        tb.ghostCap = cap;
+    }
+    if (t->nstencil > tb.stencilCap) {
+        const int cap = t->nstencil + t->nstencil / 4 + 256;
+        hipFree(tb.snode);
+        hipFree(tb.sw);
+        tb.snode = nullptr;
+        tb.sw = nullptr;
+        tb.stencilCap = 0;
+        HIP_TRY(hipMalloc(&tb.snode, (size_t)cap * sizeof(int)));
+        HIP_TRY(hipMalloc(&tb.sw, (size_t)cap * sizeof(double)));
+        tb.stencilCap = cap;
+    }
+    if (t->nghost == 0) return 0;
+    std::vector<GhostDev> host(t->nghost);
+    for (int q = 0; q < t->nghost; ++q) {
+        const ArtraGhost &s = t->ghosts[q];
+        GhostDev &d = host[q];
+        d.node = s.node;
+        d.kind = s.kind;
+        d.first = s.first;
+        d.count = s.count;
+        d.noslip = s.noslip;
+        d.isothermal = s.isothermal;
+        d.wallT = s.wallT;
+        d.wO = s.wO;
+        for (int k = 0; k < 3; ++k) {
+            d.Vs[k] = s.Vs[k];
+            d.N[k] = s.N[k];
+            d.Ta[k] = s.Ta[k];
+            d.Tb[k] = s.Tb[k];
+        }
+    }
+    HIP_TRY(hipMemcpy(tb.ghosts, host.data(), host.size() * sizeof(GhostDev), hipMemcpyHostToDevice));
+    if (t->nstencil > 0) {
+        HIP_TRY(hipMemcpy(tb.snode, t->stencil_node, (size_t)t->nstencil * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(tb.sw, t->stencil_w, (size_t)t->nstencil * sizeof(double), hipMemcpyHostToDevice));
+    }
+    tb.count = t->nghost;
+    return 0;
+}
+
+void FreeIbTable(IbTable &tb)
+{
+    hipFree(tb.ghosts);
+    hipFree(tb.snode);
+    hipFree(tb.sw);
+    tb = IbTable();
+}
+
+// mode 0: the whole external boundary; 1 + s: only the ghosts a sweep along
+// axis s reads (valid between the stages of an inviscid split sweep).
 
 int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
 {
@@ -2272,6 +2349,17 @@ const char *artra_dev_error(void)
     return gError;
 }
 
+// Every interior node fluid (the fused sweeps' condition).
+static int AllFluid(const Grid &g, const uint8_t *fluid)
+{
+    for (int k = g.lo[2]; k < g.hi[2]; ++k)
+        for (int j = g.lo[1]; j < g.hi[1]; ++j)
+            for (int i = g.lo[0]; i < g.hi[0]; ++i) {
+                if (!fluid[((long)k * g.ny + j) * g.nx + i]) return 0;
+            }
+    return 1;
+}
+
 int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevice **out)
 {
     *out = nullptr;
@@ -2330,15 +2418,7 @@ int artra_dev_create(const ArtraDevParams *par, const uint8_t *fluid, ArtraDevic
         for (int q = 0; q < 6; ++q) ba.face[p].given[q] = par->varBC[p][q];
     }
     dev->layerStart.assign(1, 0);
-    dev->allFluid = 1;
-    for (int k = g.lo[2]; k < g.hi[2] && dev->allFluid; ++k)
-        for (int j = g.lo[1]; j < g.hi[1] && dev->allFluid; ++j)
-            for (int i = g.lo[0]; i < g.hi[0]; ++i) {
-                if (!fluid[((long)k * g.ny + j) * g.nx + i]) {
-                    dev->allFluid = 0;
-                    break;
-                }
-            }
+    dev->allFluid = AllFluid(g, fluid);
     *out = dev;
     return 0;
 }
@@ -2359,9 +2439,8 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->forceSums);
     hipFree(dev->probeNodes);
     hipFree(dev->probePrim);
-    hipFree(dev->ghosts);
-    hipFree(dev->snode);
-    hipFree(dev->sw);
+    FreeIbTable(dev->ib);
+    FreeIbTable(dev->joins);
     for (int m = 0; m < 4; ++m) {
         hipFree(dev->gDst[m]);
         hipFree(dev->gSrc[m]);
@@ -2448,39 +2527,25 @@ int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial)
 
 int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *t)
 {
-    hipFree(dev->ghosts);
-    hipFree(dev->snode);
-    hipFree(dev->sw);
-    dev->ghosts = nullptr;
-    dev->snode = nullptr;
-    dev->sw = nullptr;
-    dev->layerStart.assign(t->layer_start, t->layer_start + t->layers + 1);
-    if (t->nghost == 0) return 0;
-    std::vector<GhostDev> host(t->nghost);
-    for (int q = 0; q < t->nghost; ++q) {
-        const ArtraGhost &s = t->ghosts[q];
-        GhostDev &d = host[q];
-        d.node = s.node;
-        d.kind = s.kind;
-        d.first = s.first;
-        d.count = s.count;
-        d.noslip = s.noslip;
-        d.isothermal = s.isothermal;
-        d.wallT = s.wallT;
-        d.wO = s.wO;
-        for (int k = 0; k < 3; ++k) {
-            d.Vs[k] = s.Vs[k];
-            d.N[k] = s.N[k];
-            d.Ta[k] = s.Ta[k];
-            d.Tb[k] = s.Tb[k];
-        }
-    }
-    HIP_TRY(hipMalloc(&dev->ghosts, host.size() * sizeof(GhostDev)));
-    HIP_TRY(hipMalloc(&dev->snode, (size_t)t->nstencil * sizeof(int)));
-    HIP_TRY(hipMalloc(&dev->sw, (size_t)t->nstencil * sizeof(double)));
-    HIP_TRY(hipMemcpy(dev->ghosts, host.data(), host.size() * sizeof(GhostDev), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dev->snode, t->stencil_node, (size_t)t->nstencil * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(dev->sw, t->stencil_w, (size_t)t->nstencil * sizeof(double), hipMemcpyHostToDevice));
+    dev->layerStart.assign(1, 0);
+    if (UploadIbTable(dev, dev->ib, t)) return 1;
+    if (t->nghost > 0) dev->layerStart.assign(t->layer_start, t->layer_start + t->layers + 1);
+    return 0;
+}
+
+int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *joins)
+{
+    if (level < 0 || level > 2) return Fail("bad time level %d", level);
+    if (joins->nghost == 0) return 0;
+    if (UploadIbTable(dev, dev->joins, joins)) return 1;
+    return LaunchIb(dev, Level(dev, level), dev->joins, 0, joins->nghost);
+}
+
+int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid)
+{
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    HIP_TRY(hipMemcpy(dev->fluid, fluid, (size_t)dev->g.N, hipMemcpyHostToDevice));
+    dev->allFluid = AllFluid(dev->g, fluid);
     return 0;
 }
 
@@ -2549,6 +2614,7 @@ int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 int artra_dev_set_force_table(ArtraDevice *dev, const ArtraForceNode *nodes, int count, const ArtraForceBody *bodies,
                               int nbody)
 {
+    HIP_TRY(hipStreamSynchronize(dev->stream));
     hipFree(dev->forceNodes);
     hipFree(dev->forceBodies);
     hipFree(dev->forceTerms);

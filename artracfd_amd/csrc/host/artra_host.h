@@ -71,6 +71,9 @@ int artra_prepare_case(const char *dir, Time *time, Space *space, Model *model);
 /* Release everything artra_prepare_case allocated. */
 void artra_release_case(Space *space, Model *model, Time *time);
 
+/* Immersed-boundary stencil table (defined with the device-side tables). */
+typedef struct ArtraGhostTable ArtraGhostTable;
+
 /* --------------------------------------------------------------- geometry */
 int artra_read_stl(const char *fname, Polyhedron *poly);
 int artra_convert_polyhedron(Polyhedron *poly);
@@ -82,8 +85,11 @@ Real artra_point_triangle_distance(const Real p[DIMS], const Real v0[DIMS], cons
 int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, int *fid);
 void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS],
                                   Real pm[DIMS], Real N[DIMS]);
-/* Node classification did/fid/lid/gst (immersed_boundary.c:65-264). */
-void artra_compute_geometric_field(Space *space, const Model *model);
+/* Node classification did/fid/lid/gst (immersed_boundary.c:65-264).  A
+ * fluid node that was inside a body before the call is marked fid = -1 and,
+ * when joins is not NULL, listed there with its reconstruction stencil
+ * (artra_add_join); the caller evaluates the list.  Returns 0 or an error. */
+int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins);
 void artra_identify_geometry_state(Geometry *geo);
 
 /* ------------------------------------------------- device-side tables */
@@ -144,7 +150,7 @@ typedef struct {
     Real Tb[DIMS];
 } ArtraGhost;
 
-typedef struct {
+struct ArtraGhostTable {
     int layers;          /* gl */
     int *layer_start;    /* layers+1 offsets into ghosts (ghosts grouped by layer) */
     ArtraGhost *ghosts;
@@ -152,10 +158,25 @@ typedef struct {
     int *stencil_node;
     Real *stencil_w;
     int nstencil;
-} ArtraGhostTable;
+    int ghost_cap;       /* allocated entries (tables grown one node at a time) */
+    int stencil_cap;
+};
 
 int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTable *table);
 void artra_free_ghost_table(ArtraGhostTable *table);
+
+/*
+ * Nodes that a moving body has just uncovered (immersed_boundary.c:193-204):
+ * artra_add_join appends node (k, j, i) as an entry of kind 1 (normalized
+ * inverse-distance weighting) whose stencil is the search from radius 2 over
+ * the fluid nodes that were fluid before the move and are not themselves
+ * newly uncovered, evaluated on the node flags as they stand when the
+ * classification pass reaches the node.  artra_apply_joins evaluates the
+ * entries on the host node array (level tn); the device evaluates the same
+ * table with the immersed-boundary kernel.
+ */
+int artra_add_join(const Space *space, int k, int j, int i, ArtraGhostTable *joins);
+void artra_apply_joins(Space *space, const Model *model, const ArtraGhostTable *joins, int tn);
 
 #ifdef __cplusplus
 }

@@ -75,15 +75,6 @@ static void FieldToNodes(Space *space, int tn, const double *field, long N)
 
 static int Supported(const Space *space, const Model *model)
 {
-    if (0 != model->psi) return artra_set_error(2, "moving bodies are not supported");
-    for (int n = 0; n < space->geo.totN; ++n) {
-        const Polyhedron *p = space->geo.poly + n;
-        for (int s = 0; s < DIMS; ++s) {
-            if (0.0 != p->V[TO][s] || 0.0 != p->W[TO][s]) {
-                return artra_set_error(2, "body %d moves; only stationary bodies are supported", n);
-            }
-        }
-    }
     if (model->refMu > 0.0 && OPTSPLIT == model->multidim && COLLAPSEN != space->part.collapse) {
         return artra_set_error(2, "viscous flux on a collapsed split grid reads outside the grid");
     }
@@ -280,9 +271,26 @@ static int ChunkSteps(const Time *time, Real lastDt, int left)
     return (k < 1) ? 1 : k;
 }
 
+/* The time loop with moving bodies: the host takes every step, the solid
+ * half-steps needing the surface force and moving the geometry between the
+ * fluid steps (solve.c:65-89). */
+static int MarchMoving(ArtraHipSolver *s, Time *time, Real tmInt, int max_steps, int *steps_done, Real *dts)
+{
+    int done = 0, rc = 0;
+    while (time->now < time->end && time->stepC < time->stepN && done < max_steps) {
+        Real dt = 0.0;
+        if ((rc = artra_moving_step(s, time, tmInt, &s->rcInt, &dt))) break;
+        if (dts) dts[done] = dt;
+        ++done;
+    }
+    if (steps_done) *steps_done = done;
+    return rc ? rc : artra_hip_synchronize(s);
+}
+
 int artra_hip_march(ArtraHipSolver *s, Time *time, int max_steps, int *steps_done, Real *dts)
 {
     const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
+    if (0 != s->model->psi) return MarchMoving(s, time, tmInt, max_steps, steps_done, dts);
     double floor[DIMS];
     BodySpeedFloor(s, floor);
     double rec[ARTRA_DEV_MARCH_CHUNK];
@@ -317,6 +325,10 @@ int artra_hip_march_noalloc(ArtraHipSolver *s, Time *time, int max_steps, int *s
     if (max_steps < 0 || max_steps > ARTRA_NOALLOC_MAX_STEPS) {
         return artra_set_error(1, "artra_hip_march_noalloc: max_steps %d outside 0..%d", max_steps,
                                ARTRA_NOALLOC_MAX_STEPS);
+    }
+    if (0 != s->model->psi) {
+        return artra_set_error(1, "artra_hip_march_noalloc: moving bodies (phase interaction %d) rebuild "
+                               "device tables every step; use artra_hip_march", s->model->psi);
     }
     if (!(time->now < time->end && time->stepC < time->stepN) || max_steps == 0) return 0;
     const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : time->end / (Real)(time->dataW[PROSD]);
@@ -520,15 +532,20 @@ void TreatImmersedBoundary(const int tn, Space *space, const Model *model)
 
 void ComputeGeometricField(Space *space, const Model *model)
 {
-    if (0 != model->psi) {
-        fprintf(stderr, "error: moving bodies (phase interaction %d) are not supported by this library\n", model->psi);
-        exit(EXIT_FAILURE);
+    ArtraGhostTable joins;
+    memset(&joins, 0, sizeof joins);
+    int rc = artra_compute_geometric_field(space, model, &joins);
+    /* the nodes a moving body uncovered are reconstructed in the caller's
+     * node array, which is authoritative between drop-in calls */
+    if (!rc) artra_apply_joins(space, model, &joins, TO);
+    artra_free_ghost_table(&joins);
+    /* the cached device solver's tables follow the new flags */
+    if (!rc && gDropIn.solver && gDropIn.space == space && gDropIn.node == space->node) {
+        rc = artra_hip_refresh_geometry(gDropIn.solver);
     }
-    artra_compute_geometric_field(space, model);
-    /* the cached device solver's tables were built from the previous flags */
-    if (gDropIn.solver && gDropIn.space == space) {
-        artra_hip_detach(gDropIn.solver);
-        memset(&gDropIn, 0, sizeof gDropIn);
+    if (rc) {
+        fprintf(stderr, "error: %s\n", artra_hip_error());
+        exit(EXIT_FAILURE);
     }
 }
 
@@ -739,7 +756,7 @@ int artra_case_clock(const ArtraCase *c, double *now, int *step)
 int artra_case_forces(const ArtraCase *c, Real *forces)
 {
     const Geometry *geo = &c->space.geo;
-    for (int n = 0; n < geo->totN; ++n) {
+    for (int n = 0; forces && n < geo->totN; ++n) {
         const Polyhedron *p = geo->poly + n;
         for (int d = 0; d < DIMS; ++d) {
             forces[9 * n + d] = p->Fp[d];
@@ -750,3 +767,15 @@ int artra_case_forces(const ArtraCase *c, Real *forces)
     return geo->totN;
 }
 
+int artra_case_bodies(const ArtraCase *c, Real *state)
+{
+    const Geometry *geo = &c->space.geo;
+    for (int n = 0; state && n < geo->totN; ++n) {
+        const Polyhedron *p = geo->poly + n;
+        const Real *parts[5] = {p->O, p->V[TO], p->W[TO], p->at[TO], p->ar[TO]};
+        for (int q = 0; q < 5; ++q) {
+            for (int d = 0; d < DIMS; ++d) state[15 * n + 3 * q + d] = parts[q][d];
+        }
+    }
+    return geo->totN;
+}

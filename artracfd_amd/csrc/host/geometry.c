@@ -477,7 +477,7 @@ static int InterState(const Partition *part, const Node *node, int k, int j, int
     return 0;
 }
 
-void artra_compute_geometric_field(Space *space, const Model *model)
+int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins)
 {
     (void)model;
     const Partition *part = &space->part;
@@ -531,17 +531,24 @@ void artra_compute_geometric_field(Space *space, const Model *model)
                     }
                 }
     }
-    /* interfacial and ghost layers (immersed_boundary.c:183-225).  Newly
-     * joined fluid nodes only arise for moving bodies, which are rejected
-     * when the case is loaded. */
+    /* interfacial and ghost layers (immersed_boundary.c:183-225); a fluid
+     * node that a moving body has uncovered (gst, the previous domain field,
+     * differs from did = 0) is listed for reconstruction with the stencil the
+     * flags give at this point of the pass, then marked */
     for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
         for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
             for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
                 Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                if (nd->gst != nd->did && 0 == nd->did) {
+                    if (NULL == joins) return artra_set_error(1, "a body uncovered node (%d, %d, %d)", k, j, i);
+                    if (artra_add_join(space, k, j, i, joins)) return 1;
+                    nd->fid = NONE;
+                }
                 nd->lid = 0;
                 nd->gst = 0;
                 if (0 == nd->did) continue;
                 nd->lid = InterState(part, node, k, j, i, nd->did, 0);
                 if (0 < nd->lid) nd->gst = InterState(part, node, k, j, i, 0, 1);
             }
+    return 0;
 }

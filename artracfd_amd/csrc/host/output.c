@@ -11,6 +11,7 @@
 #include <float.h>
 #include <limits.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1014,6 +1015,29 @@ static int WriteDataKind(ArtraHipSolver *s, const Time *time, int n)
  * the device and the outputs at the loop's cadence.  The device march stops
  * at the first step where an output falls due, so every output is written
  * from exactly the state the reference writes it from. */
+/* EvolveSolution with moving bodies: the host takes every step
+ * (artra_moving_step: the solid half-steps around the fluid step) and writes
+ * the outputs that fall due after it (solve.c:65-105). */
+static int SolveMoving(ArtraHipSolver *s, Time *time, const Real dtData[NPROBE], Real rcData[NPROBE], Real tmInt)
+{
+    Real rcInt = 0.0;
+    while (time->now < time->end && time->stepC < time->stepN) {
+        Real dt = 0.0;
+        if (artra_moving_step(s, time, tmInt, &rcInt, &dt)) return 1;
+        for (int n = 0; n < NPROBE; ++n) {
+            rcData[n] = rcData[n] + dt;
+            if (rcData[n] >= dtData[n] || time->now == time->end || time->stepC == time->stepN) {
+                if (PROFC == n && artra_hip_surface_force(s)) return 1;
+                if (PROSD == n) ++time->dataC;
+                if (WriteDataKind(s, time, n)) return 1;
+                rcData[n] = 0.0;
+            }
+        }
+    }
+    s->rcInt = rcInt;
+    return artra_hip_synchronize(s);
+}
+
 int artra_hip_solve(ArtraHipSolver *s, Time *time)
 {
     if (0 == time->restart) {
@@ -1026,6 +1050,7 @@ int artra_hip_solve(ArtraHipSolver *s, Time *time)
         rcData[n] = 0.0;
     }
     const Real tmInt = (INT_MAX == time->dataW[PROSD]) ? time->end : dtData[PROSD];
+    if (0 != s->model->psi) return SolveMoving(s, time, dtData, rcData, tmInt);
     Real rcInt = 0.0, lastDt = 0.0;
     double floor[DIMS];
     {
@@ -1090,25 +1115,90 @@ int artra_hip_solve(ArtraHipSolver *s, Time *time)
 }
 
 /* ------------------------------------------------------------- FP64 restart */
-/* Restart file: "ARTRARS1", int32 n[3], ng[3], stepC, dataC, double now,
+/* Restart file: "ARTRARS2", int32 n[3], ng[3], stepC, dataC, double now,
  * rcInt, then level TO of every node (ghosts included) as FP64
- * component-major [5][N].  Unlike the reference's restart from its
- * "%.6g" field text (paraview_reader.c), a run continued from it is the
- * uninterrupted run bit for bit. */
-static const char kRestartMagic[8] = {'A', 'R', 'T', 'R', 'A', 'R', 'S', '1'};
+ * component-major [5][N], then int32 moving (the phase interaction is on).
+ * A moving case adds its bodies and the node classification: int32 totN;
+ * per body int32 state, faceN, edgeN, vertN, the FP64 state from r to box
+ * as Polyhedron lays it out (centre, inertia, velocities, accelerations,
+ * forces, bounding box, ...), and for a triangulated body its vertices and
+ * face, edge and vertex normals; then int32 did, fid, lid, gst [4][N].
+ * Unlike the reference's restart from its "%.6g" text (paraview_reader.c,
+ * data_stream.c:154-188), a run continued from it is the uninterrupted run
+ * bit for bit. */
+static const char kRestartMagic[8] = {'A', 'R', 'T', 'R', 'A', 'R', 'S', '2'};
+
+/* The FP64 part of a body's state: every member from r to box. */
+#define BODY_STATE_BYTES (offsetof(Polyhedron, f) - offsetof(Polyhedron, r))
+
+static int BodyIo(Polyhedron *poly, FILE *fp, int write)
+{
+    int cnt[4] = {poly->state, poly->faceN, poly->edgeN, poly->vertN};
+    if (write) {
+        if (fwrite(cnt, sizeof cnt, 1, fp) != 1 || fwrite(&poly->r, BODY_STATE_BYTES, 1, fp) != 1) return 1;
+    } else {
+        if (fread(cnt, sizeof cnt, 1, fp) != 1) return 1;
+        if (cnt[1] != poly->faceN || cnt[2] != poly->edgeN || cnt[3] != poly->vertN) return 1;
+        if (fread(&poly->r, BODY_STATE_BYTES, 1, fp) != 1) return 1;
+        poly->state = cnt[0];
+    }
+    if (0 >= poly->faceN) return 0;
+    Real (*arrays[4])[DIMS] = {poly->v, poly->Nf, poly->Ne, poly->Nv};
+    const int rows[4] = {poly->vertN, poly->faceN, poly->edgeN, poly->vertN};
+    for (int a = 0; a < 4; ++a) {
+        const size_t n = (size_t)rows[a] * DIMS;
+        if ((write ? fwrite(arrays[a], sizeof(Real), n, fp) : fread(arrays[a], sizeof(Real), n, fp)) != n) return 1;
+    }
+    return 0;
+}
+
+static int FlagsIo(Space *space, long N, FILE *fp, int write)
+{
+    int *buf = malloc((size_t)N * sizeof(int));
+    if (!buf) return 1;
+    int bad = 0;
+    for (int f = 0; f < 4 && !bad; ++f) {
+        if (write) {
+            for (long q = 0; q < N; ++q) {
+                const Node *nd = space->node + q;
+                buf[q] = (0 == f) ? nd->did : (1 == f) ? nd->fid : (2 == f) ? nd->lid : nd->gst;
+            }
+            bad = fwrite(buf, sizeof(int), (size_t)N, fp) != (size_t)N;
+        } else {
+            bad = fread(buf, sizeof(int), (size_t)N, fp) != (size_t)N;
+            for (long q = 0; q < N && !bad; ++q) {
+                Node *nd = space->node + q;
+                if (0 == f) nd->did = buf[q];
+                else if (1 == f) nd->fid = buf[q];
+                else if (2 == f) nd->lid = buf[q];
+                else nd->gst = buf[q];
+            }
+        }
+    }
+    free(buf);
+    return bad;
+}
 
 int artra_hip_write_restart(ArtraHipSolver *s, const Time *time, const char *path)
 {
     const Partition *part = &s->space->part;
+    Geometry *geo = &s->space->geo;
     if (artra_dev_download(s->dev, TO, s->staging)) return artra_dev_failed();
     FILE *fp = OpenFile(path, "wb");
     if (!fp) return 1;
     const int hdr[8] = {part->n[X], part->n[Y], part->n[Z], part->ng[X], part->ng[Y], part->ng[Z], time->stepC,
                         time->dataC};
     const double clk[2] = {time->now, s->rcInt};
+    const int moving = (0 != s->model->psi);
     int ok = fwrite(kRestartMagic, 1, 8, fp) == 8 && fwrite(hdr, sizeof hdr, 1, fp) == 1 &&
              fwrite(clk, sizeof clk, 1, fp) == 1 &&
-             fwrite(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N;
+             fwrite(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N &&
+             fwrite(&moving, sizeof moving, 1, fp) == 1;
+    if (ok && moving) {
+        ok = fwrite(&geo->totN, sizeof geo->totN, 1, fp) == 1;
+        for (int n = 0; ok && n < geo->totN; ++n) ok = !BodyIo(geo->poly + n, fp, 1);
+        ok = ok && !FlagsIo(s->space, s->N, fp, 1);
+    }
     ok = (0 == fclose(fp)) && ok;
     return ok ? 0 : artra_set_error(1, "writing the restart file %s failed", path);
 }
@@ -1116,22 +1206,30 @@ int artra_hip_write_restart(ArtraHipSolver *s, const Time *time, const char *pat
 int artra_hip_read_restart(ArtraHipSolver *s, Time *time, const char *path)
 {
     const Partition *part = &s->space->part;
+    Geometry *geo = &s->space->geo;
     FILE *fp = OpenFile(path, "rb");
     if (!fp) return 1;
     char magic[8];
-    int hdr[8];
+    int hdr[8], moving = 0, totN = 0;
     double clk[2];
     int ok = fread(magic, 1, 8, fp) == 8 && 0 == memcmp(magic, kRestartMagic, 8) && fread(hdr, sizeof hdr, 1, fp) == 1 &&
              fread(clk, sizeof clk, 1, fp) == 1;
     for (int d = 0; ok && d < DIMS; ++d) ok = hdr[d] == part->n[d] && hdr[DIMS + d] == part->ng[d];
-    ok = ok && fread(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N;
+    ok = ok && fread(s->staging, sizeof(double), (size_t)5 * s->N, fp) == (size_t)5 * s->N &&
+         fread(&moving, sizeof moving, 1, fp) == 1 && moving == (0 != s->model->psi);
+    if (ok && moving) {
+        ok = fread(&totN, sizeof totN, 1, fp) == 1 && totN == geo->totN;
+        for (int n = 0; ok && n < geo->totN; ++n) ok = !BodyIo(geo->poly + n, fp, 0);
+        ok = ok && !FlagsIo(s->space, s->N, fp, 0);
+    }
     fclose(fp);
-    if (!ok) return artra_set_error(1, "%s is not a restart file of this grid", path);
+    if (!ok) return artra_set_error(1, "%s is not a restart file of this case", path);
     if (artra_dev_upload(s->dev, TO, s->staging)) return artra_dev_failed();
     Node *node = s->space->node;
     for (long q = 0; q < s->N; ++q) {
         for (int c = 0; c < DIMU; ++c) node[q].U[TO][c] = s->staging[c * s->N + q];
     }
+    if (moving && artra_hip_refresh_geometry(s)) return 1;
     time->stepC = hdr[6];
     time->dataC = hdr[7];
     time->now = clk[0];

@@ -79,7 +79,6 @@ int artra_compute_parameters(Time *time, Space *space, Model *model)
     model->refMu = model->refMu / (model->refRho * model->refV * model->refL);
     model->gasR = 1.0 / (model->gamma * model->refMa * model->refMa);
     model->cv = model->gasR / (model->gamma - 1.0);
-    if (0 != model->psi) return artra_set_error(2, "moving bodies (phase interaction) are not supported");
     return 0;
 }
 
@@ -338,7 +337,7 @@ int artra_initialize_domain(const char *dir, Time *time, Space *space, Model *mo
     if (InitializeField(space, model)) return 1;
     if (InitializeGeometry(dir, &space->geo)) return 1;
     artra_compute_geometry_parameters(space->part.collapse, &space->geo);
-    artra_compute_geometric_field(space, model);
+    if (artra_compute_geometric_field(space, model, NULL)) return 1;
     artra_identify_geometry_state(&space->geo);
     return 0;
 }

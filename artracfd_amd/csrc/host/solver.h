@@ -35,5 +35,15 @@ struct ArtraHipSolver {
 int artra_build_force_table(ArtraHipSolver *s);
 /* Error from the device side, as an artra_set_error code. */
 int artra_dev_failed(void);
+/* The device tables that follow the node classification (fluid flags,
+ * immersed-boundary ghosts, surface-force nodes), rebuilt from the host
+ * flags after the bodies moved. */
+int artra_hip_refresh_geometry(ArtraHipSolver *s);
+/* EvolveSolidDynamics (solid_dynamics.c:37-48) over dt at time now. */
+int artra_solid_step(ArtraHipSolver *s, Real now, Real dt);
+/* One pass of the time loop's body with moving bodies (solve.c:65-89): the
+ * step count, dt with its rectifications (rcInt, tmInt), the clock, and the
+ * solid half-steps around the fluid step.  *dt receives the step taken. */
+int artra_moving_step(ArtraHipSolver *s, Time *time, Real tmInt, Real *rcInt, Real *dt);
 
 #endif /* ARTRA_SOLVER_H_ */

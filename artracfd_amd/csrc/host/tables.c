@@ -9,11 +9,15 @@
  * sets (written and read) do not interact, so one kernel launch can serve a
  * whole group with results identical to the sequential order.
  *
- * Immersed boundary: for stationary bodies every ghost node's interpolation
- * stencil is fixed.  artra_build_ghost_table replays the neighbour search of
- * InverseDistanceWeighting (immersed_boundary.c:413-473) once, recording the
- * stencil nodes and inverse-distance weights in summation order, and the
- * boundary-point data of ReconstructFlow (immersed_boundary.c:357-412).
+ * Immersed boundary: while the node flags stay as they are, every ghost
+ * node's interpolation stencil is fixed.  artra_build_ghost_table replays the
+ * neighbour search of InverseDistanceWeighting (immersed_boundary.c:413-473)
+ * once per node classification (at attach, and after every move of the
+ * bodies), recording the stencil nodes and inverse-distance weights in
+ * summation order, and the boundary-point data of ReconstructFlow
+ * (immersed_boundary.c:357-412).  artra_add_join records, in the same form,
+ * the reconstruction of a node that a moving body has just uncovered
+ * (immersed_boundary.c:193-204).
  */
 #include <stdlib.h>
 #include <string.h>
@@ -165,11 +169,18 @@ static Real InverseDistance(Real tiny, Real d2)
     return 1.0 / d2;
 }
 
+/* Target set of the stencil search besides did (immersed_boundary.c:446-462):
+ * every node of the domain (TYPED), a ghost layer (layer >= 1), or the nodes
+ * of the domain that were in it before the bodies moved and have not been
+ * reconstructed since (TYPEF: gst, which still holds the previous domain
+ * field, equal to did, and fid not the reconstruction mark). */
+enum { kTypeDomain = 0, kTypeOriginal = -2 };
+
 /*
  * Stencil search around node c for point p: the box of half-width h grows
- * until it contains at least one node of the target set (did == did and, for
- * layer >= 1, gst == layer), scanned k-j-i within the interior range.
- * Appends the entries and returns the running weight sum in *wsum.
+ * until it contains at least one node of the target set, scanned k-j-i
+ * within the interior range.  Appends the entries and returns the running
+ * weight sum in *wsum.
  */
 static int SearchStencil(const Space *space, const int c[DIMS], const Real p[DIMS], int h, int did, int layer,
                          StencilBuf *sb, int *count, Real *wsum)
@@ -189,6 +200,7 @@ static int SearchStencil(const Space *space, const int c[DIMS], const Real p[DIM
                     const int q = artra_index(nh[Z], nh[Y], nh[X], part->n[Y], part->n[X]);
                     if (did != node[q].did) continue;
                     if (layer > 0 && layer != node[q].gst) continue;
+                    if (kTypeOriginal == layer && (did != node[q].gst || 0 > node[q].fid)) continue;
                     const Real ph[DIMS] = {artra_map_point(nh[X], part->domain[X][MIN], part->d[X], part->ng[X]),
                                            artra_map_point(nh[Y], part->domain[Y][MIN], part->d[Y], part->ng[Y]),
                                            artra_map_point(nh[Z], part->domain[Z][MIN], part->d[Z], part->ng[Z])};
@@ -254,7 +266,7 @@ int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTa
                                 artra_map_node(pI[X], part->domain[X][MIN], part->dd[X], part->ng[X]),
                                 artra_map_node(pI[Y], part->domain[Y][MIN], part->dd[Y], part->ng[Y]),
                                 artra_map_node(pI[Z], part->domain[Z][MIN], part->dd[Z], part->ng[Z])};
-                            if (SearchStencil(space, nI, pI, 2, 0, 0, &sb, &gh->count, &wsum)) return 1;
+                            if (SearchStencil(space, nI, pI, 2, 0, kTypeDomain, &sb, &gh->count, &wsum)) return 1;
                             gh->kind = 0;
                             gh->wO = InverseDistance(part->tinyL, artra_dist2(pI, pO));
                             /* body surface velocity V + W x (pO - O) */
@@ -280,6 +292,62 @@ int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTa
     table->stencil_w = sb.w;
     table->nstencil = sb.n;
     return 0;
+}
+
+int artra_add_join(const Space *space, int k, int j, int i, ArtraGhostTable *joins)
+{
+    const Partition *part = &space->part;
+    if (NULL == joins->layer_start) {
+        joins->layers = 1;
+        joins->layer_start = calloc(2, sizeof(int));
+        if (NULL == joins->layer_start) return artra_set_error(1, "out of host memory");
+    }
+    if (joins->nghost == joins->ghost_cap) {
+        const int cap = joins->ghost_cap ? 2 * joins->ghost_cap : 256;
+        ArtraGhost *g = realloc(joins->ghosts, cap * sizeof(ArtraGhost));
+        if (NULL == g) return artra_set_error(1, "out of host memory for reconstructed nodes");
+        joins->ghosts = g;
+        joins->ghost_cap = cap;
+    }
+    ArtraGhost *gh = joins->ghosts + joins->nghost;
+    memset(gh, 0, sizeof *gh);
+    gh->node = artra_index(k, j, i, part->n[Y], part->n[X]);
+    gh->kind = 1;
+    gh->first = joins->nstencil;
+    StencilBuf sb = {joins->stencil_node, joins->stencil_w, joins->nstencil, joins->stencil_cap};
+    const int c[DIMS] = {i, j, k};
+    const Real p[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                          artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                          artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+    Real wsum = 0.0;
+    const int rc = SearchStencil(space, c, p, 2, 0, kTypeOriginal, &sb, &gh->count, &wsum);
+    joins->stencil_node = sb.node;
+    joins->stencil_w = sb.w;
+    joins->nstencil = sb.n;
+    joins->stencil_cap = sb.cap;
+    if (rc) return 1;
+    joins->nghost++;
+    joins->layer_start[1] = joins->nghost;
+    return 0;
+}
+
+void artra_apply_joins(Space *space, const Model *model, const ArtraGhostTable *joins, int tn)
+{
+    Node *node = space->node;
+    for (int q = 0; q < joins->nghost; ++q) {
+        const ArtraGhost *gh = joins->ghosts + q;
+        Real Uo[DIMUo] = {0.0}, Uoh[DIMUo];
+        Real wsum = 0.0;
+        for (int e = gh->first; e < gh->first + gh->count; ++e) {
+            const Real w = joins->stencil_w[e];
+            artra_map_primitive(model->gamma, model->gasR, node[joins->stencil_node[e]].U[tn], Uoh);
+            for (int c = 0; c < DIMUo; ++c) Uo[c] = Uo[c] + Uoh[c] * w;
+            wsum = wsum + w;
+        }
+        artra_normalize(DIMUo, wsum, Uo);
+        Uo[0] = Uo[4] / (Uo[5] * model->gasR);
+        artra_map_conservative(model->gamma, Uo, node[gh->node].U[tn]);
+    }
 }
 
 void artra_free_ghost_table(ArtraGhostTable *table)

@@ -88,6 +88,7 @@ def library():
         "artra_case_get_flags": (ctypes.c_int, [vp, ip]),
         "artra_case_clock": (ctypes.c_int, [vp, dp, ip]),
         "artra_case_forces": (ctypes.c_int, [vp, dp]),
+        "artra_case_bodies": (ctypes.c_int, [vp, dp]),
         "artra_calc_expression": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_double, dp]),
         "artra_hip_face_count": (ctypes.c_int, [vp, ctypes.c_int, lp]),
@@ -230,9 +231,17 @@ class Case:
     def forces(self):
         """Surface force of every body as [bodies][9] = Fp, Fv, Tt (the last
         integration's values)."""
-        out = np.zeros((64, 9))
-        n = library().artra_case_forces(self._h, _dptr(out))
-        return out[:n].copy()
+        n = library().artra_case_forces(self._h, None)
+        out = np.zeros((n, 9))
+        library().artra_case_forces(self._h, _dptr(out))
+        return out
+
+    def bodies(self):
+        """Motion state of every body as [bodies][15] = O, V, W, at, ar."""
+        n = library().artra_case_bodies(self._h, None)
+        out = np.zeros((n, 15))
+        library().artra_case_bodies(self._h, _dptr(out))
+        return out
 
     def interior(self, arr):
         """Restrict a [..., nz, ny, nx] array to the interior range."""

@@ -89,6 +89,31 @@ def geo_text(spheres=(), stls=()):
     return "\n".join(out) + "\n"
 
 
+def body_lines(O=(0, 0, 0), r=0.1, V=(0, 0, 0), W=(0, 0, 0), rho=1.0, T=-1, cf=0, mid=0,
+               at=(0, 0, 0), g=(0, 0, 0), ar=(0, 0, 0), to=0):
+    """The two state lines of a body in the order of data_stream.c:154-188: centre, radius,
+    velocity, angular velocity, density, wall temperature (< 0 adiabatic), friction (> 0 no-slip),
+    area, volume (computed), material (restitution 0, 0.25, ..., 1 for 0..4); then two ignored
+    accelerations, the powered acceleration, gravity, the powered angular acceleration and the time
+    the power ends (<= 0: never)."""
+    f = lambda v: ", ".join(str(q) for q in v)
+    return (f"{f(O)}, {r}, {f(V)}, {f(W)}, {rho}, {T}, {cf}, 0, 0, {mid}",
+            f"0, 0, 0, 0, 0, 0, {f(at)}, {f(g)}, {f(ar)}, {to}")
+
+
+def moving_geo_text(spheres=(), stls=()):
+    """Bodies with their full state: spheres as body_lines keyword dicts, stls as (file, dict)."""
+    out = ["count begin", str(len(spheres)), str(len(stls)), "count end"]
+    if spheres:
+        out += ["sphere state begin"] + [q for sp in spheres for q in body_lines(**sp)] + ["sphere state end"]
+    if stls:
+        out += ["polyhedron geometry begin"] + [st[0] for st in stls] + ["polyhedron geometry end"]
+        out += ["polyhedron state begin"] + [q for st in stls for q in body_lines(**st[1])]
+        out += ["polyhedron state end", "polyhedron transform begin"]
+        out += ["1, 1, 1, 0, 0, 0, 0, 0, 0" for _ in stls] + ["polyhedron transform end"]
+    return "\n".join(out) + "\n"
+
+
 def cylinder_stl(path, cx, cy, r, z0, z1, segments):
     """Closed cylinder along z: 2*segments side facets plus two fans."""
     f32 = lambda v: struct.unpack("<f", struct.pack("<f", v))[0]
@@ -243,6 +268,27 @@ def cases():
                                                 [inflow(1, 0.3, 0, 0, 1), OUTFLOW, noslip(0.004), slip(), OUTFLOW,
                                                  OUTFLOW]),
                                  geo=geo_text(spheres=[(0.05, 0.0, 0.0, 0.2, 0.0038, 1, 2.7)]))
+    # 4d. moving bodies (phase interaction on): the cylinder driven by the shock while it translates
+    # and spins (psi 1: no collisions), written with probes and surface force; two spheres that meet
+    # and rebound (psi 2: collisions), one powered for a while and under gravity, the other
+    # isothermal and no-slip, with image reconstruction in the first ghost layer only
+    c["cyl64_moving"] = dict(case=case_text(((0.5, 0.5, 0), (1.5, 1.5, 0.016), (64, 64, 4)), (0, 0.04, 0.6, 0, 2, 0),
+                                            (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
+                                            [inflow(*CYL_POST), OUTFLOW, slip(), slip(), PERIODIC, PERIODIC],
+                                            [plane((0.8, 0, 0), (-1, 0, 0), CYL_POST)],
+                                            probes=([(0.62, 1.0, 0.008)], [], 1, 1, (10, 10, 10, 10))),
+                             geo=moving_geo_text(stls=[("cylinder.stl", dict(r=0.1, V=(0.8, 0.5, 0), W=(0, 0, 3),
+                                                                             rho=2.7))]),
+                             stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
+    c["spheres_collide"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (24, 24, 24)),
+                                               (0, 0.08, 0.6, 0, 2, 0), (1, 1, 0, 0, 0, 2, 1), INVISCID, REF1,
+                                               ("1", "0", "0", "0", "1"), [OUTFLOW] * 6,
+                                               probes=([(0.0, 0.3, 0.0)], [], 1, 1, (5, 5, 5, 5))),
+                                geo=moving_geo_text(spheres=[
+                                    dict(O=(-0.2, 0, 0), r=0.15, V=(1.5, 0, 0), rho=1.5, mid=4,
+                                         at=(2, 0, 0), g=(0, -1, 0), to=0.02),
+                                    dict(O=(0.19, 0.02, 0), r=0.15, V=(-1.5, 0, 0), W=(0, 0, 2), rho=2.0,
+                                         T=0.004, cf=0.2, mid=2)]))
     # 5. scheme variants on a small 2D Riemann-like problem (10 steps each)
     quad = [box((-1, -1, -1), (0.5, 0.5, 1), (0.5, 0.1, -0.2, 0, 0.4)),
             sphere((0.7, 0.3, 0.008), 0.15, (2.0, 0, 0, 0, 3.0))]

@@ -52,11 +52,12 @@ void TreatImmersedBoundary(const int tn, Space *space, const Model *model);
 
 /* Classify every interior node against the bodies: did (inside body k),
  * fid (closest facet), lid and gst (interfacial and ghost layers), on the
- * host.  Replaces immersed_boundary.c:65-264 (immersed_boundary.h:32),
- * called at initialisation (initialization.c:47).  Stationary bodies only:
- * a model with moving bodies (psi != 0) is a fatal error, as everywhere in
- * this library.  Drops the device solver cached for space, whose tables
- * derive from the flags. */
+ * host, and reconstruct in node[].U[TO] the fluid nodes a moving body has
+ * uncovered (inverse-distance weighting over the nodes that were fluid
+ * before).  Replaces immersed_boundary.c:65-264 (immersed_boundary.h:32),
+ * called at initialisation (initialization.c:47) and after every move of
+ * the bodies (solid_dynamics.c:46).  The device solver cached for space
+ * rebuilds the tables that derive from the flags. */
 void ComputeGeometricField(Space *space, const Model *model);
 
 /* ----------------------------------------------------- 2. resident API */
@@ -85,7 +86,13 @@ int artra_hip_evolve(ArtraHipSolver *solver, Real dt);
 
 /* The time loop of EvolveSolution (solve.c:48-107) without output: advance
  * until time->now reaches time->end, time->stepC reaches time->stepN or
- * max_steps steps were taken.  dts (may be NULL) receives each step's dt. */
+ * max_steps steps were taken.  dts (may be NULL) receives each step's dt.
+ * With moving bodies (model->psi != 0) every step also takes the solid
+ * half-steps of solid_dynamics.c:37-48 before and after the fluid step
+ * (surface force on the device; kinematics, collisions unless psi == 1,
+ * motion and node classification on the host; the uncovered nodes'
+ * reconstruction and the ghost fill on the device), updating the bodies in
+ * space->geo and the node flags in space->node. */
 int artra_hip_march(ArtraHipSolver *solver, Time *time, int max_steps, int *steps_done, Real *dts);
 
 /* The time loop of artra_hip_march for callers that cannot allocate device
@@ -96,7 +103,8 @@ int artra_hip_march(ArtraHipSolver *solver, Time *time, int max_steps, int *step
  * time change nothing).  Its device scratch (CFL speed maxima, clock, the
  * record of the steps) is a static array of each GPU, not an allocation:
  * max_steps is at most ARTRA_NOALLOC_MAX_STEPS, kernel timing must be off,
- * and calls that overlap on one GPU share that array and race. */
+ * and calls that overlap on one GPU share that array and race.  Moving
+ * bodies (psi != 0) are refused: their solid steps rebuild device tables. */
 #define ARTRA_NOALLOC_MAX_STEPS 1024
 int artra_hip_march_noalloc(ArtraHipSolver *solver, Time *time, int max_steps, int *steps_done, Real *dts);
 
@@ -132,10 +140,13 @@ int artra_hip_write_space_data(ArtraHipSolver *solver, const Time *time);
 int artra_hip_solve(ArtraHipSolver *solver, Time *time);
 
 /* FP64 restart: level TO of every node (ghosts included), the clock
- * (now, stepC, dataC) and the output-instant recorder, exactly; a run
+ * (now, stepC, dataC) and the output-instant recorder, and with moving
+ * bodies their whole state (position, orientation, velocities,
+ * accelerations, forces) and the node classification, exactly; a run
  * continued from it is the uninterrupted run bit for bit (the reference
- * restarts from its "%.6g" field text, paraview_reader.c).  Reading
- * checks the grid size, uploads the field and sets time and solver. */
+ * restarts from its "%.6g" text, paraview_reader.c, data_stream.c:154-188).
+ * Reading checks the grid size and bodies, uploads the field and sets
+ * time, bodies, flags and solver. */
 int artra_hip_write_restart(ArtraHipSolver *solver, const Time *time, const char *path);
 int artra_hip_read_restart(ArtraHipSolver *solver, Time *time, const char *path);
 
@@ -315,8 +326,13 @@ int artra_case_get_flags(const ArtraCase *c, int *flags);
 /* current time and step count of the case's clock */
 int artra_case_clock(const ArtraCase *c, double *now, int *step);
 /* Surface force of every body as the last integration left it:
- * forces[n][9] = Fp[3], Fv[3], Tt[3]; returns the number of bodies. */
+ * forces[n][9] = Fp[3], Fv[3], Tt[3]; returns the number of bodies (and
+ * writes nothing when forces is NULL). */
 int artra_case_forces(const ArtraCase *c, Real *forces);
+/* Motion state of every body: state[n][15] = O[3], V[3], W[3], at[3],
+ * ar[3] (velocities and accelerations of level TO); returns the number of
+ * bodies (and writes nothing when state is NULL). */
+int artra_case_bodies(const ArtraCase *c, Real *state);
 /* Evaluate an initial-condition expression (t = 0); returns 0 on success. */
 int artra_calc_expression(const char *text, Real x, Real y, Real z, Real *ans);
 

@@ -171,9 +171,10 @@ static int RunCase(int argc, char **argv)
  * Preprocess and Solve (solve.c:38-46), i.e. initialisation and
  * EvolveSolution with every output it makes (probes, surface force, field
  * and geometry files, artracfd.log) written into the working directory.
- * Afterwards the final state and each body's surface force (Fp, Fv, Tt: 9
+ * Afterwards the final state, each body's surface force (Fp, Fv, Tt: 9
  * doubles per body, as the last IntegrateSurfaceForce of the loop left
- * them) are dumped.
+ * them) and each body's motion state (O, V, W, at, ar of level TO: 15
+ * doubles per body) are dumped.
  */
 static int SolveCase(int argc, char **argv)
 {
@@ -199,6 +200,18 @@ static int SolveCase(int argc, char **argv)
         fwrite(poly->Fp, sizeof(double), DIMS, fp);
         fwrite(poly->Fv, sizeof(double), DIMS, fp);
         fwrite(poly->Tt, sizeof(double), DIMS, fp);
+    }
+    fclose(fp);
+    snprintf(fname, sizeof fname, "%s_bodies.bin", prefix);
+    fp = fopen(fname, "wb");
+    if (!fp) Die("cannot open body file");
+    for (int n = 0; n < space.geo.totN; ++n) {
+        const Polyhedron *poly = space.geo.poly + n;
+        fwrite(poly->O, sizeof(double), DIMS, fp);
+        fwrite(poly->V[TO], sizeof(double), DIMS, fp);
+        fwrite(poly->W[TO], sizeof(double), DIMS, fp);
+        fwrite(poly->at[TO], sizeof(double), DIMS, fp);
+        fwrite(poly->ar[TO], sizeof(double), DIMS, fp);
     }
     fclose(fp);
     fclose(stdout);

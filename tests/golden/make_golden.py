@@ -15,7 +15,8 @@ compressed numpy archives:
   solve_<case>.npz  the reference's whole Solve on a case with probes: the
                text of every probe and log file it writes, the SHA-256 and
                size of its field and geometry files, the final surface
-               force of each body (Fp, Fv, Tt) and the final interior field
+               force of each body (Fp, Fv, Tt), its final motion state (O,
+               V, W, at, ar) and the final interior field
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -86,7 +87,8 @@ def make_vectors(work, seed=20261202, count=400):
     print("vectors", count, nf)
 
 
-SOLVE_CASES = ("cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight")
+SOLVE_CASES = ("cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight", "cyl64_moving",
+               "spheres_collide")
 SOLVE_TEXT = (".csv", ".log", ".pvd", ".case", ".state")   # stored as text; other outputs (.vts, .vtp, EnSight binaries) by hash
 
 
@@ -123,6 +125,7 @@ def make_solve(name, work):
                         hash_sizes=np.array([len(data) for _, data in hashed], dtype=np.int64),
                         hash_lines=np.array([data.count(b"\n") for _, data in hashed], dtype=np.int64),
                         force=np.fromfile(os.path.join(d, "out_force.bin")).reshape(-1, 9),
+                        bodies=np.fromfile(os.path.join(d, "out_bodies.bin")).reshape(-1, 15),
                         steps=info["steps"], now=info["now"],
                         U_final=np.ascontiguousarray(interior(Uf, pin)))
     print(name, info, len(texts), "text files", len(hashed), "hashed files")

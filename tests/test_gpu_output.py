@@ -1,8 +1,11 @@
 """The solver loop on the device with its output cadence (artra_hip_solve):
 every probe, surface-force, field, geometry and log file it writes against
 the files the compiled reference's own Solve wrote for the same case
-(fixtures tests/golden/solve_*.npz), and the surface force itself
-(solid_dynamics.c:50-171) as a device reduction."""
+(fixtures tests/golden/solve_*.npz), the surface force itself
+(solid_dynamics.c:50-171) as a device reduction, and moving bodies
+(solid_dynamics.c:37-48: kinematics, collisions, motion, the
+reconstruction of uncovered nodes) against the reference's final body
+states."""
 import hashlib
 import os
 import re
@@ -42,31 +45,42 @@ def _solve(name, tmp_path, exact):
         os.chdir(here)
     U = case.interior(solver.field()).copy()
     F = case.forces()
+    B = case.bodies()
     now, steps = case.clock()
     solver.close()
     case.close()
-    return out, U, F, now, steps
+    return out, U, F, B, now, steps
 
 
-@pytest.mark.parametrize("name", ["cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight"])
+# inviscid cases: exact arithmetic reproduces the reference bit for bit
+BITWISE = ("cyl64_probes", "cyl64_ensight", "cyl64_moving", "spheres_collide")
+
+
+@pytest.mark.parametrize("name", ["cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight",
+                                  "cyl64_moving", "spheres_collide"])
 @pytest.mark.parametrize("exact", [True, False])
 def test_solve_writes_the_reference_outputs(name, exact, tmp_path):
-    """Exact arithmetic on the inviscid cylinder: every text file
-    byte-identical and every field and geometry file the same SHA-256 as
-    the reference's (ParaView and EnSight Gold writers).  The viscous sphere (the device's pow differs from
-    glibc's in the last bit) and the fast arithmetic: the same files with
-    the same numbers to the printed precision."""
+    """Exact arithmetic on the inviscid cases: every text file
+    byte-identical, every field and geometry file the same SHA-256 as the
+    reference's (ParaView and EnSight Gold writers) and, for the moving
+    bodies (a translating, spinning STL cylinder; two spheres that collide),
+    the same final positions, velocities and accelerations.  The viscous
+    sphere (the device's pow differs from glibc's in the last bit) and the
+    fast arithmetic: the same files with the same numbers to the printed
+    precision."""
     print(f"solve {name} exact={exact}", flush=True)
     fx = golden("solve_" + name)
-    out, U, F, now, steps = _solve(name, tmp_path, exact)
+    out, U, F, B, now, steps = _solve(name, tmp_path, exact)
     assert steps == int(fx["steps"]) and now == float(fx["now"])
-    bitwise = exact and name.startswith("cyl64")
+    bitwise = exact and name in BITWISE
     err = rel_l2(U, fx["U_final"])
     assert max(err.values()) <= (0.0 if bitwise else 1e-10), err
     if bitwise:
         assert np.array_equal(F, fx["force"])
+        assert np.array_equal(B, fx["bodies"])
     else:
         np.testing.assert_allclose(F, fx["force"], rtol=1e-8, atol=1e-12 * np.abs(fx["force"]).max())
+        np.testing.assert_allclose(B, fx["bodies"], rtol=1e-8, atol=1e-12 * np.abs(fx["bodies"]).max())
     texts = _text_files(fx)
     written = set(os.listdir(out))
     hashed = [n.decode() for n in fx["hash_names"]]
@@ -78,7 +92,10 @@ def test_solve_writes_the_reference_outputs(name, exact, tmp_path):
             continue
         a, b = _numbers(got), _numbers(ref)
         assert a.shape == b.shape, fname
-        np.testing.assert_allclose(a, b, rtol=2e-6, atol=1e-12, err_msg=fname)
+        # "%.6g" numbers: one unit in the sixth digit, or 1e-10 absolute for
+        # the near-zero values (velocity components that cancel) where the
+        # field's own 1e-10 bound decides the printed digits
+        np.testing.assert_allclose(a, b, rtol=2e-6, atol=1e-10, err_msg=fname)
     for fname, sha, size, lines in zip(hashed, fx["hash_sha256"], fx["hash_sizes"], fx["hash_lines"]):
         data = open(out / fname, "rb").read()
         if bitwise:
@@ -113,12 +130,13 @@ def test_surface_force_call_matches_the_oracle(oracle_mod):
     assert np.abs(F[0, :2]).max() > 0.0
 
 
-@pytest.mark.parametrize("name,first,second", [("cyl64", 15, 25), ("shockvortex64", 6, 9), ("tgv16", 7, 8)])
+@pytest.mark.parametrize("name,first,second", [("cyl64", 15, 25), ("shockvortex64", 6, 9), ("tgv16", 7, 8),
+                                               ("spheres_collide", 3, 5)])
 def test_restart_continues_the_run_bit_for_bit(name, first, second, tmp_path):
     """March `first` steps, write the FP64 restart, continue `second` steps;
     a fresh solver reading the restart and marching `second` steps ends
-    with the same field, clock and dt sequence bit for bit (immersed body,
-    fused sweeps, viscous flux)."""
+    with the same field, clock, dt sequence and bodies bit for bit
+    (immersed body, fused sweeps, viscous flux, colliding moving bodies)."""
     from artracfd_amd import Case, Solver
     print(f"restart {name} {first}+{second}", flush=True)
     path = str(tmp_path / "restart.bin")
@@ -129,6 +147,7 @@ def test_restart_continues_the_run_bit_for_bit(name, first, second, tmp_path):
     _, dts_a = solver.march(second)
     Ua = case.interior(solver.field()).copy()
     clock_a = case.clock()
+    bodies_a = case.bodies()
     solver.close()
     case.close()
     case = Case(case_dir(name))
@@ -138,11 +157,13 @@ def test_restart_continues_the_run_bit_for_bit(name, first, second, tmp_path):
     _, dts_b = solver.march(second)
     Ub = case.interior(solver.field()).copy()
     clock_b = case.clock()
+    bodies_b = case.bodies()
     solver.close()
     case.close()
     assert clock_b == clock_a
     assert np.array_equal(dts_b, dts_a)
     assert np.array_equal(Ub, Ua)
+    assert np.array_equal(bodies_b, bodies_a)
 
 
 def test_restart_refuses_another_grid(tmp_path):

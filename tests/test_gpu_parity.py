@@ -161,6 +161,36 @@ def test_reference_driver_through_dropin_symbols(name, tmp_path, oracle_mod):
     assert np.array_equal(Uf[:, z0:z1, y0:y1, x0:x1], g["U_final"])
 
 
+@pytest.mark.skipif(not os.path.exists(DROPIN), reason="the drop-in driver is built only where the reference is")
+def test_reference_solve_with_moving_bodies_through_dropin_symbols(tmp_path, oracle_mod):
+    """The reference's whole Solve with two colliding spheres, its own solid
+    dynamics (surface force, kinematics, collisions, motion) calling the
+    library's ComputeGeometricField (classification and reconstruction of
+    uncovered nodes on the caller's node array) and TreatImmersedBoundary,
+    and its time loop calling EvolveFluidDynamics on the GPU: in exact
+    arithmetic the final field, surface forces and body states are the
+    reference's CPU run's bit for bit."""
+    import shutil
+    import subprocess
+    name = "spheres_collide"
+    print(f"reference solve through the drop-in symbols: {name}", flush=True)
+    fx = golden("solve_" + name)
+    work = tmp_path / name
+    shutil.copytree(case_dir(name), work)
+    env = dict(os.environ, ARTRA_EXACT="1")
+    out = subprocess.run([DROPIN, "solve", "out"], cwd=work, env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    hdr, _, Uf = oracle_mod.read_dump(str(work / "out_final.bin"))
+    assert hdr["steps"] == int(fx["steps"])
+    from artracfd_amd import Case
+    case = Case(case_dir(name))
+    U = case.interior(Uf)
+    case.close()
+    assert np.array_equal(U, fx["U_final"])
+    assert np.array_equal(np.fromfile(str(work / "out_force.bin")).reshape(-1, 9), fx["force"])
+    assert np.array_equal(np.fromfile(str(work / "out_bodies.bin")).reshape(-1, 15), fx["bodies"])
+
+
 def test_compute_dt_matches_oracle(oracle_mod):
     from artracfd_amd import Case, Solver
     case = Case(case_dir("cyl64"))
