@@ -82,14 +82,54 @@ void artra_transform_polyhedron(const Real O[DIMS], const Real scale[DIMS], cons
 void artra_compute_geometry_parameters(int collapse, Geometry *geo);
 Real artra_point_triangle_distance(const Real p[DIMS], const Real v0[DIMS], const Real e01[DIMS],
                                    const Real e02[DIMS], Real para[DIMS]);
-int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, int *fid);
+/* Bounds that speed up the closest-facet search of a triangulated body in
+ * its current position: the box of every facet, of every block of
+ * ARTRA_FACET_BLOCK consecutive facets and of all facets. */
+#define ARTRA_FACET_BLOCK 32
+typedef struct {
+    int blocks;
+    Real (*facet)[2 * DIMS];   /* min xyz, max xyz */
+    Real (*block)[2 * DIMS];
+    Real all[2 * DIMS];
+} ArtraFacetBounds;
+int artra_facet_bounds(const Polyhedron *poly, ArtraFacetBounds *bounds);
+void artra_free_facet_bounds(ArtraFacetBounds *bounds);
+/* Inside-or-on test of a triangulated body with the closest facet in *fid
+ * (computational_geometry.c:463-506); bounds (may be NULL: every facet is
+ * measured) only skip facets that cannot be the closest. */
+int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, const ArtraFacetBounds *bounds, int *fid);
 void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS],
                                   Real pm[DIMS], Real N[DIMS]);
 /* Node classification did/fid/lid/gst (immersed_boundary.c:65-264).  A
  * fluid node that was inside a body before the call is marked fid = -1 and,
  * when joins is not NULL, listed there with its reconstruction stencil
  * (artra_add_join); the caller evaluates the list.  Returns 0 or an error. */
-int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins);
+int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins,
+                                  const int (*region)[LIMIT]);
+
+/*
+ * Classification after a move, restricted to where it can change anything.
+ * The two passes of ComputeGeometricField over every interior node only
+ * change nodes that a moving body covers before (reset of its interfacial
+ * nodes) or after the move (domain field), the interfacial state of nodes
+ * within gl layers of those, and the reconstruction mark of the nodes the
+ * previous classification reconstructed; every other node keeps its flags.
+ * A scope tracks, per body, the node box of its nodes (extent) and the box of
+ * the last reconstructed nodes; artra_scope_region gives, after the bodies
+ * moved, the box that the passes must cover (region argument above), and
+ * artra_scope_update refreshes the scope from the new classification (the
+ * moving bodies' nodes all lie inside that region).
+ */
+typedef struct {
+    int bodies;
+    int (*extent)[DIMS][LIMIT];   /* [bodies]: node box [min, max) per axis; empty when min >= max */
+    int join[DIMS][LIMIT];
+} ArtraGeometryScope;
+int artra_scope_init(const Space *space, ArtraGeometryScope *scope);
+void artra_scope_free(ArtraGeometryScope *scope);
+void artra_scope_region(const Space *space, const ArtraGeometryScope *scope, int region[DIMS][LIMIT]);
+void artra_scope_update(const Space *space, ArtraGeometryScope *scope, const ArtraGhostTable *joins,
+                        const int region[DIMS][LIMIT]);
 void artra_identify_geometry_state(Geometry *geo);
 
 /* ------------------------------------------------- device-side tables */

@@ -179,6 +179,10 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         artra_hip_detach(s);
         return 1;
     }
+    if (0 != model->psi && artra_scope_init(space, &s->scope)) {
+        artra_hip_detach(s);
+        return 1;
+    }
     const char *exact = getenv("ARTRA_EXACT");
     if (exact && exact[0] == '1') artra_dev_set_exact(s->dev, 1);
     *out = s;
@@ -192,6 +196,7 @@ int artra_hip_detach(ArtraHipSolver *s)
     artra_free_ghost_table(&s->ghosts);
     free(s->staging);
     free(s->force);
+    artra_scope_free(&s->scope);
     free(s);
     return 0;
 }
@@ -534,7 +539,7 @@ void ComputeGeometricField(Space *space, const Model *model)
 {
     ArtraGhostTable joins;
     memset(&joins, 0, sizeof joins);
-    int rc = artra_compute_geometric_field(space, model, &joins);
+    int rc = artra_compute_geometric_field(space, model, &joins, NULL);
     /* the nodes a moving body uncovered are reconstructed in the caller's
      * node array, which is authoritative between drop-in calls */
     if (!rc) artra_apply_joins(space, model, &joins, TO);

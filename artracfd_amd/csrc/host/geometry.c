@@ -17,6 +17,7 @@
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
+#include <limits.h>
 #include <string.h>
 
 #include "artra_host.h"
@@ -422,19 +423,152 @@ static void ClosestOnFacet(const Real p[DIMS], int fid, const Polyhedron *poly, 
     for (int s = 0; s < DIMS; ++s) N[s] = normal[s];
 }
 
-int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, int *fid)
+/* ------------------------------------------------ closest-facet search */
+/*
+ * The closest facet is the first facet, in facet order, at the minimum of
+ * the computed squared distance (computational_geometry.c:486-494).  With
+ * bounds, facets that cannot reach that minimum are skipped: a facet whose
+ * box lies farther than a distance already reached (plus a slack of 1e-12
+ * times the squared extent around the point, far above the rounding of
+ * either quantity) can neither lower the minimum nor tie it, so the
+ * remaining facets, still visited in facet order with the same strict
+ * comparison, give the same facet as the full scan.  Blocks of consecutive
+ * facets carry their union box, so most facets are ruled out by block.
+ */
+static Real BoxDistance2(const Real p[DIMS], const Real b[2 * DIMS])
 {
-    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
-    Real best = FLT_MAX;
-    int cid = 0;
-    for (int n = 0; n < poly->faceN; ++n) {
-        TriangleOf(n, poly, v0, v1, v2, e01, e02);
-        const Real d2 = artra_point_triangle_distance(p, v0, e01, e02, para);
-        if (best > d2) { /* the first facet at the minimum wins */
-            best = d2;
-            cid = n;
+    Real d2 = 0.0;
+    for (int s = 0; s < DIMS; ++s) {
+        Real g = 0.0;
+        if (p[s] < b[s]) g = b[s] - p[s];
+        else if (p[s] > b[DIMS + s]) g = p[s] - b[DIMS + s];
+        d2 = d2 + g * g;
+    }
+    return d2;
+}
+
+static void GrowBox(Real b[2 * DIMS], const Real v[DIMS])
+{
+    for (int s = 0; s < DIMS; ++s) {
+        if (v[s] < b[s]) b[s] = v[s];
+        if (v[s] > b[DIMS + s]) b[DIMS + s] = v[s];
+    }
+}
+
+int artra_facet_bounds(const Polyhedron *poly, ArtraFacetBounds *fb)
+{
+    memset(fb, 0, sizeof *fb);
+    if (0 >= poly->faceN) return 0;
+    fb->blocks = (poly->faceN + ARTRA_FACET_BLOCK - 1) / ARTRA_FACET_BLOCK;
+    fb->facet = malloc((size_t)poly->faceN * sizeof(*fb->facet));
+    fb->block = malloc((size_t)fb->blocks * sizeof(*fb->block));
+    if (!fb->facet || !fb->block) {
+        artra_free_facet_bounds(fb);
+        return artra_set_error(1, "out of host memory for facet bounds");
+    }
+    for (int s = 0; s < DIMS; ++s) {
+        fb->all[s] = FLT_MAX;
+        fb->all[DIMS + s] = -FLT_MAX;
+    }
+    for (int b = 0; b < fb->blocks; ++b) {
+        Real *bb = fb->block[b];
+        for (int s = 0; s < DIMS; ++s) {
+            bb[s] = FLT_MAX;
+            bb[DIMS + s] = -FLT_MAX;
+        }
+        const int end = (b + 1) * ARTRA_FACET_BLOCK < poly->faceN ? (b + 1) * ARTRA_FACET_BLOCK : poly->faceN;
+        for (int n = b * ARTRA_FACET_BLOCK; n < end; ++n) {
+            Real *fx = fb->facet[n];
+            for (int s = 0; s < DIMS; ++s) {
+                fx[s] = FLT_MAX;
+                fx[DIMS + s] = -FLT_MAX;
+            }
+            for (int q = 0; q < 3; ++q) {
+                GrowBox(fx, poly->v[poly->f[n][q]]);
+                GrowBox(bb, poly->v[poly->f[n][q]]);
+                GrowBox(fb->all, poly->v[poly->f[n][q]]);
+            }
         }
     }
+    return 0;
+}
+
+void artra_free_facet_bounds(ArtraFacetBounds *fb)
+{
+    free(fb->facet);
+    free(fb->block);
+    memset(fb, 0, sizeof *fb);
+}
+
+static Real FacetDistance2(const Real p[DIMS], int n, const Polyhedron *poly)
+{
+    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
+    TriangleOf(n, poly, v0, v1, v2, e01, e02);
+    return artra_point_triangle_distance(p, v0, e01, e02, para);
+}
+
+static int ClosestFacet(const Real p[DIMS], const Polyhedron *poly, const ArtraFacetBounds *fb)
+{
+    Real best = FLT_MAX;
+    int cid = 0;
+    if (NULL == fb || NULL == fb->facet) {
+        for (int n = 0; n < poly->faceN; ++n) {
+            const Real d2 = FacetDistance2(p, n, poly);
+            if (best > d2) { /* the first facet at the minimum wins */
+                best = d2;
+                cid = n;
+            }
+        }
+        return cid;
+    }
+    /* a distance that some facet reaches: that of the facet whose box is
+     * nearest within the block whose box is nearest */
+    int near = 0;
+    Real nearLb = FLT_MAX;
+    for (int b = 0; b < fb->blocks; ++b) {
+        const Real lb = BoxDistance2(p, fb->block[b]);
+        if (lb < nearLb) {
+            nearLb = lb;
+            near = b;
+        }
+    }
+    const int nearEnd = (near + 1) * ARTRA_FACET_BLOCK < poly->faceN ? (near + 1) * ARTRA_FACET_BLOCK : poly->faceN;
+    int probe = near * ARTRA_FACET_BLOCK;
+    nearLb = FLT_MAX;
+    for (int n = near * ARTRA_FACET_BLOCK; n < nearEnd; ++n) {
+        const Real lb = BoxDistance2(p, fb->facet[n]);
+        if (lb < nearLb) {
+            nearLb = lb;
+            probe = n;
+        }
+    }
+    const Real reach = FacetDistance2(p, probe, poly);
+    /* slack: 1e-12 of the squared distance to the far corner of all facets */
+    Real extent = 0.0;
+    for (int s = 0; s < DIMS; ++s) {
+        const Real g = fabs(p[s] - fb->all[s]) > fabs(p[s] - fb->all[DIMS + s]) ? fabs(p[s] - fb->all[s])
+                                                                                 : fabs(p[s] - fb->all[DIMS + s]);
+        extent = extent + g * g;
+    }
+    const Real bound = reach + 1.0e-12 * extent;
+    for (int b = 0; b < fb->blocks; ++b) {
+        if (BoxDistance2(p, fb->block[b]) > bound) continue;
+        const int end = (b + 1) * ARTRA_FACET_BLOCK < poly->faceN ? (b + 1) * ARTRA_FACET_BLOCK : poly->faceN;
+        for (int n = b * ARTRA_FACET_BLOCK; n < end; ++n) {
+            if (BoxDistance2(p, fb->facet[n]) > bound) continue;
+            const Real d2 = FacetDistance2(p, n, poly);
+            if (best > d2) {
+                best = d2;
+                cid = n;
+            }
+        }
+    }
+    return cid;
+}
+
+int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, const ArtraFacetBounds *bounds, int *fid)
+{
+    const int cid = ClosestFacet(p, poly, bounds);
     *fid = cid;
     Real pi[DIMS], N[DIMS];
     ClosestOnFacet(p, cid, poly, pi, N);
@@ -477,16 +611,27 @@ static int InterState(const Partition *part, const Node *node, int k, int j, int
     return 0;
 }
 
-int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins)
+int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTable *joins,
+                                  const int (*region)[LIMIT])
 {
     (void)model;
     const Partition *part = &space->part;
     Node *node = space->node;
     const Geometry *geo = &space->geo;
+    /* the node range of the two whole-field passes */
+    int lo[DIMS], hi[DIMS];
+    for (int s = 0; s < DIMS; ++s) {
+        lo[s] = part->ns[PIN][s][MIN];
+        hi[s] = part->ns[PIN][s][MAX];
+        if (region) {
+            if (region[s][MIN] > lo[s]) lo[s] = region[s][MIN];
+            if (region[s][MAX] < hi[s]) hi[s] = region[s][MAX];
+        }
+    }
     /* keep the previous domain field in gst (immersed_boundary.c:72-113) */
-    for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
-        for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
-            for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i) {
                 Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
                 const int gid = nd->did;
                 nd->gst = nd->did;
@@ -501,6 +646,8 @@ int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTa
     for (int n = 0; n < geo->totN; ++n) {
         const Polyhedron *poly = geo->poly + n;
         if (1 == poly->state) continue;
+        ArtraFacetBounds bounds;
+        if (artra_facet_bounds(poly, &bounds)) return 1;
         int box[DIMS][LIMIT];
         for (int s = 0; s < DIMS; ++s) {
             const int lo = part->ns[PIN][s][MIN], hi = part->ns[PIN][s][MAX];
@@ -509,35 +656,41 @@ int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTa
             box[s][MAX] = artra_confine(artra_map_node(poly->box[s][MAX], part->domain[s][MIN], part->dd[s],
                                                        part->ng[s]), lo, hi) + 1;
         }
-        for (int k = box[Z][MIN]; k < box[Z][MAX]; ++k)
-            for (int j = box[Y][MIN]; j < box[Y][MAX]; ++j)
-                for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
-                    Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
-                    if (0 != nd->did) continue;
-                    const Real p[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
-                                          artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
-                                          artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
-                    if (0 >= poly->faceN) {
-                        if (poly->r * poly->r >= artra_dist2(poly->O, p)) {
-                            nd->did = n + 1;
-                            nd->fid = 0;
-                        }
-                    } else {
-                        int fid = 0;
-                        if (artra_point_in_polyhedron(p, poly, &fid)) {
-                            nd->did = n + 1;
-                            nd->fid = fid;
-                        }
+        /* the nodes of one body's box are independent of each other */
+        const int rowsY = box[Y][MAX] - box[Y][MIN];
+        const int rows = (box[Z][MAX] - box[Z][MIN]) * (rowsY > 0 ? rowsY : 0);
+#pragma omp parallel for schedule(dynamic, 4) if (rows > 16)
+        for (int row = 0; row < rows; ++row) {
+            const int k = box[Z][MIN] + row / rowsY, j = box[Y][MIN] + row % rowsY;
+            for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
+                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                if (0 != nd->did) continue;
+                const Real p[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                                      artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                                      artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+                if (0 >= poly->faceN) {
+                    if (poly->r * poly->r >= artra_dist2(poly->O, p)) {
+                        nd->did = n + 1;
+                        nd->fid = 0;
+                    }
+                } else {
+                    int fid = 0;
+                    if (artra_point_in_polyhedron(p, poly, &bounds, &fid)) {
+                        nd->did = n + 1;
+                        nd->fid = fid;
                     }
                 }
+            }
+        }
+        artra_free_facet_bounds(&bounds);
     }
     /* interfacial and ghost layers (immersed_boundary.c:183-225); a fluid
      * node that a moving body has uncovered (gst, the previous domain field,
      * differs from did = 0) is listed for reconstruction with the stencil the
      * flags give at this point of the pass, then marked */
-    for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
-        for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
-            for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i) {
                 Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
                 if (nd->gst != nd->did && 0 == nd->did) {
                     if (NULL == joins) return artra_set_error(1, "a body uncovered node (%d, %d, %d)", k, j, i);
@@ -551,4 +704,114 @@ int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTa
                 if (0 < nd->lid) nd->gst = InterState(part, node, k, j, i, 0, 1);
             }
     return 0;
+}
+
+/* ------------------------------------------------ classification scope */
+static void EmptyBox(int box[DIMS][LIMIT])
+{
+    for (int s = 0; s < DIMS; ++s) {
+        box[s][MIN] = INT_MAX;
+        box[s][MAX] = INT_MIN;
+    }
+}
+
+static void AddNode(int box[DIMS][LIMIT], int k, int j, int i)
+{
+    const int at[DIMS] = {i, j, k};
+    for (int s = 0; s < DIMS; ++s) {
+        if (at[s] < box[s][MIN]) box[s][MIN] = at[s];
+        if (at[s] + 1 > box[s][MAX]) box[s][MAX] = at[s] + 1;
+    }
+}
+
+static void AddBox(int box[DIMS][LIMIT], const int add[DIMS][LIMIT])
+{
+    for (int s = 0; s < DIMS; ++s) {
+        if (add[s][MIN] >= add[s][MAX]) return;
+    }
+    for (int s = 0; s < DIMS; ++s) {
+        if (add[s][MIN] < box[s][MIN]) box[s][MIN] = add[s][MIN];
+        if (add[s][MAX] > box[s][MAX]) box[s][MAX] = add[s][MAX];
+    }
+}
+
+int artra_scope_init(const Space *space, ArtraGeometryScope *sc)
+{
+    const Partition *part = &space->part;
+    const Geometry *geo = &space->geo;
+    memset(sc, 0, sizeof *sc);
+    sc->bodies = geo->totN;
+    sc->extent = malloc((size_t)(geo->totN > 0 ? geo->totN : 1) * sizeof(*sc->extent));
+    if (NULL == sc->extent) return artra_set_error(1, "out of host memory");
+    for (int n = 0; n < geo->totN; ++n) EmptyBox(sc->extent[n]);
+    EmptyBox(sc->join);
+    for (int k = part->ns[PIN][Z][MIN]; k < part->ns[PIN][Z][MAX]; ++k)
+        for (int j = part->ns[PIN][Y][MIN]; j < part->ns[PIN][Y][MAX]; ++j)
+            for (int i = part->ns[PIN][X][MIN]; i < part->ns[PIN][X][MAX]; ++i) {
+                const Node *nd = space->node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                if (0 < nd->did && nd->did <= geo->totN) AddNode(sc->extent[nd->did - 1], k, j, i);
+                if (0 == nd->did && 0 > nd->fid) AddNode(sc->join, k, j, i);
+            }
+    return 0;
+}
+
+void artra_scope_free(ArtraGeometryScope *sc)
+{
+    free(sc->extent);
+    memset(sc, 0, sizeof *sc);
+}
+
+void artra_scope_region(const Space *space, const ArtraGeometryScope *sc, int region[DIMS][LIMIT])
+{
+    const Partition *part = &space->part;
+    const Geometry *geo = &space->geo;
+    EmptyBox(region);
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        if (1 == poly->state) continue;
+        int box[DIMS][LIMIT];
+        for (int s = 0; s < DIMS; ++s) {
+            const int lo = part->ns[PIN][s][MIN], hi = part->ns[PIN][s][MAX];
+            box[s][MIN] = artra_confine(artra_map_node(poly->box[s][MIN], part->domain[s][MIN], part->dd[s],
+                                                       part->ng[s]), lo, hi);
+            box[s][MAX] = artra_confine(artra_map_node(poly->box[s][MAX], part->domain[s][MIN], part->dd[s],
+                                                       part->ng[s]), lo, hi) + 1;
+        }
+        AddBox(region, box);
+        AddBox(region, (const int (*)[LIMIT])sc->extent[n]);
+    }
+    int empty = 0;
+    for (int s = 0; s < DIMS; ++s) empty |= (region[s][MIN] >= region[s][MAX]);
+    for (int s = 0; s < DIMS && !empty; ++s) {
+        region[s][MIN] -= part->gl;
+        region[s][MAX] += part->gl;
+    }
+    AddBox(region, (const int (*)[LIMIT])sc->join);
+}
+
+void artra_scope_update(const Space *space, ArtraGeometryScope *sc, const ArtraGhostTable *joins,
+                        const int region[DIMS][LIMIT])
+{
+    const Partition *part = &space->part;
+    const Geometry *geo = &space->geo;
+    int lo[DIMS], hi[DIMS];
+    for (int s = 0; s < DIMS; ++s) {
+        lo[s] = (region[s][MIN] > part->ns[PIN][s][MIN]) ? region[s][MIN] : part->ns[PIN][s][MIN];
+        hi[s] = (region[s][MAX] < part->ns[PIN][s][MAX]) ? region[s][MAX] : part->ns[PIN][s][MAX];
+    }
+    for (int n = 0; n < geo->totN; ++n) {
+        if (1 != geo->poly[n].state) EmptyBox(sc->extent[n]);
+    }
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i) {
+                const int did = space->node[artra_index(k, j, i, part->n[Y], part->n[X])].did;
+                if (0 < did && did <= geo->totN && 1 != geo->poly[did - 1].state) AddNode(sc->extent[did - 1], k, j, i);
+            }
+    EmptyBox(sc->join);
+    for (int q = 0; joins && q < joins->nghost; ++q) {
+        const int idx = joins->ghosts[q].node;
+        const int i = idx % part->n[X], j = (idx / part->n[X]) % part->n[Y], k = idx / (part->n[X] * part->n[Y]);
+        AddNode(sc->join, k, j, i);
+    }
 }

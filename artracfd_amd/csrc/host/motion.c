@@ -294,9 +294,13 @@ int artra_solid_step(ArtraHipSolver *s, Real now, Real dt)
     if (Kinematics(space, now, dt)) return 1;
     if (1 != model->psi && Collisions(space)) return 1;
     Motion(space, dt);
+    /* the classification only where the bodies were or are now */
+    int region[DIMS][LIMIT];
+    artra_scope_region(space, &s->scope, region);
     ArtraGhostTable joins;
     memset(&joins, 0, sizeof joins);
-    int rc = artra_compute_geometric_field(space, model, &joins);
+    int rc = artra_compute_geometric_field(space, model, &joins, (const int (*)[LIMIT])region);
+    if (!rc) artra_scope_update(space, &s->scope, &joins, (const int (*)[LIMIT])region);
     if (!rc) rc = artra_hip_refresh_geometry(s);
     if (!rc && artra_dev_reconstruct(s->dev, TO, &joins)) rc = artra_dev_failed();
     if (!rc && artra_dev_treat_immersed(s->dev, TO)) rc = artra_dev_failed();

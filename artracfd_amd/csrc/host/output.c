@@ -1229,7 +1229,10 @@ int artra_hip_read_restart(ArtraHipSolver *s, Time *time, const char *path)
     for (long q = 0; q < s->N; ++q) {
         for (int c = 0; c < DIMU; ++c) node[q].U[TO][c] = s->staging[c * s->N + q];
     }
-    if (moving && artra_hip_refresh_geometry(s)) return 1;
+    if (moving) {
+        artra_scope_free(&s->scope);
+        if (artra_scope_init(s->space, &s->scope) || artra_hip_refresh_geometry(s)) return 1;
+    }
     time->stepC = hdr[6];
     time->dataC = hdr[7];
     time->now = clk[0];

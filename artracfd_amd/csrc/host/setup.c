@@ -337,7 +337,7 @@ int artra_initialize_domain(const char *dir, Time *time, Space *space, Model *mo
     if (InitializeField(space, model)) return 1;
     if (InitializeGeometry(dir, &space->geo)) return 1;
     artra_compute_geometry_parameters(space->part.collapse, &space->geo);
-    if (artra_compute_geometric_field(space, model, NULL)) return 1;
+    if (artra_compute_geometric_field(space, model, NULL, NULL)) return 1;
     artra_identify_geometry_state(&space->geo);
     return 0;
 }

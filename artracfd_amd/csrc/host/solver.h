@@ -28,6 +28,7 @@ struct ArtraHipSolver {
     Real rcInt;               /* time since the last output instant (solve.c:64) */
     int forceBodies;          /* bodies in force[] */
     ArtraForceBodyInfo *force;
+    ArtraGeometryScope scope; /* moving bodies: where the next classification must look */
 };
 
 /* Build the surface-force table of the solver's bodies and hand it to the

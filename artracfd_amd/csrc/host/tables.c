@@ -24,6 +24,10 @@
 
 #include "artra_host.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 /* --------------------------------------------------- external boundary */
 int artra_bc_boxes(const Space *space, ArtraBcBox *out, int cap)
 {
@@ -213,6 +217,60 @@ static int SearchStencil(const Space *space, const int c[DIMS], const Real p[DIM
     return 0;
 }
 
+/* A ghost node found by the scan, in table order. */
+typedef struct {
+    int node, k, j, i, body;
+} GhostSite;
+
+/* One ghost's entry and stencil (appended to sb; gh->first is the offset
+ * in sb). */
+static int BuildGhost(const Space *space, const Model *model, const GhostSite *site, int r, StencilBuf *sb,
+                      ArtraGhost *gh)
+{
+    const Partition *part = &space->part;
+    const Polyhedron *poly = space->geo.poly + site->body;
+    const Node *nd = space->node + site->node;
+    memset(gh, 0, sizeof *gh);
+    gh->node = site->node;
+    gh->first = sb->n;
+    const Real pG[DIMS] = {artra_map_point(site->i, part->domain[X][MIN], part->d[X], part->ng[X]),
+                           artra_map_point(site->j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
+                           artra_map_point(site->k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
+    Real wsum = 0.0;
+    if (model->ibmLayer >= r) {
+        Real pO[DIMS], pI[DIMS], N[DIMS];
+        artra_compute_geometric_data(pG, nd->fid, poly, pO, pI, N);
+        const int nI[DIMS] = {artra_map_node(pI[X], part->domain[X][MIN], part->dd[X], part->ng[X]),
+                              artra_map_node(pI[Y], part->domain[Y][MIN], part->dd[Y], part->ng[Y]),
+                              artra_map_node(pI[Z], part->domain[Z][MIN], part->dd[Z], part->ng[Z])};
+        if (SearchStencil(space, nI, pI, 2, 0, kTypeDomain, sb, &gh->count, &wsum)) return 1;
+        gh->kind = 0;
+        gh->wO = InverseDistance(part->tinyL, artra_dist2(pI, pO));
+        /* body surface velocity V + W x (pO - O) */
+        const Real arm[DIMS] = {pO[X] - poly->O[X], pO[Y] - poly->O[Y], pO[Z] - poly->O[Z]};
+        artra_cross(poly->W[TO], arm, gh->Vs);
+        for (int s = 0; s < DIMS; ++s) gh->Vs[s] = poly->V[TO][s] + gh->Vs[s];
+        for (int s = 0; s < DIMS; ++s) gh->N[s] = N[s];
+        artra_orthogonal_space(N, gh->Ta, gh->Tb);
+        gh->noslip = (0.0 < poly->cf);
+        gh->isothermal = !(0.0 > poly->T);
+        gh->wallT = poly->T;
+    } else {
+        const int nG[DIMS] = {site->i, site->j, site->k};
+        if (SearchStencil(space, nG, pG, 1, site->body + 1, r - 1, sb, &gh->count, &wsum)) return 1;
+        gh->kind = 1;
+    }
+    return 0;
+}
+
+/*
+ * The ghost table: ghosts by layer, then body, then k-j-i within the body's
+ * box (the order of immersed_boundary.c:301-337 up to the body/layer loop
+ * order, which the fill does not depend on: a layer reads fluid nodes or
+ * the previous layer of the same body only).  The scan is serial; the
+ * entries and stencils, independent of each other, are built in parallel
+ * into per-thread buffers and joined in table order.
+ */
 int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTable *table)
 {
     const Partition *part = &space->part;
@@ -222,10 +280,11 @@ int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTa
     table->layers = part->gl;
     table->layer_start = calloc(part->gl + 1, sizeof(int));
     if (NULL == table->layer_start) return artra_set_error(1, "out of host memory");
-    StencilBuf sb = {0};
-    int cap = 0;
+    int cap = 0, count = 0;
+    GhostSite *sites = NULL;
+    int *layer = NULL;
     for (int r = 1; r <= part->gl; ++r) {
-        table->layer_start[r - 1] = table->nghost;
+        table->layer_start[r - 1] = count;
         for (int n = 0; n < geo->totN; ++n) {
             const Polyhedron *poly = geo->poly + n;
             int box[DIMS][LIMIT];
@@ -241,56 +300,79 @@ int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTa
                     for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
                         const int idx = artra_index(k, j, i, part->n[Y], part->n[X]);
                         if (r != node[idx].gst || n + 1 != node[idx].did) continue;
-                        if (table->nghost == cap) {
+                        if (count == cap) {
                             cap = cap ? 2 * cap : 1024;
-                            ArtraGhost *g = realloc(table->ghosts, cap * sizeof(ArtraGhost));
-                            if (NULL == g) {
-                                free(sb.node);
-                                free(sb.w);
+                            GhostSite *gs = realloc(sites, cap * sizeof *sites);
+                            int *gl = realloc(layer, cap * sizeof *layer);
+                            if (gs) sites = gs;
+                            if (gl) layer = gl;
+                            if (!gs || !gl) {
+                                free(sites);
+                                free(layer);
                                 return artra_set_error(1, "out of host memory for ghost table");
                             }
-                            table->ghosts = g;
                         }
-                        ArtraGhost *gh = table->ghosts + table->nghost;
-                        memset(gh, 0, sizeof *gh);
-                        gh->node = idx;
-                        gh->first = sb.n;
-                        const Real pG[DIMS] = {artra_map_point(i, part->domain[X][MIN], part->d[X], part->ng[X]),
-                                               artra_map_point(j, part->domain[Y][MIN], part->d[Y], part->ng[Y]),
-                                               artra_map_point(k, part->domain[Z][MIN], part->d[Z], part->ng[Z])};
-                        Real wsum = 0.0;
-                        if (model->ibmLayer >= r) {
-                            Real pO[DIMS], pI[DIMS], N[DIMS];
-                            artra_compute_geometric_data(pG, node[idx].fid, poly, pO, pI, N);
-                            const int nI[DIMS] = {
-                                artra_map_node(pI[X], part->domain[X][MIN], part->dd[X], part->ng[X]),
-                                artra_map_node(pI[Y], part->domain[Y][MIN], part->dd[Y], part->ng[Y]),
-                                artra_map_node(pI[Z], part->domain[Z][MIN], part->dd[Z], part->ng[Z])};
-                            if (SearchStencil(space, nI, pI, 2, 0, kTypeDomain, &sb, &gh->count, &wsum)) return 1;
-                            gh->kind = 0;
-                            gh->wO = InverseDistance(part->tinyL, artra_dist2(pI, pO));
-                            /* body surface velocity V + W x (pO - O) */
-                            const Real arm[DIMS] = {pO[X] - poly->O[X], pO[Y] - poly->O[Y], pO[Z] - poly->O[Z]};
-                            artra_cross(poly->W[TO], arm, gh->Vs);
-                            for (int s = 0; s < DIMS; ++s) gh->Vs[s] = poly->V[TO][s] + gh->Vs[s];
-                            for (int s = 0; s < DIMS; ++s) gh->N[s] = N[s];
-                            artra_orthogonal_space(N, gh->Ta, gh->Tb);
-                            gh->noslip = (0.0 < poly->cf);
-                            gh->isothermal = !(0.0 > poly->T);
-                            gh->wallT = poly->T;
-                        } else {
-                            const int nG[DIMS] = {i, j, k};
-                            if (SearchStencil(space, nG, pG, 1, n + 1, r - 1, &sb, &gh->count, &wsum)) return 1;
-                            gh->kind = 1;
-                        }
-                        table->nghost++;
+                        sites[count] = (GhostSite){idx, k, j, i, n};
+                        layer[count] = r;
+                        ++count;
                     }
         }
     }
-    table->layer_start[part->gl] = table->nghost;
+    table->layer_start[part->gl] = count;
+    table->ghosts = calloc(count > 0 ? count : 1, sizeof(ArtraGhost));
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+#endif
+    StencilBuf *tb = calloc(threads, sizeof *tb);
+    int *owner = malloc((count > 0 ? count : 1) * sizeof *owner);
+    int failed = (NULL == table->ghosts || NULL == tb || NULL == owner);
+    if (!failed) {
+#pragma omp parallel if (count > 64)
+        {
+            int tid = 0;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+#endif
+#pragma omp for schedule(dynamic, 16)
+            for (int q = 0; q < count; ++q) {
+                int stop;
+#pragma omp atomic read
+                stop = failed;
+                if (stop) continue;
+                owner[q] = tid;
+                if (BuildGhost(space, model, sites + q, layer[q], tb + tid, table->ghosts + q)) {
+#pragma omp atomic write
+                    failed = 1;
+                }
+            }
+        }
+    }
+    /* join the stencils in table order */
+    StencilBuf sb = {0};
+    for (int q = 0; q < count && !failed; ++q) {
+        ArtraGhost *gh = table->ghosts + q;
+        const StencilBuf *src = tb + owner[q];
+        const int from = gh->first;
+        gh->first = sb.n;
+        for (int e = 0; e < gh->count && !failed; ++e) failed = Push(&sb, src->node[from + e], src->w[from + e]);
+    }
+    for (int t = 0; tb && t < threads; ++t) {
+        free(tb[t].node);
+        free(tb[t].w);
+    }
+    free(tb);
+    free(owner);
+    free(sites);
+    free(layer);
+    table->nghost = count;
+    table->ghost_cap = count;
     table->stencil_node = sb.node;
     table->stencil_w = sb.w;
     table->nstencil = sb.n;
+    table->stencil_cap = sb.cap;
+    if (failed) return artra_set_error(1, "immersed boundary: ghost stencil search failed (no stencil node or "
+                                          "out of host memory)");
     return 0;
 }
 

@@ -280,6 +280,14 @@ def cases():
                              geo=moving_geo_text(stls=[("cylinder.stl", dict(r=0.1, V=(0.8, 0.5, 0), W=(0, 0, 3),
                                                                              rho=2.7))]),
                              stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
+    # the benchmark's immersed cylinder (512x512x4) moving the same way, 20 steps
+    c["cylinder_moving"] = dict(case=case_text(((0, 0, 0), (2, 2, 0.016), (512, 512, 4)), (0, 1e9, 0.6, 20, 1, 0),
+                                               (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
+                                               [inflow(*CYL_POST), OUTFLOW, slip(), slip(), PERIODIC, PERIODIC],
+                                               [plane((0.6, 0, 0), (-1, 0, 0), CYL_POST)]),
+                                geo=moving_geo_text(stls=[("cylinder.stl", dict(r=0.1, V=(0.8, 0.5, 0), W=(0, 0, 3),
+                                                                                rho=2.7))]),
+                                stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
     c["spheres_collide"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (24, 24, 24)),
                                                (0, 0.08, 0.6, 0, 2, 0), (1, 1, 0, 0, 0, 2, 1), INVISCID, REF1,
                                                ("1", "0", "0", "0", "1"), [OUTFLOW] * 6,

@@ -187,3 +187,31 @@ def test_integration_link_recipe():
     assert needed == {"EvolveFluidDynamics", "TreatBoundary", "TreatImmersedBoundary", "ComputeGeometricField"}
     assert needed <= exported
     assert not (defined_kept & exported), defined_kept & exported
+
+
+@pytest.mark.parametrize("name, moves, motion", [
+    ("cyl64_moving", 12, ("0.004", "0.0025", "0", "0.02")),     # sub-cell steps with spin
+    ("cyl64_moving", 5, ("0.031", "-0.02", "0", "0.3")),         # two cells a move
+    ("spheres_collide", 10, ("0.012", "-0.006", "0.003", "0")),
+    ("spheres_collide", 6, ("-0.05", "0.03", "0.02", "0")),
+])
+def test_moving_body_classification_shortcuts(name, moves, motion, tmp_path):
+    """After each move of the bodies, the classification restricted to the
+    scope's region (where the moving bodies were or are, grown by the ghost
+    layers, and the last reconstructed nodes) leaves every node's did, fid,
+    lid and gst and the reconstruction lists exactly as the classification
+    of the whole interior (immersed_boundary.c:65-264); the bounded
+    closest-facet search picks the same facet as the full scan on every
+    interior node and on vertices, edge midpoints and facet centroids; the
+    ghost table built in parallel equals the serial one (tests/scope_check.c)."""
+    import subprocess
+    from artracfd_amd import library
+    lib = library()._name
+    exe = str(tmp_path / "scope_check")
+    subprocess.run(["gcc", "-O2", "-std=c11", "-fopenmp", "-I" + os.path.join(ROOT, "include"),
+                    "-I" + os.path.join(ROOT, "artracfd_amd", "csrc", "host"),
+                    os.path.join(ROOT, "tests", "scope_check.c"), "-o", exe, lib,
+                    "-Wl,-rpath," + os.path.dirname(lib), "-lm"], check=True)
+    out = subprocess.run([exe, case_dir(name), str(moves), *motion], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ") and int(out.stdout.split()[1]) > 0, out.stdout
