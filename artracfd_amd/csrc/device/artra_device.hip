@@ -2541,10 +2541,11 @@ int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *jo
     return LaunchIb(dev, Level(dev, level), dev->joins, 0, joins->nghost);
 }
 
-int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid)
+int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count)
 {
+    if (first < 0 || count < 0 || first + count > dev->g.N) return Fail("fluid flags [%ld, %ld) outside the grid", first, first + count);
     HIP_TRY(hipStreamSynchronize(dev->stream));
-    HIP_TRY(hipMemcpy(dev->fluid, fluid, (size_t)dev->g.N, hipMemcpyHostToDevice));
+    if (count > 0) HIP_TRY(hipMemcpy(dev->fluid + first, fluid + first, (size_t)count, hipMemcpyHostToDevice));
     dev->allFluid = AllFluid(dev->g, fluid);
     return 0;
 }

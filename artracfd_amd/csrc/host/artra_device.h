@@ -82,8 +82,9 @@ int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *table);
 /* the nodes a moving body has just uncovered: their reconstruction
  * (artra_add_join entries) evaluated on level `level` */
 int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *joins);
-/* new node flags after the bodies moved: fluid[q] != 0 for a fluid node */
-int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid);
+/* new node flags after the bodies moved: fluid[q] != 0 for a fluid node
+ * (the whole grid's array; entries [first, first + count) are copied) */
+int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count);
 /* copy one time level: soa is [5][N] doubles */
 int artra_dev_upload(ArtraDevice *dev, int level, const double *soa);
 int artra_dev_download(ArtraDevice *dev, int level, double *soa);

@@ -117,12 +117,12 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
     s->model = model;
     s->N = (long)part->n[X] * part->n[Y] * part->n[Z];
     s->staging = malloc((size_t)5 * s->N * sizeof(double));
-    uint8_t *fluid = malloc((size_t)s->N);
-    if (NULL == s->staging || NULL == fluid) {
-        free(fluid);
+    s->fluid = malloc((size_t)s->N);
+    if (NULL == s->staging || NULL == s->fluid) {
         artra_hip_detach(s);
         return artra_set_error(1, "out of host memory for %ld nodes", s->N);
     }
+    uint8_t *fluid = s->fluid;
     for (long q = 0; q < s->N; ++q) fluid[q] = (0 == space->node[q].did);
     ArtraDevParams par;
     memset(&par, 0, sizeof par);
@@ -154,7 +154,6 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         for (int v = 0; v < VARBC; ++v) par.varBC[p][v] = part->varBC[p][v];
     }
     const int rc = artra_dev_create(&par, fluid, &s->dev);
-    free(fluid);
     if (rc) {
         artra_hip_detach(s);
         return DevError();
@@ -196,6 +195,7 @@ int artra_hip_detach(ArtraHipSolver *s)
     artra_free_ghost_table(&s->ghosts);
     free(s->staging);
     free(s->force);
+    free(s->fluid);
     artra_scope_free(&s->scope);
     free(s);
     return 0;
@@ -546,7 +546,7 @@ void ComputeGeometricField(Space *space, const Model *model)
     artra_free_ghost_table(&joins);
     /* the cached device solver's tables follow the new flags */
     if (!rc && gDropIn.solver && gDropIn.space == space && gDropIn.node == space->node) {
-        rc = artra_hip_refresh_geometry(gDropIn.solver);
+        rc = artra_hip_refresh_geometry(gDropIn.solver, NULL);
     }
     if (rc) {
         fprintf(stderr, "error: %s\n", artra_hip_error());

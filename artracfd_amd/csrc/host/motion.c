@@ -271,15 +271,32 @@ static void Motion(Space *space, Real dt)
     }
 }
 
-int artra_hip_refresh_geometry(ArtraHipSolver *s)
+int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT])
 {
     const Space *space = s->space;
-    uint8_t *fluid = malloc((size_t)s->N);
-    if (NULL == fluid) return artra_set_error(1, "out of host memory for %ld node flags", s->N);
-    for (long q = 0; q < s->N; ++q) fluid[q] = (0 == space->node[q].did);
-    const int rc = artra_dev_set_fluid(s->dev, fluid);
-    free(fluid);
-    if (rc) return artra_dev_failed();
+    const Partition *part = &space->part;
+    long first = 0, last = s->N;          /* index span of the changed flags */
+    if (NULL == region) {
+        for (long q = 0; q < s->N; ++q) s->fluid[q] = (0 == space->node[q].did);
+    } else {
+        int lo[DIMS], hi[DIMS];
+        for (int d = 0; d < DIMS; ++d) {
+            lo[d] = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
+            hi[d] = (region[d][MAX] < part->ns[PIN][d][MAX]) ? region[d][MAX] : part->ns[PIN][d][MAX];
+        }
+        first = last = 0;
+        if (lo[X] < hi[X] && lo[Y] < hi[Y] && lo[Z] < hi[Z]) {
+            for (int k = lo[Z]; k < hi[Z]; ++k)
+                for (int j = lo[Y]; j < hi[Y]; ++j)
+                    for (int i = lo[X]; i < hi[X]; ++i) {
+                        const int q = artra_index(k, j, i, part->n[Y], part->n[X]);
+                        s->fluid[q] = (0 == space->node[q].did);
+                    }
+            first = artra_index(lo[Z], lo[Y], lo[X], part->n[Y], part->n[X]);
+            last = artra_index(hi[Z] - 1, hi[Y] - 1, hi[X] - 1, part->n[Y], part->n[X]) + 1;
+        }
+    }
+    if (artra_dev_set_fluid(s->dev, s->fluid, first, last - first)) return artra_dev_failed();
     artra_free_ghost_table(&s->ghosts);
     if (artra_build_ghost_table(space, s->model, &s->ghosts)) return 1;
     if (artra_dev_set_ghosts(s->dev, &s->ghosts)) return artra_dev_failed();
@@ -301,7 +318,7 @@ int artra_solid_step(ArtraHipSolver *s, Real now, Real dt)
     memset(&joins, 0, sizeof joins);
     int rc = artra_compute_geometric_field(space, model, &joins, (const int (*)[LIMIT])region);
     if (!rc) artra_scope_update(space, &s->scope, &joins, (const int (*)[LIMIT])region);
-    if (!rc) rc = artra_hip_refresh_geometry(s);
+    if (!rc) rc = artra_hip_refresh_geometry(s, (const int (*)[LIMIT])region);
     if (!rc && artra_dev_reconstruct(s->dev, TO, &joins)) rc = artra_dev_failed();
     if (!rc && artra_dev_treat_immersed(s->dev, TO)) rc = artra_dev_failed();
     artra_free_ghost_table(&joins);

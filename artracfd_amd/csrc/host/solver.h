@@ -29,6 +29,7 @@ struct ArtraHipSolver {
     int forceBodies;          /* bodies in force[] */
     ArtraForceBodyInfo *force;
     ArtraGeometryScope scope; /* moving bodies: where the next classification must look */
+    uint8_t *fluid;           /* [N] host copy of the device's fluid flags */
 };
 
 /* Build the surface-force table of the solver's bodies and hand it to the
@@ -38,8 +39,9 @@ int artra_build_force_table(ArtraHipSolver *s);
 int artra_dev_failed(void);
 /* The device tables that follow the node classification (fluid flags,
  * immersed-boundary ghosts, surface-force nodes), rebuilt from the host
- * flags after the bodies moved. */
-int artra_hip_refresh_geometry(ArtraHipSolver *s);
+ * flags after the bodies moved; region (may be NULL: everywhere) bounds the
+ * nodes whose flags changed. */
+int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT]);
 /* EvolveSolidDynamics (solid_dynamics.c:37-48) over dt at time now. */
 int artra_solid_step(ArtraHipSolver *s, Real now, Real dt);
 /* One pass of the time loop's body with moving bodies (solve.c:65-89): the

@@ -161,13 +161,18 @@ SIDE_WORKLOADS = (
     ("tgv128", 20, "tgv64", 4, "sweep_x/sweep_march with the viscous flux (stage-by-stage sweeps)", 176.6),
     ("cylinder", 40, "cyl64", 100, "sweep_x/sweep_march with the immersed-boundary fill (stage-by-stage sweeps)",
      126.0),
+    ("cylinder_moving", 20, "cyl64_moving", 10,
+     "sweep_x/sweep_march with the immersed-boundary fill; solid half-steps (surface force, kinematics, motion, "
+     "reclassification, ghost tables) between the fluid steps", 126.0),
 )
 
 
 def side_workloads(warmup, with_cpu):
     """Rate, kernel times and the sweep roofline of the viscous Taylor-Green
-    vortex (128^3) and the shock over the immersed cylinder (512x512x4),
-    with the reference's CPU rate on a smaller sample of each."""
+    vortex (128^3), the shock over the immersed cylinder (512x512x4) and the
+    same cylinder moving (translating and spinning under the shock's force),
+    with the reference's CPU rate on a smaller sample of each.  host_ms_per_step
+    is the wall time not covered by kernels (solid half-steps, launches)."""
     from artracfd_amd import Case, Solver
     out = {}
     for name, steps, cpu_case, cpu_steps, kernel, traffic_b in SIDE_WORKLOADS:
@@ -187,6 +192,7 @@ def side_workloads(warmup, with_cpu):
             "workload": describe(case_dir, name, cells), "value": round(cells * 18 * done / elapsed / 1e6, 3),
             "unit": "Mcell-stage updates/s", "steps": done, "ms_per_step": round(elapsed * 1e3 / max(done, 1), 4),
             "kernel_ms": {k: round(v, 3) for k, v in busy.items()},
+            "host_ms_per_step": round(max(elapsed * 1e3 - sum(busy.values()), 0.0) / max(done, 1), 4),
             "kernel_launches": {k: v[1] for k, v in tm.items()},
             "dominant": max(busy, key=busy.get),
             "roofline": {"kernel": kernel, "bound": "hbm", "algorithmic_bytes_per_cell_stage": round(stage_b, 1),

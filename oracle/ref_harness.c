@@ -37,6 +37,7 @@
 #include "solve.h"
 #include "initialization.h"
 #include "fluid_dynamics.h"
+#include "solid_dynamics.h"
 #include "boundary_treatment.h"
 #include "convective_flux.h"
 #include "diffusive_flux.h"
@@ -95,8 +96,9 @@ static double WallSeconds(void)
 }
 
 /*
- * Time loop of EvolveSolution (solve.c:48-107) minus output and solid
- * dynamics.  maxSteps caps the loop in addition to the case's own stepN.
+ * Time loop of EvolveSolution (solve.c:48-107) minus output, with the solid
+ * dynamics half-steps where bodies move (psi != 0).  maxSteps caps the loop
+ * in addition to the case's own stepN.
  */
 static int March(Time *time, Space *space, const Model *model, int maxSteps, FILE *dtLog, double *elapsed)
 {
@@ -120,7 +122,9 @@ static int March(Time *time, Space *space, const Model *model, int maxSteps, FIL
             dt = time->end - (time->now - dt);
             time->now = time->end;
         }
+        if (0 != model->psi) EvolveSolidDynamics(time->now, 0.5 * dt, space, model);
         EvolveFluidDynamics(dt, space, model);
+        if (0 != model->psi) EvolveSolidDynamics(time->now, 0.5 * dt, space, model);
         if (dtLog) fwrite(&dt, sizeof(double), 1, dtLog);
         ++done;
     }
