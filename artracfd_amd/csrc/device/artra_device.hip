@@ -1329,27 +1329,42 @@ __global__ __launch_bounds__(256) void force_terms(const double *U, const ArtraF
 // Per body, the sums of its nodes' terms in table order (the reference's
 // loop order, so the sums are those of its serial loop) and the pressure
 // statistics: fvar = (p of the first node, sum (p - p0), sum (p - p0)^2).
-__global__ void force_sum(const ArtraForceBody *bodies, int nbody, const double *terms, double *sums)
+// One wavefront per body: the lanes load 64 nodes' terms at once, then the
+// sums run over the lanes' values in node order (the serial loop's
+// additions, without its memory latency per node).
+__global__ __launch_bounds__(64) void force_sum(const ArtraForceBody *bodies, int nbody, const double *terms,
+                                                double *sums)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x;
     if (b >= nbody) return;
+    const int lane = threadIdx.x;
     const ArtraForceBody bd = bodies[b];
+    const int last = bd.first + bd.count;
     double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     double fvar[3] = {0.0, 0.0, 0.0};
-    for (int t = bd.first; t < bd.first + bd.count; ++t) {
-        const double *o = terms + (size_t)t * 10;
-        if (t == bd.first) fvar[0] = o[9];
-        fvar[1] = fvar[1] + o[9] - fvar[0];
-        fvar[2] = fvar[2] + (o[9] - fvar[0]) * (o[9] - fvar[0]);
+    if (bd.count > 0) fvar[0] = terms[(size_t)bd.first * 10 + 9];
+    for (int base = bd.first; base < last; base += 64) {
+        const int t = base + lane;
+        double o[10];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] = acc[q] + o[q];
+        for (int q = 0; q < 10; ++q) o[q] = (t < last) ? terms[(size_t)t * 10 + q] : 0.0;
+        const int m = min(64, last - base);
+        for (int e = 0; e < m; ++e) {
+            const double pe = LaneValue(o[9], e);
+            fvar[1] = fvar[1] + pe - fvar[0];
+            fvar[2] = fvar[2] + (pe - fvar[0]) * (pe - fvar[0]);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) acc[q] = acc[q] + LaneValue(o[q], e);
+        }
     }
-    double *out = sums + (size_t)b * 12;
+    if (lane == 0) {
+        double *out = sums + (size_t)b * 12;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) out[q] = acc[q];
-    out[9] = fvar[0];
-    out[10] = fvar[1];
-    out[11] = fvar[2];
+        for (int q = 0; q < 9; ++q) out[q] = acc[q];
+        out[9] = fvar[0];
+        out[10] = fvar[1];
+        out[11] = fvar[2];
+    }
 }
 
 // Primitive variables of level TO at a list of nodes (the probes).
@@ -2648,7 +2663,7 @@ int artra_dev_surface_force(ArtraDevice *dev, double *sums)
                            dev->par.gamma, dev->par.gasR, dev->par.refMu, dev->par.refT, dev->forceTerms);
         HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(force_sum, dim3((dev->forceBodyCount + 63) / 64), dim3(64), 0, dev->stream, dev->forceBodies,
+    hipLaunchKernelGGL(force_sum, dim3(dev->forceBodyCount), dim3(64), 0, dev->stream, dev->forceBodies,
                        dev->forceBodyCount, dev->forceTerms, dev->forceSums);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(sums, dev->forceSums, (size_t)dev->forceBodyCount * 12 * sizeof(double),

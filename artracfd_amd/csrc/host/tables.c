@@ -19,6 +19,7 @@
  * the reconstruction of a node that a moving body has just uncovered
  * (immersed_boundary.c:193-204).
  */
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -348,15 +349,35 @@ int artra_build_ghost_table(const Space *space, const Model *model, ArtraGhostTa
             }
         }
     }
-    /* join the stencils in table order */
+    /* join the stencils in table order: offsets by a running sum, then the
+     * segments copied in parallel */
     StencilBuf sb = {0};
+    int *from = malloc((count > 0 ? count : 1) * sizeof *from);
+    failed = failed || NULL == from;
+    long total = 0;
     for (int q = 0; q < count && !failed; ++q) {
         ArtraGhost *gh = table->ghosts + q;
-        const StencilBuf *src = tb + owner[q];
-        const int from = gh->first;
-        gh->first = sb.n;
-        for (int e = 0; e < gh->count && !failed; ++e) failed = Push(&sb, src->node[from + e], src->w[from + e]);
+        from[q] = gh->first;
+        gh->first = (int)total;
+        total += gh->count;
     }
+    if (!failed && total > INT_MAX) failed = 1;
+    if (!failed && total > 0) {
+        sb.node = malloc((size_t)total * sizeof(int));
+        sb.w = malloc((size_t)total * sizeof(Real));
+        sb.n = sb.cap = (int)total;
+        failed = (NULL == sb.node || NULL == sb.w);
+    }
+    if (!failed) {
+#pragma omp parallel for schedule(static) if (count > 64)
+        for (int q = 0; q < count; ++q) {
+            const ArtraGhost *gh = table->ghosts + q;
+            const StencilBuf *src = tb + owner[q];
+            memcpy(sb.node + gh->first, src->node + from[q], (size_t)gh->count * sizeof(int));
+            memcpy(sb.w + gh->first, src->w + from[q], (size_t)gh->count * sizeof(Real));
+        }
+    }
+    free(from);
     for (int t = 0; tb && t < threads; ++t) {
         free(tb[t].node);
         free(tb[t].w);

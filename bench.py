@@ -341,6 +341,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="run only the 64-case batched workload")
     ap.add_argument("--batched-plain", action="store_true", help="batched workload without the indicator run")
     ap.add_argument("--no-side", action="store_true", help="skip the viscous TGV and immersed-cylinder workloads")
+    ap.add_argument("--side-only", action="store_true",
+                    help="run only the viscous TGV, immersed-cylinder and moving-cylinder workloads")
     ap.add_argument("--no-end-time", action="store_true",
                     help="skip the plain and hybrid marches to the end time (profiling runs)")
     args = ap.parse_args()
@@ -351,6 +353,9 @@ def main():
     from artracfd_amd import Case, Solver
     if args.batched_only:
         print(json.dumps({"batched": batched_both(not args.batched_plain)}))
+        return
+    if args.side_only:
+        print(json.dumps({"viscous_and_immersed": side_workloads(args.warmup, not args.no_cpu)}))
         return
     case_dir = os.path.join(ROOT, "cases", args.case)
     t_init = time.perf_counter()

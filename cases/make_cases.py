@@ -280,6 +280,17 @@ def cases():
                              geo=moving_geo_text(stls=[("cylinder.stl", dict(r=0.1, V=(0.8, 0.5, 0), W=(0, 0, 3),
                                                                              rho=2.7))]),
                              stl=("cylinder.stl", 1.0, 1.0, 0.1, -0.05, 0.066, 512))
+    # the viscous sphere moving through the flow: translating and spinning no-slip isothermal body
+    # whose surface force has its viscous part (psi 1)
+    c["sphere20_moving"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (20, 20, 20)),
+                                               (0, 0.15, 0.6, 0, 2, 0), (1, 1, 0, 0, 0, 1, 0),
+                                               (0, 0.05, 0, (0, 0, 0)), (1, 1, 1, 1),
+                                               ("1", "0.3", "0.1*sin(3*y)", "0", "1+0.1*cos(4*z)"),
+                                               [inflow(1, 0.3, 0, 0, 1), OUTFLOW, noslip(0.004), slip(), OUTFLOW,
+                                                OUTFLOW],
+                                               probes=([(0.3, 0.1, -0.05)], [], 1, 1, (5, 5, 5, 5))),
+                                geo=moving_geo_text(spheres=[dict(O=(0.05, 0, 0), r=0.2, V=(-0.6, 0.2, 0.1),
+                                                                  W=(0, 0, 2), rho=2.7, T=0.0038, cf=1)]))
     # the benchmark's immersed cylinder (512x512x4) moving the same way, 20 steps
     c["cylinder_moving"] = dict(case=case_text(((0, 0, 0), (2, 2, 0.016), (512, 512, 4)), (0, 1e9, 0.6, 20, 1, 0),
                                                (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1, ("1.4", "0", "0", "0", "1"),

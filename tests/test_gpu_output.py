@@ -57,7 +57,7 @@ BITWISE = ("cyl64_probes", "cyl64_ensight", "cyl64_moving", "spheres_collide")
 
 
 @pytest.mark.parametrize("name", ["cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight",
-                                  "cyl64_moving", "spheres_collide"])
+                                  "cyl64_moving", "spheres_collide", "sphere20_moving"])
 @pytest.mark.parametrize("exact", [True, False])
 def test_solve_writes_the_reference_outputs(name, exact, tmp_path):
     """Exact arithmetic on the inviscid cases: every text file
@@ -65,9 +65,9 @@ def test_solve_writes_the_reference_outputs(name, exact, tmp_path):
     reference's (ParaView and EnSight Gold writers) and, for the moving
     bodies (a translating, spinning STL cylinder; two spheres that collide),
     the same final positions, velocities and accelerations.  The viscous
-    sphere (the device's pow differs from glibc's in the last bit) and the
-    fast arithmetic: the same files with the same numbers to the printed
-    precision."""
+    spheres, still and moving (the device's pow differs from glibc's in the
+    last bit), and the fast arithmetic: the same files with the same numbers
+    to the printed precision."""
     print(f"solve {name} exact={exact}", flush=True)
     fx = golden("solve_" + name)
     out, U, F, B, now, steps = _solve(name, tmp_path, exact)
