@@ -12,8 +12,9 @@
 //                  coalesced along x across the wavefront.
 //   bc_group       a group of TreatBoundary boxes whose nodes do not
 //                  interact (boundary_treatment.c:28-192).
-//   ib_layer_wave  one ghost layer of the immersed-boundary fill from the
-//                  host-precomputed stencil table, a wavefront per ghost node
+//   ib_fill        the immersed-boundary fill of one ghost layer (or of all
+//                  layers when none reads another) from the host-precomputed
+//                  stencil table, a 64-lane block per ghost node
 //                  (immersed_boundary.c:270-487).
 //   cfl_max        per-axis maximum characteristic speed (solve.c:108-153).
 //
@@ -1234,48 +1235,49 @@ __device__ __forceinline__ void IbFinish(double *U, const GhostDev &gh, double U
     StoreNode(U, gh.node, out);
 }
 
-// Lane e's value of x (e the same on every lane).
-__device__ __forceinline__ double LaneValue(double x, int e)
+// One 64-lane block per filled node: the lanes load and convert 64 stencil
+// entries at a time into LDS (the loop's memory latency overlapped), then
+// lane c < 6 sums component c of the weighted primitive states and lane 6
+// the weights, each over the entries in stencil order: the same operations
+// as a loop over the stencil (immersed_boundary.c:393-440), so the node's
+// state is bit for bit the reference's.  Grid: one block per entry of
+// [begin, end).
+__global__ __launch_bounds__(64) void ib_fill(double *U, const GhostDev *ghosts, int begin, const int *snode,
+                                              const double *sw, double gamma, double gasR)
 {
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)bits, e);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(bits >> 32), e);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// One wavefront per ghost node: the lanes load and convert the stencil
-// entries in parallel (the loop's memory latency), then the weighted sums
-// run in the stencil's order over the lanes' values, the same operations
-// as a loop over the stencil (immersed_boundary.c:393-440), so the ghost's
-// state is bit for bit the reference's.
-__global__ __launch_bounds__(256) void ib_layer_wave(double *U, const GhostDev *ghosts, int begin, int end,
-                                                     const int *snode, const double *sw, double gamma, double gasR)
-{
-    const int gidx = begin + (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    if (gidx >= end) return;                        // whole wavefronts leave together
-    const int lane = __lane_id();
-    const GhostDev gh = ghosts[gidx];
-    double Uo[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double wsum = 0.0;
+    __shared__ double stage[64 * 7];
+    __shared__ double part[7];
+    const int lane = threadIdx.x;
+    const GhostDev gh = ghosts[begin + blockIdx.x];
+    double acc = 0.0;                               // lane c < 6: Uo[c]; lane 6: the weight sum
     const int last = gh.first + gh.count;
     for (int base = gh.first; base < last; base += 64) {
         const int q = base + lane;
-        double Uoh[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, w = 0.0;
         if (q < last) {
-            double Uh[5];
+            double Uh[5], Uoh[6];
             LoadNode(U, snode[q], Uh);
             Primitive(gamma, gasR, Uh, Uoh);
-            w = sw[q];
-        }
-        const int m = min(64, last - base);
-        for (int e = 0; e < m; ++e) {
-            const double we = LaneValue(w, e);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) Uo[c] = Uo[c] + LaneValue(Uoh[c], e) * we;
-            wsum = wsum + we;
+            for (int c = 0; c < 6; ++c) stage[lane * 7 + c] = Uoh[c];
+            stage[lane * 7 + 6] = sw[q];
         }
+        __syncthreads();
+        const int m = min(64, last - base);
+        if (lane < 6) {
+            for (int e = 0; e < m; ++e) acc = acc + stage[e * 7 + lane] * stage[e * 7 + 6];
+        } else if (lane == 6) {
+            for (int e = 0; e < m; ++e) acc = acc + stage[e * 7 + 6];
+        }
+        __syncthreads();
     }
-    if (lane == 0) IbFinish(U, gh, Uo, wsum, gamma, gasR);
+    if (lane < 7) part[lane] = acc;
+    __syncthreads();
+    if (lane == 0) {
+        double Uo[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Uo[c] = part[c];
+        IbFinish(U, gh, Uo, part[6], gamma, gasR);
+    }
 }
 
 // ----------------------------------------------------------- surface force
@@ -1329,42 +1331,45 @@ __global__ __launch_bounds__(256) void force_terms(const double *U, const ArtraF
 // Per body, the sums of its nodes' terms in table order (the reference's
 // loop order, so the sums are those of its serial loop) and the pressure
 // statistics: fvar = (p of the first node, sum (p - p0), sum (p - p0)^2).
-// One wavefront per body: the lanes load 64 nodes' terms at once, then the
-// sums run over the lanes' values in node order (the serial loop's
-// additions, without its memory latency per node).
+// One 64-lane block per body: 64 nodes' terms at a time are staged in LDS,
+// then lane q < 9 sums term q and lanes 9 and 10 the two statistics, each
+// over the nodes in table order (the serial loop's additions, without its
+// memory latency per node).
 __global__ __launch_bounds__(64) void force_sum(const ArtraForceBody *bodies, int nbody, const double *terms,
                                                 double *sums)
 {
+    __shared__ double stage[64 * 10];
     const int b = blockIdx.x;
-    if (b >= nbody) return;
+    if (b >= nbody) return;                         // uniform over the block
     const int lane = threadIdx.x;
     const ArtraForceBody bd = bodies[b];
     const int last = bd.first + bd.count;
-    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double fvar[3] = {0.0, 0.0, 0.0};
-    if (bd.count > 0) fvar[0] = terms[(size_t)bd.first * 10 + 9];
+    const double p0 = (bd.count > 0) ? terms[(size_t)bd.first * 10 + 9] : 0.0;
+    double acc = 0.0;
     for (int base = bd.first; base < last; base += 64) {
         const int t = base + lane;
-        double o[10];
+        if (t < last) {
 #pragma unroll
-        for (int q = 0; q < 10; ++q) o[q] = (t < last) ? terms[(size_t)t * 10 + q] : 0.0;
-        const int m = min(64, last - base);
-        for (int e = 0; e < m; ++e) {
-            const double pe = LaneValue(o[9], e);
-            fvar[1] = fvar[1] + pe - fvar[0];
-            fvar[2] = fvar[2] + (pe - fvar[0]) * (pe - fvar[0]);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) acc[q] = acc[q] + LaneValue(o[q], e);
+            for (int q = 0; q < 10; ++q) stage[lane * 10 + q] = terms[(size_t)t * 10 + q];
         }
+        __syncthreads();
+        const int m = min(64, last - base);
+        if (lane < 9) {
+            for (int e = 0; e < m; ++e) acc = acc + stage[e * 10 + lane];
+        } else if (lane == 9) {
+            for (int e = 0; e < m; ++e) acc = acc + stage[e * 10 + 9] - p0;
+        } else if (lane == 10) {
+            for (int e = 0; e < m; ++e) acc = acc + (stage[e * 10 + 9] - p0) * (stage[e * 10 + 9] - p0);
+        }
+        __syncthreads();
     }
-    if (lane == 0) {
-        double *out = sums + (size_t)b * 12;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) out[q] = acc[q];
-        out[9] = fvar[0];
-        out[10] = fvar[1];
-        out[11] = fvar[2];
+    double *out = sums + (size_t)b * 12;
+    if (lane < 9) out[lane] = acc;
+    if (lane == 9) {
+        out[9] = p0;
+        out[10] = acc;
     }
+    if (lane == 10) out[11] = acc;
 }
 
 // Primitive variables of level TO at a list of nodes (the probes).
@@ -1675,6 +1680,7 @@ struct ArtraDevice {
     int *gSrc[4] = {nullptr, nullptr, nullptr, nullptr};
     long gCount[4] = {0, 0, 0, 0};
     IbTable ib;                     // ghost nodes, grouped by layer
+    int ibImageOnly = 0;            // every ghost reconstructs from its image point (no layer reads another)
     IbTable joins;                  // nodes a moving body has just uncovered
     std::vector<int> layerStart;
     FluxModel mdl{};
@@ -1827,23 +1833,29 @@ int Sweep(ArtraDevice *dev, double dt, double a, double b, int to, int tn, int t
     return 0;
 }
 
-// Entries [b, e) of a table, one wavefront each.
+// Entries [b, e) of a table, one block each.
 int LaunchIb(ArtraDevice *dev, double *U, const IbTable &tb, int b, int e)
 {
     int slot;
     if (Begin(dev, ARTRA_T_IB, &slot)) return 1;
-    hipLaunchKernelGGL(ib_layer_wave, dim3((e - b + 3) / 4), dim3(256), 0, dev->stream, U, tb.ghosts, b, e, tb.snode,
-                       tb.sw, dev->par.gamma, dev->par.gasR);
+    hipLaunchKernelGGL(ib_fill, dim3((unsigned)(e - b)), dim3(64), 0, dev->stream, U, tb.ghosts, b, tb.snode, tb.sw,
+                       dev->par.gamma, dev->par.gasR);
     HIP_TRY(hipGetLastError());
     return End(dev, slot);
 }
 
 // The immersed-boundary ghost fill of one level, layer by layer
-// (immersed_boundary.c:270-340).
+// (immersed_boundary.c:270-340).  When every ghost reconstructs from its
+// image point (its stencil holds fluid nodes only, which the fill never
+// writes), no layer reads another and one launch fills them all.
 int ImmersedFill(ArtraDevice *dev, int level)
 {
     double *U = Level(dev, level);
     const int layers = (int)dev->layerStart.size() - 1;
+    if (dev->ibImageOnly) {
+        const int b = dev->layerStart[0], e = dev->layerStart[layers];
+        return (e > b) ? LaunchIb(dev, U, dev->ib, b, e) : 0;
+    }
     for (int r = 0; r < layers; ++r) {
         const int b = dev->layerStart[r], e = dev->layerStart[r + 1];
         if (e > b && LaunchIb(dev, U, dev->ib, b, e)) return 1;
@@ -2543,8 +2555,12 @@ int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial)
 int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *t)
 {
     dev->layerStart.assign(1, 0);
+    dev->ibImageOnly = 0;
     if (UploadIbTable(dev, dev->ib, t)) return 1;
     if (t->nghost > 0) dev->layerStart.assign(t->layer_start, t->layer_start + t->layers + 1);
+    int image = 1;
+    for (int q = 0; q < t->nghost; ++q) image = image && (0 == t->ghosts[q].kind);
+    dev->ibImageOnly = image;
     return 0;
 }
 
