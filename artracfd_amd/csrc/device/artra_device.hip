@@ -68,7 +68,7 @@ constexpr int kMarchChunk = ARTRA_DEV_MARCH_CHUNK;   // steps per device-stepped
 constexpr int kMarchClock = ARTRA_DEV_MARCH_CLOCK;     // doubles of a device march's clock
 constexpr size_t MarchScratchBytes(int cap)
 {
-    return (size_t)(4 + kMarchClock + 2 + cap) * sizeof(double);
+    return (size_t)(4 + kMarchClock + 3 + cap) * sizeof(double);
 }
 
 // Scratch of the allocation-free march (artra_dev_march_noalloc): speed
@@ -76,7 +76,7 @@ constexpr size_t MarchScratchBytes(int cap)
 // as MarchScratch.  Every GPU the library runs on holds its own copy until
 // the process ends; calls that overlap on one GPU share it and race.
 constexpr int kNoAllocSteps = ARTRA_DEV_NOALLOC_STEPS;
-static __device__ __align__(256) double scratch[4 + kMarchClock + 2 + kNoAllocSteps];
+static __device__ __align__(256) double scratch[4 + kMarchClock + 3 + kNoAllocSteps];
 static_assert(sizeof scratch == MarchScratchBytes(kNoAllocSteps), "scratch layout");
 
 constexpr int kMaxDevices = 64;
@@ -1552,7 +1552,8 @@ struct StepGeometry {
 };
 
 // Time step of a single case on the device: step[0] = dt, step[1] = dt / 2
-// (the operators' step), hist[0] = dt (the march's record of its steps).
+// (the operators' step), hist[step[2]] = dt (the march's record of its
+// steps, step[2] counting the steps of the current march call).
 // The clock carries the output cadence of the solver loop (solve.c:57-62,
 // 91-104): clock[7 + n] accumulates the steps of data kind n, clock[12 + n]
 // is its interval; after a step at which some output is due (its
@@ -1576,7 +1577,8 @@ __global__ void dt_single(StepGeometry sg, const unsigned long long *vmax, doubl
     }
     step[0] = h;
     step[1] = 0.5 * h;
-    hist[0] = h;
+    hist[(int)step[2]] = h;
+    step[2] = step[2] + 1.0;
 }
 
 struct GatherCase {
@@ -1670,6 +1672,12 @@ struct ArtraDevice {
     double *probePrim = nullptr;
     int probeCap = 0;
     const double *stepDev = nullptr; // when set, the operators read their step from here (device march)
+    // the device march's step as a captured graph
+    int useGraph = 1;
+    int configGen = 0;              // bumped by every change to what a step launches
+    int graphGen = -1;
+    double graphKey[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    hipGraphExec_t stepGraph = nullptr;
     int ordinal = 0;                // the GPU this solver lives on
     hipStream_t stream = nullptr;
     std::vector<BcGroup> groups[4];          // mode 0: full fill; 1 + s: faces of axis s only
@@ -2327,37 +2335,97 @@ MarchScratch MarchScratchAt(void *base, int cap)
     m.vmax = (unsigned long long *)base;
     m.clock = (double *)base + 4;
     m.step = m.clock + kMarchClock;
-    m.hist = m.step + 2;
+    m.hist = m.step + 3;
     m.cap = cap;
     return m;
 }
 
-// steps time steps with the dt computed on the device: per step the CFL
-// maxima, the clock's step (dt_single) and EvolveFluidDynamics reading its
-// step from device memory, with no host round trip in between.  clock
-// (in/out) is the solver loop's clock, dts[i] the steps taken (0 once the
-// clock has reached its end), *advanced the steps that moved the clock.
+// One time step with its dt computed on the device: the CFL maxima, the
+// clock's step (dt_single) and EvolveFluidDynamics reading its step from
+// device memory.
+int EnqueueStep(ArtraDevice *dev, const MarchScratch &ms, const double floor[3], double cfl)
+{
+    StepGeometry sg;
+    for (int s = 0; s < 3; ++s) sg.d[s] = dev->par.d[s];
+    sg.cfl = cfl;
+    if (EnqueueSpeeds(dev, floor, ms.vmax)) return 1;
+    hipLaunchKernelGGL(dt_single, dim3(1), dim3(1), 0, dev->stream, sg, ms.vmax, ms.clock, ms.step, ms.hist);
+    if (hipGetLastError() != hipSuccess) return Fail("dt_single launch failed");
+    return artra_dev_evolve(dev, 0.0);
+}
+
+// The step as a captured graph, replayed once per step: one launch instead
+// of the step's 20 to 150 kernel launches.  Rebuilt when anything the
+// step's launches depend on changed (configGen) or the body-speed floors or
+// the CFL number differ; not used with kernel timing (its events) or the
+// learned indicator (its lean-pass policy changes the launches between
+// march calls).
+int StepGraph(ArtraDevice *dev, const MarchScratch &ms, const double floor[3], double cfl, hipGraphExec_t *out)
+{
+    *out = nullptr;
+    if (!dev->useGraph || dev->timing || dev->hybrid) return 0;
+    const double key[5] = {floor[0], floor[1], floor[2], cfl, (double)(uintptr_t)ms.clock};
+    if (dev->stepGraph && dev->graphGen == dev->configGen && 0 == memcmp(key, dev->graphKey, sizeof key)) {
+        *out = dev->stepGraph;
+        return 0;
+    }
+    if (dev->stepGraph) {
+        HIP_TRY(hipGraphExecDestroy(dev->stepGraph));
+        dev->stepGraph = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    dev->stepDev = ms.step + 1;
+    HIP_TRY(hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = EnqueueStep(dev, ms, floor, cfl);
+    const hipError_t e = hipStreamEndCapture(dev->stream, &graph);
+    dev->stepDev = nullptr;
+    if (rc || e != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return rc ? 1 : Fail("step capture failed: %s", hipGetErrorString(e));
+    }
+    const hipError_t ie = hipGraphInstantiate(&dev->stepGraph, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        dev->stepGraph = nullptr;
+        return Fail("step graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    dev->graphGen = dev->configGen;
+    memcpy(dev->graphKey, key, sizeof key);
+    *out = dev->stepGraph;
+    return 0;
+}
+
+// steps time steps with the dt computed on the device, with no host round
+// trip in between.  clock (in/out) is the solver loop's clock, dts[i] the
+// steps taken (0 once the clock has reached its end), *advanced the steps
+// that moved the clock.  allowGraph: the step may be replayed from a
+// captured graph (never for the allocation-free march: instantiation
+// allocates).
 int MarchSteps(ArtraDevice *dev, const MarchScratch &ms, double clock[kMarchClock], const double floor[3], double cfl,
-               int steps, double *dts, int *advanced)
+               int steps, double *dts, int *advanced, bool allowGraph)
 {
     *advanced = 0;
     if (steps <= 0) return 0;
     if (steps > ms.cap) return Fail("a device march takes at most %d steps per call (%d asked)", ms.cap, steps);
     const double step0 = clock[4];
-    HIP_TRY(hipMemcpyAsync(ms.clock, clock, kMarchClock * sizeof(double), hipMemcpyHostToDevice, dev->stream));
-    StepGeometry sg;
-    for (int s = 0; s < 3; ++s) sg.d[s] = dev->par.d[s];
-    sg.cfl = cfl;
-    dev->stepDev = ms.step + 1;
+    // the clock and the step words (dt, dt / 2, the record index from 0)
+    // are contiguous
+    double head[kMarchClock + 3];
+    memcpy(head, clock, kMarchClock * sizeof(double));
+    head[kMarchClock] = head[kMarchClock + 1] = head[kMarchClock + 2] = 0.0;
+    HIP_TRY(hipMemcpyAsync(ms.clock, head, sizeof head, hipMemcpyHostToDevice, dev->stream));
+    hipGraphExec_t graph = nullptr;
+    if (allowGraph && StepGraph(dev, ms, floor, cfl, &graph)) return 1;
     int rc = 0;
-    for (int i = 0; i < steps && !rc; ++i) {
-        rc = EnqueueSpeeds(dev, floor, ms.vmax);
-        if (rc) break;
-        hipLaunchKernelGGL(dt_single, dim3(1), dim3(1), 0, dev->stream, sg, ms.vmax, ms.clock, ms.step, ms.hist + i);
-        if (hipGetLastError() != hipSuccess) rc = Fail("dt_single launch failed");
-        if (!rc) rc = artra_dev_evolve(dev, 0.0);
+    if (graph) {
+        for (int i = 0; i < steps && !rc; ++i) {
+            if (hipGraphLaunch(graph, dev->stream) != hipSuccess) rc = Fail("step graph launch failed");
+        }
+    } else {
+        dev->stepDev = ms.step + 1;
+        for (int i = 0; i < steps && !rc; ++i) rc = EnqueueStep(dev, ms, floor, cfl);
+        dev->stepDev = nullptr;
     }
-    dev->stepDev = nullptr;
     if (rc) return 1;
     HIP_TRY(hipMemcpyAsync(clock, ms.clock, kMarchClock * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipMemcpyAsync(dts, ms.hist, (size_t)steps * sizeof(double), hipMemcpyDeviceToHost, dev->stream));
@@ -2454,6 +2522,7 @@ void artra_dev_destroy(ArtraDevice *dev)
 {
     if (!dev) return;
     if (dev->stream) hipStreamSynchronize(dev->stream);
+    if (dev->stepGraph) hipGraphExecDestroy(dev->stepGraph);
     for (hipEvent_t ev : dev->evPool) hipEventDestroy(ev);
     hipFree(dev->U);
     hipFree(dev->exchange);
@@ -2487,6 +2556,7 @@ void artra_dev_destroy(ArtraDevice *dev)
 int artra_dev_set_boundary(ArtraDevice *dev, int mode, const ArtraBcBox *boxes, int nbox, const int *group_start,
                            int ngroup)
 {
+    ++dev->configGen;
     if (mode < 0 || mode > 3) return Fail("bad boundary mode %d", mode);
     std::vector<BcGroup> &groups = dev->groups[mode];
     groups.clear();
@@ -2521,6 +2591,7 @@ int artra_dev_set_boundary(ArtraDevice *dev, int mode, const ArtraBcBox *boxes, 
 
 int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
 {
+    ++dev->configGen;
     for (int m = 0; m < 4; ++m) {
         hipFree(dev->gDst[m]);
         hipFree(dev->gSrc[m]);
@@ -2547,6 +2618,7 @@ int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
 
 int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial)
 {
+    ++dev->configGen;
     dev->useGather = (use_gather != 0);
     dev->partialFill = (partial != 0);
     return 0;
@@ -2554,6 +2626,7 @@ int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial)
 
 int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *t)
 {
+    ++dev->configGen;
     dev->layerStart.assign(1, 0);
     dev->ibImageOnly = 0;
     if (UploadIbTable(dev, dev->ib, t)) return 1;
@@ -2574,6 +2647,7 @@ int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *jo
 
 int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count)
 {
+    ++dev->configGen;
     if (first < 0 || count < 0 || first + count > dev->g.N) return Fail("fluid flags [%ld, %ld) outside the grid", first, first + count);
     HIP_TRY(hipStreamSynchronize(dev->stream));
     if (count > 0) HIP_TRY(hipMemcpy(dev->fluid + first, fluid + first, (size_t)count, hipMemcpyHostToDevice));
@@ -2717,7 +2791,7 @@ int artra_dev_gather_primitive(ArtraDevice *dev, const int *nodes, int count, do
 int artra_dev_march_steps(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[3], double cfl, int steps,
                           double *dts, int *advanced)
 {
-    return MarchSteps(dev, MarchScratchAt(dev->marchBuf, kMarchChunk), clock, floor, cfl, steps, dts, advanced);
+    return MarchSteps(dev, MarchScratchAt(dev->marchBuf, kMarchChunk), clock, floor, cfl, steps, dts, advanced, true);
 }
 
 int artra_dev_march_noalloc(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK], const double floor[3], double cfl, int steps,
@@ -2730,7 +2804,7 @@ int artra_dev_march_noalloc(ArtraDevice *dev, double clock[ARTRA_DEV_MARCH_CLOCK
     if (dev->timing) return Fail("kernel timing is on: its events are allocated, so the allocation-free march refuses it");
     void *base = gScratch[dev->ordinal];
     if (!base) return Fail("no scratch address for device %d", dev->ordinal);
-    return MarchSteps(dev, MarchScratchAt(base, kNoAllocSteps), clock, floor, cfl, steps, dts, advanced);
+    return MarchSteps(dev, MarchScratchAt(base, kNoAllocSteps), clock, floor, cfl, steps, dts, advanced, false);
 }
 
 int artra_dev_evolve(ArtraDevice *dev, double dt)
@@ -2763,6 +2837,7 @@ int artra_dev_sync(ArtraDevice *dev)
 
 int artra_dev_set_timing(ArtraDevice *dev, int on)
 {
+    ++dev->configGen;
     if (Harvest(dev)) return 1;
     dev->timing = (on != 0);
     for (int q = 0; q < ARTRA_T_COUNT; ++q) {
@@ -2812,6 +2887,7 @@ int artra_dev_copy_probe(long bytes, int iters, double *gbps)
 
 int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float threshold)
 {
+    ++dev->configGen;
     if (params == nullptr) {
         dev->hybrid = 0;
         return 0;
@@ -2869,6 +2945,7 @@ int artra_dev_set_indicator(ArtraDevice *dev, const float *params, float thresho
 
 int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smooth_variation)
 {
+    ++dev->configGen;
     if (every_stage != 0)
         return Fail("the indicator classifies once per directional operator; per-stage classification is not provided");
     if (!(smooth_variation >= 0.0f)) return Fail("the smooth-variation gate must be non-negative");
@@ -2878,6 +2955,7 @@ int artra_dev_set_indicator_options(ArtraDevice *dev, int every_stage, float smo
 
 int artra_dev_set_fused(ArtraDevice *dev, int fused)
 {
+    ++dev->configGen;
     if (!fused && dev->hybrid) return Fail("the learned indicator needs the fused line sweeps");
     dev->fused = (fused != 0);
     return 0;
@@ -2950,14 +3028,23 @@ int artra_dev_sample_faces(ArtraDevice *dev, int axis, float *feat, float *label
     return 0;
 }
 
+int artra_dev_set_graph(ArtraDevice *dev, int on)
+{
+    ++dev->configGen;
+    dev->useGraph = on ? 1 : 0;
+    return 0;
+}
+
 int artra_dev_set_exact(ArtraDevice *dev, int exact)
 {
+    ++dev->configGen;
     dev->exact = (exact != 0);
     return 0;
 }
 
 int artra_dev_set_tuning(ArtraDevice *dev, int march_segment, int xblock)
 {
+    ++dev->configGen;
     (void)xblock;
     if (march_segment > 0) dev->segment = march_segment;
     return 0;

@@ -85,6 +85,9 @@ int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *jo
 /* new node flags after the bodies moved: fluid[q] != 0 for a fluid node
  * (the whole grid's array; entries [first, first + count) are copied) */
 int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count);
+/* replay the device march's step from a captured graph (1, the default) or
+ * launch its kernels one by one (0) */
+int artra_dev_set_graph(ArtraDevice *dev, int on);
 /* copy one time level: soa is [5][N] doubles */
 int artra_dev_upload(ArtraDevice *dev, int level, const double *soa);
 int artra_dev_download(ArtraDevice *dev, int level, double *soa);

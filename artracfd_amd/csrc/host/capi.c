@@ -405,6 +405,11 @@ int artra_hip_set_fill_options(ArtraHipSolver *s, int use_gather, int partial)
     return artra_dev_set_fill_options(s->dev, use_gather, partial) ? DevError() : 0;
 }
 
+int artra_hip_set_graph(ArtraHipSolver *s, int on)
+{
+    return artra_dev_set_graph(s->dev, on) ? DevError() : 0;
+}
+
 int artra_hip_set_arithmetic(ArtraHipSolver *s, int exact)
 {
     return artra_dev_set_exact(s->dev, exact) ? DevError() : 0;

@@ -72,6 +72,7 @@ def library():
         "artra_hip_set_tuning": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_arithmetic": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_fill_options": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "artra_hip_set_graph": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_fused": (ctypes.c_int, [vp, ctypes.c_int]),
         "artra_hip_set_indicator_options": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_float]),
         "artra_hip_fused_axes": (ctypes.c_int, [vp, ip]),
@@ -266,7 +267,8 @@ class Solver:
     ``initial_boundary`` applies TreatBoundary(TO) after the upload, which is
     the last step of the solver's InitializeComputeDomain."""
 
-    def __init__(self, case, initial_boundary=True, segment=0, exact=None, gather=True, partial=True, fused=True):
+    def __init__(self, case, initial_boundary=True, segment=0, exact=None, gather=True, partial=True, fused=True,
+                 graph=True):
         self.case = case
         lib = library()
         h = ctypes.c_void_p()
@@ -280,6 +282,8 @@ class Solver:
             _check(lib.artra_hip_set_tuning(h, segment))
         if not fused:
             _check(lib.artra_hip_set_fused(h, 0))
+        if not graph:
+            _check(lib.artra_hip_set_graph(h, 0))
         if initial_boundary:
             self.treat_boundary(TO)
 

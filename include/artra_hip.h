@@ -186,6 +186,13 @@ int artra_hip_set_arithmetic(ArtraHipSolver *solver, int exact);
  * result unchanged; 0, 0 restores the literal fill after every stage. */
 int artra_hip_set_fill_options(ArtraHipSolver *solver, int use_gather, int partial);
 
+/* on = 1 (default): artra_hip_march and artra_hip_solve replay each device
+ * step (CFL maxima, dt, the operators and fills) from a captured HIP graph,
+ * one launch per step; on = 0 launches the kernels one by one.  Same
+ * kernels, same results; kernel timing and the learned indicator always
+ * launch one by one.  No counterpart in the reference (a tuning switch). */
+int artra_hip_set_graph(ArtraHipSolver *solver, int on);
+
 /* Fused directional sweeps.  fused = 1 (default) runs all RK stages of a
  * split directional operator as one launch per grid line with the line
  * held in LDS, wherever a stage along that axis reads nothing but its own

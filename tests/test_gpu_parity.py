@@ -219,6 +219,29 @@ def test_fill_strategies_give_identical_results(name):
         np.testing.assert_array_equal(case_interior(name, f), case_interior(name, fields[0]))
 
 
+@pytest.mark.parametrize("name", ["sod", "tgv16", "cyl64", "v_gravity", "v_walls", "shockvortex64"])
+def test_graph_replayed_steps_equal_launched_steps(name):
+    """The device march replaying each step from a captured graph gives the
+    field and dt sequence of the march that launches the kernels one by one,
+    across march calls and after a change of arithmetic between them (the
+    graph is captured again)."""
+    print(f"graph replay {name}", flush=True)
+    from artracfd_amd import Case, Solver, library
+    runs = []
+    for graph in (True, False):
+        case = Case(case_dir(name))
+        solver = Solver(case, graph=graph)
+        _, d1 = solver.march(4)
+        _, d2 = solver.march(3)
+        library().artra_hip_set_arithmetic(solver._h, 1)
+        _, d3 = solver.march(3)
+        runs.append((np.concatenate([d1, d2, d3]), case.interior(solver.field()).copy()))
+        solver.close()
+        case.close()
+    assert len(runs[0][0]) == 10 and np.array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
 def case_interior(name, arr):
     g = golden(name)
     (x0, x1), (y0, y1), (z0, z1) = g["pin"]
