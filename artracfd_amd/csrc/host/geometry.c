@@ -684,25 +684,39 @@ int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTa
         }
         artra_free_facet_bounds(&bounds);
     }
-    /* interfacial and ghost layers (immersed_boundary.c:183-225); a fluid
-     * node that a moving body has uncovered (gst, the previous domain field,
-     * differs from did = 0) is listed for reconstruction with the stencil the
-     * flags give at this point of the pass, then marked */
+    /* interfacial and ghost layers (immersed_boundary.c:183-225).  A fluid
+     * node that a moving body has uncovered (gst, still the previous domain
+     * field, differs from did = 0) is reconstructed from the nodes that were
+     * fluid before and are fluid now; that is the set the reference's single
+     * pass finds whatever its order (a node the pass has reached holds gst 0
+     * and, if uncovered, the mark fid < 0; one it has not reached still holds
+     * its previous domain in gst), so the list is built first, on the flags
+     * as they stand, and the interfacial states, which read did only, follow
+     * in parallel */
     for (int k = lo[Z]; k < hi[Z]; ++k)
         for (int j = lo[Y]; j < hi[Y]; ++j)
             for (int i = lo[X]; i < hi[X]; ++i) {
-                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                const Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
                 if (nd->gst != nd->did && 0 == nd->did) {
                     if (NULL == joins) return artra_set_error(1, "a body uncovered node (%d, %d, %d)", k, j, i);
                     if (artra_add_join(space, k, j, i, joins)) return 1;
-                    nd->fid = NONE;
                 }
-                nd->lid = 0;
-                nd->gst = 0;
-                if (0 == nd->did) continue;
-                nd->lid = InterState(part, node, k, j, i, nd->did, 0);
-                if (0 < nd->lid) nd->gst = InterState(part, node, k, j, i, 0, 1);
             }
+    for (int q = 0; joins && q < joins->nghost; ++q) node[joins->ghosts[q].node].fid = NONE;
+    const int spanY = hi[Y] - lo[Y];
+    const int rows = (hi[Z] - lo[Z]) * (spanY > 0 ? spanY : 0);
+#pragma omp parallel for schedule(static) if (rows > 64)
+    for (int row = 0; row < rows; ++row) {
+        const int k = lo[Z] + row / spanY, j = lo[Y] + row % spanY;
+        for (int i = lo[X]; i < hi[X]; ++i) {
+            Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+            nd->lid = 0;
+            nd->gst = 0;
+            if (0 == nd->did) continue;
+            nd->lid = InterState(part, node, k, j, i, nd->did, 0);
+            if (0 < nd->lid) nd->gst = InterState(part, node, k, j, i, 0, 1);
+        }
+    }
     return 0;
 }
 
