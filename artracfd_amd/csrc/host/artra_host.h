@@ -209,9 +209,10 @@ void artra_free_ghost_table(ArtraGhostTable *table);
  * Nodes that a moving body has just uncovered (immersed_boundary.c:193-204):
  * artra_add_join appends node (k, j, i) as an entry of kind 1 (normalized
  * inverse-distance weighting) whose stencil is the search from radius 2 over
- * the fluid nodes that were fluid before the move and are not themselves
- * newly uncovered, evaluated on the node flags as they stand when the
- * classification pass reaches the node.  artra_apply_joins evaluates the
+ * the fluid nodes that were fluid before the move (gst, still holding the
+ * previous domain field, is 0, and fid is not the reconstruction mark),
+ * read from the flags before the interfacial pass changes them.
+ * artra_apply_joins evaluates the
  * entries on the host node array (level tn); the device evaluates the same
  * table with the immersed-boundary kernel.
  */
