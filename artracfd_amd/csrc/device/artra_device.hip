@@ -1683,6 +1683,7 @@ struct ArtraDevice {
     std::vector<BcGroup> groups[4];          // mode 0: full fill; 1 + s: faces of axis s only
     BcArgs bc{};
     int gatherResolved = 0;
+    int axisGather[3] = {0, 0, 0};  // the fill between the stages of a sweep along s is a gather (walls elsewhere)
     int useGather = 1;
     int *gDst[4] = {nullptr, nullptr, nullptr, nullptr};
     int *gSrc[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -1941,7 +1942,7 @@ int TreatBoundary(ArtraDevice *dev, int level, int mode = 0)
     double *U = Level(dev, level);
     // immersed boundary first
     if (ImmersedFill(dev, level)) return 1;
-    if (dev->gatherResolved && dev->useGather) {
+    if ((dev->gatherResolved || (mode > 0 && dev->axisGather[mode - 1])) && dev->useGather) {
         const long count = dev->gCount[mode];
         if (count <= 0) return 0;
         GatherArgs ga;
@@ -2599,6 +2600,7 @@ int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
         dev->gCount[m] = 0;
     }
     dev->gatherResolved = 0;
+    for (int s = 0; s < 3; ++s) dev->axisGather[s] = 0;
     DeriveLineRules(dev, gt);
     if (!gt->resolved) return 0;
     for (int m = 0; m < 4; ++m) {
@@ -2613,6 +2615,30 @@ int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gt)
         HIP_TRY(hipMemcpy(dev->gSrc[m], q, n * sizeof(int), hipMemcpyHostToDevice));
     }
     dev->gatherResolved = 1;
+    return 0;
+}
+
+int artra_dev_set_axis_gather(ArtraDevice *dev, int s, const ArtraBcGather *gt)
+{
+    ++dev->configGen;
+    if (s < 0 || s > 2) return Fail("bad axis %d", s);
+    if (dev->gatherResolved) return 0;              // the full gather's axis tables serve already
+    const int m = 1 + s;
+    hipFree(dev->gDst[m]);
+    hipFree(dev->gSrc[m]);
+    dev->gDst[m] = dev->gSrc[m] = nullptr;
+    dev->gCount[m] = 0;
+    dev->axisGather[s] = 0;
+    if (!gt->resolved) return 0;
+    const long n = gt->count;
+    if (n > 0) {
+        HIP_TRY(hipMalloc(&dev->gDst[m], n * sizeof(int)));
+        HIP_TRY(hipMalloc(&dev->gSrc[m], n * sizeof(int)));
+        HIP_TRY(hipMemcpy(dev->gDst[m], gt->dst, n * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(dev->gSrc[m], gt->src, n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    dev->gCount[m] = n;
+    dev->axisGather[s] = 1;
     return 0;
 }
 

@@ -78,6 +78,9 @@ int artra_dev_set_gather(ArtraDevice *dev, const ArtraBcGather *gather);
 /* use_gather = 0 forces the box groups; partial = 0 fills everything after
  * every stage */
 int artra_dev_set_fill_options(ArtraDevice *dev, int use_gather, int partial);
+/* the resolved fill between the stages of a sweep along s when the full
+ * fill is not resolved (artra_bc_resolve_axis) */
+int artra_dev_set_axis_gather(ArtraDevice *dev, int s, const ArtraBcGather *gather);
 int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *table);
 /* the nodes a moving body has just uncovered: their reconstruction
  * (artra_add_join entries) evaluated on level `level` */

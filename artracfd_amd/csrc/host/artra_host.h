@@ -166,6 +166,12 @@ typedef struct {
 } ArtraBcGather;
 
 int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, ArtraBcGather *out);
+/* The same replay for the boxes of the faces normal to axis s alone (the fill
+ * between the stages of a sweep along s), kept to the nodes outside the
+ * interior range along s only, the ghosts that sweep reads: resolved when
+ * each of those is a copy of a node the fill does not write or an inflow
+ * state (dst/src/count hold them; the axis tables are unused). */
+int artra_bc_resolve_axis(const Space *space, const ArtraBcBox *boxes, int nbox, int s, ArtraBcGather *out);
 void artra_free_bc_gather(ArtraBcGather *g);
 
 /*

@@ -101,9 +101,23 @@ static int SetBoundaryTables(ArtraHipSolver *s)
     }
     ArtraBcGather gather;
     if (artra_bc_resolve(s->space, boxes, nbox, &gather)) return 1;
-    const int rc = artra_dev_set_gather(s->dev, &gather);
+    int rc = artra_dev_set_gather(s->dev, &gather);
+    const int resolved = gather.resolved;
     artra_free_bc_gather(&gather);
-    return rc ? DevError() : 0;
+    if (rc) return DevError();
+    /* walls keep the full fill ordered; the fills between the stages of a
+     * sweep along an axis whose faces resolve become gathers */
+    for (int a = 0; a < DIMS && !resolved; ++a) {
+        int n = 0;
+        for (int b = 0; b < nbox; ++b) {
+            if ((boxes[b].face - PWB) / 2 == a) axisBoxes[n++] = boxes[b];
+        }
+        if (artra_bc_resolve_axis(s->space, axisBoxes, n, a, &gather)) return 1;
+        rc = artra_dev_set_axis_gather(s->dev, a, &gather);
+        artra_free_bc_gather(&gather);
+        if (rc) return DevError();
+    }
+    return 0;
 }
 
 int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)

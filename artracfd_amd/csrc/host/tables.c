@@ -479,13 +479,11 @@ static int LabelOf(const int *label, const Partition *part, int k, int j, int i)
     return label[artra_index(k, j, i, part->n[Y], part->n[X])];
 }
 
-int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, ArtraBcGather *out)
+/* The symbolic replay of the boxes in order: label[q] as described above. */
+static void ReplayLabels(const Space *space, const ArtraBcBox *boxes, int nbox, int *label)
 {
     const Partition *part = &space->part;
     const long N = (long)part->n[X] * part->n[Y] * part->n[Z];
-    memset(out, 0, sizeof *out);
-    int *label = malloc(N * sizeof(int));
-    if (NULL == label) return artra_set_error(1, "out of host memory");
     for (long q = 0; q < N; ++q) label[q] = LABEL_STALE;
     for (int b = 0; b < nbox; ++b) {
         const ArtraBcBox *bx = boxes + b;
@@ -511,6 +509,74 @@ int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, Artr
                     label[artra_index(k, j, i, part->n[Y], part->n[X])] = lab;
                 }
     }
+}
+
+/* A node outside the interior range along axis s only. */
+static int AxisOnly(const Partition *part, int q, int s)
+{
+    const int c[DIMS] = {q % part->n[X], (q / part->n[X]) % part->n[Y], q / (part->n[X] * part->n[Y])};
+    for (int t = 0; t < DIMS; ++t) {
+        if (t == s) continue;
+        if (c[t] < part->ns[PIN][t][MIN] || c[t] >= part->ns[PIN][t][MAX]) return 0;
+    }
+    return !(c[s] >= part->ns[PIN][s][MIN] && c[s] < part->ns[PIN][s][MAX]);
+}
+
+int artra_bc_resolve_axis(const Space *space, const ArtraBcBox *boxes, int nbox, int s, ArtraBcGather *out)
+{
+    const Partition *part = &space->part;
+    const long N = (long)part->n[X] * part->n[Y] * part->n[Z];
+    memset(out, 0, sizeof *out);
+    int *label = malloc(N * sizeof(int));
+    if (NULL == label) return artra_set_error(1, "out of host memory");
+    ReplayLabels(space, boxes, nbox, label);
+    /* every written node outside the interior along s only must resolve */
+    long count = 0;
+    int ok = 1;
+    for (int b = 0; b < nbox && ok; ++b) {
+        const ArtraBcBox *bx = boxes + b;
+        for (int k = bx->box[Z][MIN]; k < bx->box[Z][MAX] && ok; ++k)
+            for (int j = bx->box[Y][MIN]; j < bx->box[Y][MAX] && ok; ++j)
+                for (int i = bx->box[X][MIN]; i < bx->box[X][MAX]; ++i) {
+                    const int q = artra_index(k, j, i, part->n[Y], part->n[X]);
+                    if (AxisOnly(part, q, s) && !(label[q] >= 0 || label[q] <= LABEL_CONST)) {
+                        ok = 0;
+                        break;
+                    }
+                }
+    }
+    for (long q = 0; q < N && ok; ++q) count += AxisOnly(part, (int)q, s) && (label[q] >= 0 || label[q] <= LABEL_CONST);
+    if (ok) {
+        out->dst = malloc((count ? count : 1) * sizeof(int));
+        out->src = malloc((count ? count : 1) * sizeof(int));
+        if (!out->dst || !out->src) {
+            free(label);
+            artra_free_bc_gather(out);
+            return artra_set_error(1, "out of host memory");
+        }
+        long n = 0;
+        for (long q = 0; q < N; ++q) {
+            if (AxisOnly(part, (int)q, s) && (label[q] >= 0 || label[q] <= LABEL_CONST)) {
+                out->dst[n] = (int)q;
+                out->src[n] = label[q];
+                ++n;
+            }
+        }
+        out->count = n;
+        out->resolved = 1;
+    }
+    free(label);
+    return 0;
+}
+
+int artra_bc_resolve(const Space *space, const ArtraBcBox *boxes, int nbox, ArtraBcGather *out)
+{
+    const Partition *part = &space->part;
+    const long N = (long)part->n[X] * part->n[Y] * part->n[Z];
+    memset(out, 0, sizeof *out);
+    int *label = malloc(N * sizeof(int));
+    if (NULL == label) return artra_set_error(1, "out of host memory");
+    ReplayLabels(space, boxes, nbox, label);
     /* count entries; any written node that does not resolve disables the gather */
     long count = 0;
     int ok = 1;
