@@ -2211,21 +2211,26 @@ void DeriveLineRules(ArtraDevice *dev, const ArtraBcGather *gt)
 
 // Runge-Kutta stages of one directional operator (fluid_dynamics.c:121-148).
 // Between the stages of an inviscid sweep along one axis only that axis's
-// ghosts are read, so only they are filled; the last stage fills everything,
-// exactly as TreatBoundary does after every stage.
-int Stages(ArtraDevice *dev, double dt, int p)
+// ghosts are read, so only they are filled.  After the last stage the fill
+// covers the ghosts the next operator of the step reads (next = 1 + its
+// axis), or everything at the end of the step (next = 0), exactly as
+// TreatBoundary after every stage leaves them: each fill's values derive
+// from interior nodes and from nodes written earlier in the same fill.
+int Stages(ArtraDevice *dev, double dt, int p, int next)
 {
-    if (p < 3 && LineEligible(dev, p, dev->hybrid != 0)) return LineSweep(dev, dt, p) || TreatBoundary(dev, 0);
+    const int lazy = (p < 3 && !(dev->mdl.refMu > 0.0) && dev->partialFill);
+    const int mid = lazy ? 1 + p : 0;
+    const int last = lazy ? next : 0;
+    if (p < 3 && LineEligible(dev, p, dev->hybrid != 0)) return LineSweep(dev, dt, p) || TreatBoundary(dev, 0, last);
     if (dev->hybrid) return Fail("the learned indicator is on but axis %d cannot run fused", p);
-    const int mid = (p < 3 && !(dev->mdl.refMu > 0.0) && dev->partialFill) ? 1 + p : 0;
     if (dev->par.tScheme == 0) {
         if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1, mid)) return 1;
-        if (Sweep(dev, dt, 1.0 / 2.0, 1.0 / 2.0, 0, 1, 0, p) || TreatBoundary(dev, 0)) return 1;
+        if (Sweep(dev, dt, 1.0 / 2.0, 1.0 / 2.0, 0, 1, 0, p) || TreatBoundary(dev, 0, last)) return 1;
         return 0;
     }
     if (Sweep(dev, dt, 0.0, 1.0, 0, 0, 1, p) || TreatBoundary(dev, 1, mid)) return 1;
     if (Sweep(dev, dt, 3.0 / 4.0, 1.0 / 4.0, 0, 1, 2, p) || TreatBoundary(dev, 2, mid)) return 1;
-    if (Sweep(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0, p) || TreatBoundary(dev, 0)) return 1;
+    if (Sweep(dev, dt, 1.0 / 3.0, 2.0 / 3.0, 0, 2, 0, p) || TreatBoundary(dev, 0, last)) return 1;
     return 0;
 }
 
@@ -2839,7 +2844,7 @@ int artra_dev_evolve(ArtraDevice *dev, double dt)
     // gravity source half-steps around the fluid operators (fluid_dynamics.c:57-59, 112-114)
     if (dev->par.sState && SourceStages(dev, h)) return 1;
     if (dev->par.multidim == 1) {
-        if (Stages(dev, h, 3) || Stages(dev, h, 3)) return 1;
+        if (Stages(dev, h, 3, 0) || Stages(dev, h, 3, 0)) return 1;
     } else {
         static const int order[9][6] = {{2, 1, 0, 0, 1, 2},    {2, 1, 1, 2, -1, -1},   {2, 0, 0, 2, -1, -1},
                                         {1, 0, 0, 1, -1, -1},  {-1, -1, -1, -1, -1, -1}, {2, 2, -1, -1, -1, -1},
@@ -2847,7 +2852,8 @@ int artra_dev_evolve(ArtraDevice *dev, double dt)
         const int c = dev->par.collapse;
         if (c < 0 || c > 8 || order[c][0] < 0) return Fail("unsupported collapse flag %d", c);
         for (int q = 0; q < 6 && order[c][q] >= 0; ++q) {
-            if (Stages(dev, h, order[c][q])) return 1;
+            const int more = (q + 1 < 6 && order[c][q + 1] >= 0);
+            if (Stages(dev, h, order[c][q], more ? 1 + order[c][q + 1] : 0)) return 1;
         }
     }
     if (dev->par.sState && SourceStages(dev, h)) return 1;
