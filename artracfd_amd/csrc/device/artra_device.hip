@@ -1177,6 +1177,22 @@ struct GhostDev {
     double Vs[3], N[3], Ta[3], Tb[3];
 };
 
+// Node flags of a box, packed k-j-i, written into the grid's flag array.
+struct FlagBox {
+    int lo[3], n[3];
+};
+
+__global__ __launch_bounds__(256) void scatter_flags(uint8_t *fluid, const uint8_t *packed, Grid g, FlagBox b,
+                                                     long count)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int i = b.lo[0] + (int)(t % b.n[0]);
+    const int j = b.lo[1] + (int)((t / b.n[0]) % b.n[1]);
+    const int k = b.lo[2] + (int)(t / ((long)b.n[0] * b.n[1]));
+    fluid[((long)k * g.ny + j) * g.nx + i] = packed[t];
+}
+
 // A table of filled nodes on the device with its stencil entries; the
 // buffers grow as needed and are reused when the table is rebuilt.
 struct IbTable {
@@ -1660,6 +1676,8 @@ struct ArtraDevice {
     double *U = nullptr;            // [3][N][5]: three levels, node-major
     double *exchange = nullptr;     // [5][N]: host-layout staging of one level
     uint8_t *fluid = nullptr;
+    uint8_t *flagStage = nullptr;   // packed flags of a changed box (moving bodies)
+    long flagStageCap = 0;
     unsigned long long *vmax = nullptr;
     double *marchBuf = nullptr;     // device march: speeds, clock, step, step record (kMarchChunk)
     // surface force and probes
@@ -2533,6 +2551,7 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->U);
     hipFree(dev->exchange);
     hipFree(dev->fluid);
+    hipFree(dev->flagStage);
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
     hipFree(dev->forceNodes);
@@ -2676,13 +2695,51 @@ int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *jo
     return LaunchIb(dev, Level(dev, level), dev->joins, 0, joins->nghost);
 }
 
-int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count)
+int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, const int (*box)[2])
 {
     ++dev->configGen;
-    if (first < 0 || count < 0 || first + count > dev->g.N) return Fail("fluid flags [%ld, %ld) outside the grid", first, first + count);
+    const Grid &g = dev->g;
     HIP_TRY(hipStreamSynchronize(dev->stream));
-    if (count > 0) HIP_TRY(hipMemcpy(dev->fluid + first, fluid + first, (size_t)count, hipMemcpyHostToDevice));
-    dev->allFluid = AllFluid(dev->g, fluid);
+    if (!box) {
+        HIP_TRY(hipMemcpy(dev->fluid, fluid, (size_t)g.N, hipMemcpyHostToDevice));
+    } else {
+        const int ext[3] = {g.nx, g.ny, g.nz};
+        int lo[3], hi[3];
+        long count = 1;
+        for (int s = 0; s < 3; ++s) {
+            lo[s] = box[s][0] > 0 ? box[s][0] : 0;
+            hi[s] = box[s][1] < ext[s] ? box[s][1] : ext[s];
+            count *= (hi[s] > lo[s]) ? (hi[s] - lo[s]) : 0;
+        }
+        if (count > 0) {
+            // the box packed on the host, one copy, scattered on the device
+            std::vector<uint8_t> packed((size_t)count);
+            long n = 0;
+            for (int k = lo[2]; k < hi[2]; ++k)
+                for (int j = lo[1]; j < hi[1]; ++j) {
+                    memcpy(packed.data() + n, fluid + ((long)k * g.ny + j) * g.nx + lo[0], (size_t)(hi[0] - lo[0]));
+                    n += hi[0] - lo[0];
+                }
+            if (count > dev->flagStageCap) {
+                hipFree(dev->flagStage);
+                dev->flagStage = nullptr;
+                dev->flagStageCap = 0;
+                HIP_TRY(hipMalloc(&dev->flagStage, (size_t)count));
+                dev->flagStageCap = count;
+            }
+            HIP_TRY(hipMemcpy(dev->flagStage, packed.data(), (size_t)count, hipMemcpyHostToDevice));
+            FlagBox fbx;
+            for (int s = 0; s < 3; ++s) {
+                fbx.lo[s] = lo[s];
+                fbx.n[s] = hi[s] - lo[s];
+            }
+            hipLaunchKernelGGL(scatter_flags, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, dev->stream,
+                               dev->fluid, dev->flagStage, g, fbx, count);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(dev->stream));
+        }
+    }
+    dev->allFluid = AllFluid(g, fluid);
     return 0;
 }
 

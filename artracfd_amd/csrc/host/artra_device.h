@@ -86,8 +86,9 @@ int artra_dev_set_ghosts(ArtraDevice *dev, const ArtraGhostTable *table);
  * (artra_add_join entries) evaluated on level `level` */
 int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *joins);
 /* new node flags after the bodies moved: fluid[q] != 0 for a fluid node
- * (the whole grid's array; entries [first, first + count) are copied) */
-int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, long first, long count);
+ * (the whole grid's array); box (may be NULL: all) bounds the nodes copied,
+ * [min, max) per axis */
+int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, const int (*box)[2]);
 /* replay the device march's step from a captured graph (1, the default) or
  * launch its kernels one by one (0) */
 int artra_dev_set_graph(ArtraDevice *dev, int on);

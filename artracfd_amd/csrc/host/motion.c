@@ -275,28 +275,22 @@ int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT])
 {
     const Space *space = s->space;
     const Partition *part = &space->part;
-    long first = 0, last = s->N;          /* index span of the changed flags */
+    int box[DIMS][LIMIT];                   /* the changed flags' node box */
     if (NULL == region) {
         for (long q = 0; q < s->N; ++q) s->fluid[q] = (0 == space->node[q].did);
     } else {
-        int lo[DIMS], hi[DIMS];
         for (int d = 0; d < DIMS; ++d) {
-            lo[d] = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
-            hi[d] = (region[d][MAX] < part->ns[PIN][d][MAX]) ? region[d][MAX] : part->ns[PIN][d][MAX];
+            box[d][MIN] = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
+            box[d][MAX] = (region[d][MAX] < part->ns[PIN][d][MAX]) ? region[d][MAX] : part->ns[PIN][d][MAX];
         }
-        first = last = 0;
-        if (lo[X] < hi[X] && lo[Y] < hi[Y] && lo[Z] < hi[Z]) {
-            for (int k = lo[Z]; k < hi[Z]; ++k)
-                for (int j = lo[Y]; j < hi[Y]; ++j)
-                    for (int i = lo[X]; i < hi[X]; ++i) {
-                        const int q = artra_index(k, j, i, part->n[Y], part->n[X]);
-                        s->fluid[q] = (0 == space->node[q].did);
-                    }
-            first = artra_index(lo[Z], lo[Y], lo[X], part->n[Y], part->n[X]);
-            last = artra_index(hi[Z] - 1, hi[Y] - 1, hi[X] - 1, part->n[Y], part->n[X]) + 1;
-        }
+        for (int k = box[Z][MIN]; k < box[Z][MAX]; ++k)
+            for (int j = box[Y][MIN]; j < box[Y][MAX]; ++j)
+                for (int i = box[X][MIN]; i < box[X][MAX]; ++i) {
+                    const int q = artra_index(k, j, i, part->n[Y], part->n[X]);
+                    s->fluid[q] = (0 == space->node[q].did);
+                }
     }
-    if (artra_dev_set_fluid(s->dev, s->fluid, first, last - first)) return artra_dev_failed();
+    if (artra_dev_set_fluid(s->dev, s->fluid, region ? (const int (*)[LIMIT])box : NULL)) return artra_dev_failed();
     artra_free_ghost_table(&s->ghosts);
     if (artra_build_ghost_table(space, s->model, &s->ghosts)) return 1;
     if (artra_dev_set_ghosts(s->dev, &s->ghosts)) return artra_dev_failed();
