@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +36,7 @@
 extern "C" {
 #include "artra_device.h"
 }
+#include "geomcore.h"
 
 using artra::FluxModel;
 
@@ -1193,6 +1195,161 @@ __global__ __launch_bounds__(256) void scatter_flags(uint8_t *fluid, const uint8
     fluid[((long)k * g.ny + j) * g.nx + i] = packed[t];
 }
 
+// ------------------------------------------ moving-body classification
+// (the grid geometry and search path: ArtraGeoSetup, artra_device.h)
+typedef ArtraGeoSetup GeoSetup;
+
+// A body's arrays on the device (triangulated bodies; the connectivity is
+// uploaded once, the positions and normals after every move).
+struct GeoBodyDev {
+    int faceN = 0, edgeN = 0, vertN = 0;
+    int *f = nullptr, *e = nullptr;
+    double *v = nullptr, *Nf = nullptr, *Ne = nullptr, *Nv = nullptr;
+    bool uploaded = false;
+};
+
+// A node box [lo, hi) and its node count.
+struct GeoBox {
+    int lo[3], n[3];
+    long count;
+};
+
+__device__ __forceinline__ long GeoNode(const Grid &g, const GeoBox &b, long t, int &i, int &j, int &k)
+{
+    i = b.lo[0] + (int)(t % b.n[0]);
+    j = b.lo[1] + (int)((t / b.n[0]) % b.n[1]);
+    k = b.lo[2] + (int)(t / ((long)b.n[0] * b.n[1]));
+    return ((long)k * g.ny + j) * g.nx + i;
+}
+
+__device__ __forceinline__ void GeoPoint(const GeoSetup &gs, int i, int j, int k, double p[3])
+{
+    p[0] = gs.smin[0] + (i - gs.ng[0]) * gs.d[0];
+    p[1] = gs.smin[1] + (j - gs.ng[1]) * gs.d[1];
+    p[2] = gs.smin[2] + (k - gs.ng[2]) * gs.d[2];
+}
+
+// Pass 1 (immersed_boundary.c:72-113): keep the domain field in gst, clear
+// the fluid nodes' face links, release the interfacial nodes of moving bodies.
+__global__ __launch_bounds__(256) void geo_reset(Grid g, GeoBox b, int *did, int *fid, const int *lid, int *gst,
+                                                 const int *moving, int nbody)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count) return;
+    int i, j, k;
+    const long q = GeoNode(g, b, t, i, j, k);
+    const int gid = did[q];
+    gst[q] = gid;
+    if (0 == gid) {
+        fid[q] = 0;
+        return;
+    }
+    if (gid <= nbody && !moving[gid - 1]) return;
+    if (0 < lid[q]) did[q] = 0;
+}
+
+// Pass 2 for a sphere (immersed_boundary.c:128-182): nodes of its box not yet
+// classified that lie in or on it.
+__global__ __launch_bounds__(256) void geo_sphere(Grid g, GeoSetup gs, GeoBox b, int *did, int *fid, double Ox,
+                                                  double Oy, double Oz, double r, int body)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count) return;
+    int i, j, k;
+    const long q = GeoNode(g, b, t, i, j, k);
+    if (0 != did[q]) return;
+    double p[3];
+    GeoPoint(gs, i, j, k, p);
+    const double dv[3] = {p[0] - Ox, p[1] - Oy, p[2] - Oz};
+    if (r * r >= GcDot(dv, dv)) {
+        did[q] = body + 1;
+        fid[q] = 0;
+    }
+}
+
+// Pass 2 for a triangulated body (computational_geometry.c:463-506): one
+// wavefront per node; lane l measures facets l, l + 64, ... keeping its first
+// facet at its minimum, the lanes' (distance, facet) pairs reduce to the
+// lexicographic minimum (the first facet at the global minimum, as the
+// serial scan), and the pseudo-normal at that facet decides.
+__global__ __launch_bounds__(256) void geo_mesh(Grid g, GeoSetup gs, GeoBox b, int *did, int *fid, ArtraPolyView pv,
+                                                int body)
+{
+    const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (t >= b.count) return;                       // whole wavefronts leave together
+    const int lane = __lane_id();
+    int i, j, k;
+    const long q = GeoNode(g, b, t, i, j, k);
+    if (0 != did[q]) return;                        // the same node on every lane
+    double p[3];
+    GeoPoint(gs, i, j, k, p);
+    double best = FLT_MAX;                          // the serial scan's start: facet 0 at FLT_MAX
+    int cid = 0;
+    for (int n = lane; n < pv.faceN; n += 64) {
+        const double d2 = GcFacetDistance2(p, n, &pv);
+        if (best > d2) {
+            best = d2;
+            cid = n;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oc = __shfl_xor(cid, off);
+        if (ob < best || (ob == best && oc < cid)) {
+            best = ob;
+            cid = oc;
+        }
+    }
+    if (lane == 0 && GcInsideOfFacet(p, cid, &pv)) {
+        did[q] = body + 1;
+        fid[q] = cid;
+    }
+}
+
+// The first search-path layer (1..gl) holding a node whose did equals (same)
+// or differs from (!same) `did`, or 0 (immersed_boundary.c:227-264).
+__device__ int GeoInterState(const Grid &g, const GeoSetup &gs, const int *didv, int k, int j, int i, int did,
+                             bool same)
+{
+    for (int n = 0; n < gs.pathSep[0]; ++n) {
+        const int kh = k + gs.path[n][2], jh = j + gs.path[n][1], ih = i + gs.path[n][0];
+        if (kh < g.lo[2] || kh >= g.hi[2] || jh < g.lo[1] || jh >= g.hi[1] || ih < g.lo[0] || ih >= g.hi[0]) continue;
+        const int other = didv[((long)kh * g.ny + jh) * g.nx + ih];
+        if ((same && other == did) || (!same && other != did)) {
+            for (int r = 1; r <= gs.gl; ++r) {
+                if (gs.pathSep[r] > n) return r;
+            }
+        }
+    }
+    return 0;
+}
+
+// Pass 3's interfacial states (immersed_boundary.c:183-225, which read did
+// only), the sweeps' fluid flags, and the region's flags packed for the host:
+// packed[0..4][t] = did, fid, lid, gst and the previous domain field.
+__global__ __launch_bounds__(256) void geo_interface(Grid g, GeoSetup gs, GeoBox b, const int *did, const int *fid,
+                                                     int *lid, int *gst, uint8_t *fluid, int *packed)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count) return;
+    int i, j, k;
+    const long q = GeoNode(g, b, t, i, j, k);
+    const int dq = did[q];
+    int l = 0, s = 0;
+    if (0 != dq) {
+        l = GeoInterState(g, gs, did, k, j, i, dq, false);
+        if (0 < l) s = GeoInterState(g, gs, did, k, j, i, 0, true);
+    }
+    packed[t] = dq;
+    packed[b.count + t] = fid[q];
+    packed[2 * b.count + t] = l;
+    packed[3 * b.count + t] = s;
+    packed[4 * b.count + t] = gst[q];               // the previous domain field (pass 1)
+    lid[q] = l;
+    gst[q] = s;
+    fluid[q] = (0 == dq);
+}
+
 // A table of filled nodes on the device with its stencil entries; the
 // buffers grow as needed and are reused when the table is rebuilt.
 struct IbTable {
@@ -1709,6 +1866,14 @@ struct ArtraDevice {
     IbTable ib;                     // ghost nodes, grouped by layer
     int ibImageOnly = 0;            // every ghost reconstructs from its image point (no layer reads another)
     IbTable joins;                  // nodes a moving body has just uncovered
+    // the node classification of moving bodies (did, fid, lid, gst), kept
+    // on the device and reclassified there after every move
+    int *geoFlags = nullptr;        // [4][N]
+    int *geoPacked = nullptr;       // [4][region nodes] for the host's copy
+    long geoPackedCap = 0;
+    int *geoMoving = nullptr;       // [bodies]: 1 for a body that moves
+    GeoSetup geo{};
+    std::vector<GeoBodyDev> geoBodies;
     std::vector<int> layerStart;
     FluxModel mdl{};
     int segment = 32;
@@ -2552,6 +2717,12 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->exchange);
     hipFree(dev->fluid);
     hipFree(dev->flagStage);
+    hipFree(dev->geoFlags);
+    hipFree(dev->geoPacked);
+    hipFree(dev->geoMoving);
+    for (GeoBodyDev &b : dev->geoBodies) {
+        hipFree(b.f); hipFree(b.e); hipFree(b.v); hipFree(b.Nf); hipFree(b.Ne); hipFree(b.Nv);
+    }
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
     hipFree(dev->forceNodes);
@@ -3681,6 +3852,136 @@ int artra_dev_batch_tune(ArtraBatch *b, int reps, int *threads_out, int *grid_ou
     for (int c = 0; c < b->C; ++c)
         if (threads_out) threads_out[c] = b->threads[c];
     if (grid_out) *grid_out = b->grid;
+    return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------ moving-body classification API
+extern "C" {
+
+int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *flags, int nbody)
+{
+    const long N = dev->g.N;
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    if (!dev->geoFlags) HIP_TRY(hipMalloc(&dev->geoFlags, (size_t)4 * N * sizeof(int)));
+    HIP_TRY(hipMemcpy(dev->geoFlags, flags, (size_t)4 * N * sizeof(int), hipMemcpyHostToDevice));
+    if ((int)dev->geoBodies.size() != nbody) {
+        for (GeoBodyDev &b : dev->geoBodies) {
+            hipFree(b.f); hipFree(b.e); hipFree(b.v); hipFree(b.Nf); hipFree(b.Ne); hipFree(b.Nv);
+        }
+        dev->geoBodies.assign(nbody, GeoBodyDev());
+        hipFree(dev->geoMoving);
+        dev->geoMoving = nullptr;
+        HIP_TRY(hipMalloc(&dev->geoMoving, (size_t)(nbody > 0 ? nbody : 1) * sizeof(int)));
+    }
+    dev->geo = *setup;
+    return 0;
+}
+
+int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int region[3][2],
+                           int *packed, long *count)
+{
+    *count = 0;
+    if (!dev->geoFlags || nbody != (int)dev->geoBodies.size()) return Fail("moving-body classification not set up");
+    ++dev->configGen;
+    dev->allFluid = 0;                              // bodies present: the stage-by-stage sweeps
+    const Grid &g = dev->g;
+    const long N = g.N;
+    int *did = dev->geoFlags, *fid = did + N, *lid = fid + N, *gst = lid + N;
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    std::vector<int> moving(nbody > 0 ? nbody : 1, 0);
+    for (int n = 0; n < nbody; ++n) {
+        const ArtraGeoBody &bd = bodies[n];
+        moving[n] = bd.moving;
+        if (bd.pv.faceN <= 0) continue;
+        GeoBodyDev &gb = dev->geoBodies[n];
+        const size_t fb = (size_t)bd.pv.faceN * 3 * sizeof(double), eb = (size_t)bd.pv.edgeN * 3 * sizeof(double),
+                     vb = (size_t)bd.pv.vertN * 3 * sizeof(double);
+        if (!gb.uploaded) {
+            gb.faceN = bd.pv.faceN;
+            gb.edgeN = bd.pv.edgeN;
+            gb.vertN = bd.pv.vertN;
+            HIP_TRY(hipMalloc(&gb.f, (size_t)gb.faceN * 3 * sizeof(int)));
+            HIP_TRY(hipMalloc(&gb.e, (size_t)gb.edgeN * 4 * sizeof(int)));
+            HIP_TRY(hipMalloc(&gb.v, vb));
+            HIP_TRY(hipMalloc(&gb.Nf, fb));
+            HIP_TRY(hipMalloc(&gb.Ne, eb));
+            HIP_TRY(hipMalloc(&gb.Nv, vb));
+            HIP_TRY(hipMemcpy(gb.f, bd.pv.f, (size_t)gb.faceN * 3 * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(gb.e, bd.pv.e, (size_t)gb.edgeN * 4 * sizeof(int), hipMemcpyHostToDevice));
+        } else if (gb.faceN != bd.pv.faceN || gb.edgeN != bd.pv.edgeN || gb.vertN != bd.pv.vertN) {
+            return Fail("body %d changed its facet count", n);
+        }
+        if (!gb.uploaded || bd.moving) {
+            HIP_TRY(hipMemcpy(gb.v, bd.pv.v, vb, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(gb.Nf, bd.pv.Nf, fb, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(gb.Ne, bd.pv.Ne, eb, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(gb.Nv, bd.pv.Nv, vb, hipMemcpyHostToDevice));
+        }
+        gb.uploaded = true;
+    }
+    HIP_TRY(hipMemcpy(dev->geoMoving, moving.data(), moving.size() * sizeof(int), hipMemcpyHostToDevice));
+    // the region within the interior range
+    GeoBox rb;
+    rb.count = 1;
+    for (int s = 0; s < 3; ++s) {
+        const int lo = region[s][0] > g.lo[s] ? region[s][0] : g.lo[s];
+        const int hi = region[s][1] < g.hi[s] ? region[s][1] : g.hi[s];
+        rb.lo[s] = lo;
+        rb.n[s] = hi > lo ? hi - lo : 0;
+        rb.count *= rb.n[s];
+    }
+    if (rb.count == 0) return 0;
+    if (5 * rb.count > dev->geoPackedCap) {
+        hipFree(dev->geoPacked);
+        dev->geoPacked = nullptr;
+        dev->geoPackedCap = 0;
+        HIP_TRY(hipMalloc(&dev->geoPacked, (size_t)5 * rb.count * sizeof(int)));
+        dev->geoPackedCap = 5 * rb.count;
+    }
+    const GeoSetup gs = dev->geo;
+    hipLaunchKernelGGL(geo_reset, dim3((unsigned)((rb.count + 255) / 256)), dim3(256), 0, dev->stream, g, rb, did, fid,
+                       lid, gst, dev->geoMoving, nbody);
+    HIP_TRY(hipGetLastError());
+    for (int n = 0; n < nbody; ++n) {
+        const ArtraGeoBody &bd = bodies[n];
+        if (!bd.moving) continue;
+        GeoBox bb;
+        bb.count = 1;
+        for (int s = 0; s < 3; ++s) {
+            bb.lo[s] = bd.box[s][0];
+            bb.n[s] = bd.box[s][1] > bd.box[s][0] ? bd.box[s][1] - bd.box[s][0] : 0;
+            bb.count *= bb.n[s];
+        }
+        if (bb.count == 0) continue;
+        if (bd.pv.faceN <= 0) {
+            hipLaunchKernelGGL(geo_sphere, dim3((unsigned)((bb.count + 255) / 256)), dim3(256), 0, dev->stream, g, gs,
+                               bb, did, fid, bd.O[0], bd.O[1], bd.O[2], bd.r, n);
+        } else {
+            const GeoBodyDev &gb = dev->geoBodies[n];
+            ArtraPolyView pv;
+            pv.faceN = gb.faceN;
+            pv.edgeN = gb.edgeN;
+            pv.vertN = gb.vertN;
+            pv.f = (const int (*)[3])gb.f;
+            pv.e = (const int (*)[4])gb.e;
+            pv.v = (const double (*)[3])gb.v;
+            pv.Nf = (const double (*)[3])gb.Nf;
+            pv.Ne = (const double (*)[3])gb.Ne;
+            pv.Nv = (const double (*)[3])gb.Nv;
+            hipLaunchKernelGGL(geo_mesh, dim3((unsigned)((bb.count + 3) / 4)), dim3(256), 0, dev->stream, g, gs, bb,
+                               did, fid, pv, n);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(geo_interface, dim3((unsigned)((rb.count + 255) / 256)), dim3(256), 0, dev->stream, g, gs, rb,
+                       did, fid, lid, gst, dev->fluid, dev->geoPacked);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(packed, dev->geoPacked, (size_t)5 * rb.count * sizeof(int), hipMemcpyDeviceToHost,
+                           dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    *count = rb.count;
     return 0;
 }
 

@@ -3,9 +3,9 @@
  * side of libartra_hip.  All device memory is owned by an ArtraDevice; the
  * host passes plain arrays (structure-of-arrays fields, flag masks, tables).
  *
- * Field layout on the device: one FP64 array per (time level, component),
- * each covering the padded node grid in the solver's index order
- * idx = (k * ny + j) * nx + i, i.e. U[level][component][idx].
+ * The host exchanges a time level as component-major arrays [5][N] in the
+ * solver's index order idx = (k * ny + j) * nx + i; on the device each
+ * level is node-major, U[level][idx][5].
  */
 #ifndef ARTRA_DEVICE_H_
 #define ARTRA_DEVICE_H_
@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include "artra_host.h"
 #include "artra_hip.h"
+#include "geomcore.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -89,6 +90,36 @@ int artra_dev_reconstruct(ArtraDevice *dev, int level, const ArtraGhostTable *jo
  * (the whole grid's array); box (may be NULL: all) bounds the nodes copied,
  * [min, max) per axis */
 int artra_dev_set_fluid(ArtraDevice *dev, const uint8_t *fluid, const int (*box)[2]);
+/* ---- moving bodies: the node classification on the device ----
+ * Node coordinates x = smin + (i - ng) d (cfd_commons.c:348-351) and the
+ * interfacial search path (path offsets, layer r ending at pathSep[r]). */
+typedef struct {
+    double smin[DIMS], d[DIMS];
+    int ng[DIMS];
+    int path[PATHN][DIMS];
+    int pathSep[PATHSEP];
+    int gl;
+} ArtraGeoSetup;
+
+/* A body as the classification takes it: moving (state != 1), the node box
+ * its domain field searches ([min, max) per axis), the sphere (faceN <= 0)
+ * or the triangulated body's host arrays in their current position. */
+typedef struct {
+    int moving;
+    int box[DIMS][LIMIT];
+    double O[DIMS], r;
+    ArtraPolyView pv;
+} ArtraGeoBody;
+
+/* The flags did, fid, lid, gst ([4][N], host order) onto the device. */
+int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *flags, int nbody);
+/* ComputeGeometricField's three passes (immersed_boundary.c:65-264) over
+ * region on the device (the domain field over each moving body's box), the
+ * sweeps' fluid flags updated; packed[5][*count] returns the region's new
+ * did, fid, lid, gst and previous domain field in k-j-i order. */
+int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int region[DIMS][LIMIT],
+                           int *packed, long *count);
+
 /* replay the device march's step from a captured graph (1, the default) or
  * launch its kernels one by one (0) */
 int artra_dev_set_graph(ArtraDevice *dev, int on);

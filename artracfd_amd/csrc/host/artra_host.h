@@ -14,6 +14,7 @@
 #include <stddef.h>
 #include <stdint.h>
 #include "artra_types.h"
+#include "geomcore.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -94,6 +95,9 @@ typedef struct {
 } ArtraFacetBounds;
 int artra_facet_bounds(const Polyhedron *poly, ArtraFacetBounds *bounds);
 void artra_free_facet_bounds(ArtraFacetBounds *bounds);
+/* The arrays of a triangulated body as the shared closest-point queries
+ * (geomcore.h) take them. */
+ArtraPolyView artra_poly_view(const Polyhedron *poly);
 /* Inside-or-on test of a triangulated body with the closest facet in *fid
  * (computational_geometry.c:463-506); bounds (may be NULL: every facet is
  * measured) only skip facets that cannot be the closest. */
@@ -120,6 +124,14 @@ int artra_compute_geometric_field(Space *space, const Model *model, ArtraGhostTa
  * artra_scope_update refreshes the scope from the new classification (the
  * moving bodies' nodes all lie inside that region).
  */
+/* The host's copy of a classification made on the device over region:
+ * packed[5][count] = did, fid, lid, gst and the previous domain field of
+ * the region's nodes in k-j-i order (artra_dev_geo_classify).  The nodes a
+ * body uncovered are listed in joins (artra_add_join, on the flags as the
+ * reference's interfacial pass finds them) and marked fid = -1. */
+int artra_apply_classification(Space *space, const int region[DIMS][LIMIT], const int *packed, long count,
+                               ArtraGhostTable *joins);
+
 typedef struct {
     int bodies;
     int (*extent)[DIMS][LIMIT];   /* [bodies]: node box [min, max) per axis; empty when min >= max */

@@ -192,7 +192,7 @@ int artra_hip_attach(Space *space, const Model *model, ArtraHipSolver **out)
         artra_hip_detach(s);
         return 1;
     }
-    if (0 != model->psi && artra_scope_init(space, &s->scope)) {
+    if (0 != model->psi && (artra_scope_init(space, &s->scope) || artra_geo_device_init(s))) {
         artra_hip_detach(s);
         return 1;
     }

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include "artra_host.h"
+#include "geomcore.h"
 
 static const Real kPi = 3.14159265358979323846;
 
@@ -54,10 +55,23 @@ static void AddEdgeOf(Polyhedron *poly, int a, int b, int face)
 
 static int EdgeKeyCompare(const void *pa, const void *pb)
 {
-    const int *a = pa, *b = pb;
-    if (a[0] != b[0]) return (a[0] < b[0]) ? -1 : 1;
-    if (a[1] != b[1]) return (a[1] < b[1]) ? -1 : 1;
-    return 0;
+    return GcEdgeKeyCompare((const int *)pa, (const int *)pb);
+}
+
+/* The arrays of a triangulated body as the shared queries take them. */
+ArtraPolyView artra_poly_view(const Polyhedron *poly)
+{
+    ArtraPolyView pv;
+    pv.faceN = poly->faceN;
+    pv.edgeN = poly->edgeN;
+    pv.vertN = poly->vertN;
+    pv.f = (const int (*)[POLYN])poly->f;
+    pv.e = (const int (*)[EVF])poly->e;
+    pv.v = (const Real (*)[DIMS])poly->v;
+    pv.Nf = (const Real (*)[DIMS])poly->Nf;
+    pv.Ne = (const Real (*)[DIMS])poly->Ne;
+    pv.Nv = (const Real (*)[DIMS])poly->Nv;
+    return pv;
 }
 
 int artra_convert_polyhedron(Polyhedron *poly)
@@ -89,30 +103,7 @@ int artra_convert_polyhedron(Polyhedron *poly)
     return 0;
 }
 
-static int FindEdgeOf(const Polyhedron *poly, int a, int b)
-{
-    const int key[2] = {(a > b) ? a : b, (a > b) ? b : a};
-    int lo = 0, hi = poly->edgeN - 1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) / 2;
-        const int c = EdgeKeyCompare(key, poly->e[mid]);
-        if (0 == c) return mid;
-        if (c > 0) lo = mid + 1; else hi = mid - 1;
-    }
-    return -1;
-}
 
-static void TriangleOf(int fid, const Polyhedron *poly, Real v0[DIMS], Real v1[DIMS], Real v2[DIMS],
-                       Real e01[DIMS], Real e02[DIMS])
-{
-    for (int s = 0; s < DIMS; ++s) {
-        v0[s] = poly->v[poly->f[fid][0]][s];
-        v1[s] = poly->v[poly->f[fid][1]][s];
-        v2[s] = poly->v[poly->f[fid][2]][s];
-        e01[s] = v1[s] - v0[s];
-        e02[s] = v2[s] - v0[s];
-    }
-}
 
 /* ---------------------------------------------------- transformation */
 static void TransformPoints(const Real O[DIMS], const Real scale[DIMS], const Real rot[DIMS][DIMS],
@@ -220,8 +211,9 @@ static void PolyhedronParameters(int collapse, Polyhedron *poly)
             box[MAX][s] = (box[MAX][s] > poly->v[n][s]) ? box[MAX][s] : poly->v[n][s];
         }
     }
+    const ArtraPolyView pv = artra_poly_view(poly);
     for (int n = 0; n < poly->faceN; ++n) {
-        TriangleOf(n, poly, v0, v1, v2, e01, e02);
+        GcTriangleOf(n, &pv, v0, v1, v2, e01, e02);
         artra_cross(e01, e02, Nf);
         for (int s = 0; s < DIMS; ++s) {
             const Real t0 = v0[s] + v1[s];
@@ -302,126 +294,9 @@ void artra_compute_geometry_parameters(int collapse, Geometry *geo)
 Real artra_point_triangle_distance(const Real p[DIMS], const Real v0[DIMS], const Real e01[DIMS],
                                    const Real e02[DIMS], Real para[DIMS])
 {
-    const Real D[DIMS] = {v0[X] - p[X], v0[Y] - p[Y], v0[Z] - p[Z]};
-    const Real a = artra_dot(e01, e01), b = artra_dot(e01, e02), c = artra_dot(e02, e02);
-    const Real d = artra_dot(e01, D), e = artra_dot(e02, D), f = artra_dot(D, D);
-    const Real det = a * c - b * b;
-    Real s = b * e - c * d;
-    Real t = b * d - a * e;
-    Real dist;
-    /* the closest point on an edge parameterised from its start */
-    #define ON_E01(sv) do { s = (sv); dist = d * s + f; } while (0)
-    #define ON_E02(tv) do { t = (tv); dist = e * t + f; } while (0)
-    #define INTERIOR() (s * (a * s + b * t + 2.0 * d) + t * (b * s + c * t + 2.0 * e) + f)
-    if (s + t <= det) {
-        if (s < 0.0) {
-            if (t < 0.0) {                                    /* region 4 */
-                if (d < 0.0) {
-                    t = 0.0;
-                    if (-d >= a) { s = 1.0; dist = a + 2.0 * d + f; } else ON_E01(-d / a);
-                } else {
-                    s = 0.0;
-                    if (e >= 0.0) { t = 0.0; dist = f; }
-                    else if (-e >= c) { t = 1.0; dist = c + 2.0 * e + f; }
-                    else ON_E02(-e / c);
-                }
-            } else {                                          /* region 3 */
-                s = 0.0;
-                if (e >= 0.0) { t = 0.0; dist = f; }
-                else if (-e >= c) { t = 1.0; dist = c + 2.0 * e + f; }
-                else ON_E02(-e / c);
-            }
-        } else if (t < 0.0) {                                 /* region 5 */
-            t = 0.0;
-            if (d >= 0.0) { s = 0.0; dist = f; }
-            else if (-d >= a) { s = 1.0; dist = a + 2.0 * d + f; }
-            else ON_E01(-d / a);
-        } else {                                              /* region 0 */
-            s = s / det;
-            t = t / det;
-            dist = INTERIOR();
-        }
-    } else if (s < 0.0) {                                     /* region 2 */
-        const Real t0 = b + d, t1 = c + e;
-        if (t1 > t0) {
-            const Real num = t1 - t0, den = a - 2.0 * b + c;
-            if (num >= den) { s = 1.0; t = 0.0; dist = a + 2.0 * d + f; }
-            else { s = num / den; t = 1.0 - s; dist = INTERIOR(); }
-        } else {
-            s = 0.0;
-            if (t1 <= 0.0) { t = 1.0; dist = c + 2.0 * e + f; }
-            else if (e >= 0.0) { t = 0.0; dist = f; }
-            else ON_E02(-e / c);
-        }
-    } else if (t < 0.0) {                                     /* region 6 */
-        const Real t0 = b + e, t1 = a + d;
-        if (t1 > t0) {
-            const Real num = t1 - t0, den = a - 2.0 * b + c;
-            if (num >= den) { t = 1.0; s = 0.0; dist = c + 2.0 * e + f; }
-            else { t = num / den; s = 1.0 - t; dist = INTERIOR(); }
-        } else {
-            t = 0.0;
-            if (t1 <= 0.0) { s = 1.0; dist = a + 2.0 * d + f; }
-            else if (d >= 0.0) { s = 0.0; dist = f; }
-            else ON_E01(-d / a);
-        }
-    } else {                                                  /* region 1 */
-        const Real num = c + e - b - d;
-        if (num <= 0.0) { s = 0.0; t = 1.0; dist = c + 2.0 * e + f; }
-        else {
-            const Real den = a - 2.0 * b + c;
-            if (num >= den) { s = 1.0; t = 0.0; dist = a + 2.0 * d + f; }
-            else { s = num / den; t = 1.0 - s; dist = INTERIOR(); }
-        }
-    }
-    #undef ON_E01
-    #undef ON_E02
-    #undef INTERIOR
-    para[0] = 1.0 - s - t;
-    para[1] = s;
-    para[2] = t;
-    return (dist < 0.0) ? 0.0 : dist;
+    return GcPointTriangleDistance(p, v0, e01, e02, para);
 }
 
-/* Closest point on facet fid and the matching face, edge or vertex
- * pseudo-normal, chosen by exact tests on the barycentric parameters. */
-static void ClosestOnFacet(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS], Real N[DIMS])
-{
-    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
-    const int vid[3] = {poly->f[fid][0], poly->f[fid][1], poly->f[fid][2]};
-    TriangleOf(fid, poly, v0, v1, v2, e01, e02);
-    artra_point_triangle_distance(p, v0, e01, e02, para);
-    const Real *normal;
-    int edge;
-    if (0.0 == para[1]) {
-        if (0.0 == para[2]) {
-            for (int s = 0; s < DIMS; ++s) pi[s] = v0[s];
-            normal = poly->Nv[vid[0]];
-        } else if (1.0 == para[2]) {
-            for (int s = 0; s < DIMS; ++s) pi[s] = v2[s];
-            normal = poly->Nv[vid[2]];
-        } else {
-            for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[2] * e02[s];
-            edge = FindEdgeOf(poly, vid[0], vid[2]);
-            normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
-        }
-    } else if (1.0 == para[1]) {
-        for (int s = 0; s < DIMS; ++s) pi[s] = v1[s];
-        normal = poly->Nv[vid[1]];
-    } else if (0.0 == para[2]) {
-        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s];
-        edge = FindEdgeOf(poly, vid[0], vid[1]);
-        normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
-    } else if (0.0 == para[0]) {
-        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s] + para[2] * e02[s];
-        edge = FindEdgeOf(poly, vid[1], vid[2]);
-        normal = (edge >= 0) ? poly->Ne[edge] : poly->Nf[fid];
-    } else {
-        for (int s = 0; s < DIMS; ++s) pi[s] = v0[s] + para[1] * e01[s] + para[2] * e02[s];
-        normal = poly->Nf[fid];
-    }
-    for (int s = 0; s < DIMS; ++s) N[s] = normal[s];
-}
 
 /* ------------------------------------------------ closest-facet search */
 /*
@@ -502,9 +377,8 @@ void artra_free_facet_bounds(ArtraFacetBounds *fb)
 
 static Real FacetDistance2(const Real p[DIMS], int n, const Polyhedron *poly)
 {
-    Real v0[DIMS], v1[DIMS], v2[DIMS], e01[DIMS], e02[DIMS], para[DIMS];
-    TriangleOf(n, poly, v0, v1, v2, e01, e02);
-    return artra_point_triangle_distance(p, v0, e01, e02, para);
+    const ArtraPolyView pv = artra_poly_view(poly);
+    return GcFacetDistance2(p, n, &pv);
 }
 
 static int ClosestFacet(const Real p[DIMS], const Polyhedron *poly, const ArtraFacetBounds *fb)
@@ -570,10 +444,8 @@ int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, const 
 {
     const int cid = ClosestFacet(p, poly, bounds);
     *fid = cid;
-    Real pi[DIMS], N[DIMS];
-    ClosestOnFacet(p, cid, poly, pi, N);
-    const Real r[DIMS] = {p[X] - pi[X], p[Y] - pi[Y], p[Z] - pi[Z]};
-    return !(0.0 < artra_dot(r, N)); /* inside or on the surface */
+    const ArtraPolyView pv = artra_poly_view(poly);
+    return GcInsideOfFacet(p, cid, &pv); /* inside or on the surface */
 }
 
 /* Boundary point pi, image point pm = 2 pi - p and normal N of a ghost node. */
@@ -587,7 +459,8 @@ void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron 
         dist = poly->r - dist;
         for (int s = 0; s < DIMS; ++s) pi[s] = p[s] + dist * N[s];
     } else {
-        ClosestOnFacet(p, fid, poly, pi, N);
+        const ArtraPolyView pv = artra_poly_view(poly);
+        GcClosestOnFacet(p, fid, &pv, pi, N);
     }
     for (int s = 0; s < DIMS; ++s) pm[s] = pi[s] + pi[s] - p[s];
 }
@@ -828,4 +701,46 @@ void artra_scope_update(const Space *space, ArtraGeometryScope *sc, const ArtraG
         const int i = idx % part->n[X], j = (idx / part->n[X]) % part->n[Y], k = idx / (part->n[X] * part->n[Y]);
         AddNode(sc->join, k, j, i);
     }
+}
+
+int artra_apply_classification(Space *space, const int region[DIMS][LIMIT], const int *packed, long count,
+                               ArtraGhostTable *joins)
+{
+    const Partition *part = &space->part;
+    Node *node = space->node;
+    int lo[DIMS], hi[DIMS];
+    long span = 1;
+    for (int s = 0; s < DIMS; ++s) {
+        lo[s] = (region[s][MIN] > part->ns[PIN][s][MIN]) ? region[s][MIN] : part->ns[PIN][s][MIN];
+        hi[s] = (region[s][MAX] < part->ns[PIN][s][MAX]) ? region[s][MAX] : part->ns[PIN][s][MAX];
+        span *= (hi[s] > lo[s]) ? hi[s] - lo[s] : 0;
+    }
+    if (span != count) return artra_set_error(1, "classification of %ld nodes for a region of %ld", count, span);
+    /* the flags as the reference's interfacial pass finds them: the new
+     * domain field, gst still holding the previous one */
+    long t = 0;
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i, ++t) {
+                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                nd->did = packed[t];
+                nd->fid = packed[count + t];
+                nd->gst = packed[4 * count + t];
+            }
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i) {
+                const Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                if (nd->gst != nd->did && 0 == nd->did && artra_add_join(space, k, j, i, joins)) return 1;
+            }
+    for (int q = 0; q < joins->nghost; ++q) node[joins->ghosts[q].node].fid = NONE;
+    t = 0;
+    for (int k = lo[Z]; k < hi[Z]; ++k)
+        for (int j = lo[Y]; j < hi[Y]; ++j)
+            for (int i = lo[X]; i < hi[X]; ++i, ++t) {
+                Node *nd = node + artra_index(k, j, i, part->n[Y], part->n[X]);
+                nd->lid = packed[2 * count + t];
+                nd->gst = packed[3 * count + t];
+            }
+    return 0;
 }

@@ -4,17 +4,20 @@
  * EvolveSolidDynamics (solid_dynamics.c:37-48) for a field that lives on the
  * device.  The surface force is the device reduction of output.c; the
  * rigid-body kinematics, the collisions and the motion of the bodies run on
- * the host over the Geometry shared with the caller; the node
- * classification runs on the host node flags.  What changed goes back to the
- * device as tables: the fluid flags of the sweeps, the ghost stencils of the
- * immersed-boundary fill, the surface-force nodes, and the reconstruction of
- * the nodes a body has uncovered, which the device evaluates on level TO
- * before the ghost fill.  The field never crosses PCIe.
+ * the host over the Geometry shared with the caller.  The node
+ * classification runs on the device (its own copy of the flags, where the
+ * moving bodies were or are now) and comes back to the host's flags for the
+ * host's consumers (collisions, tables, outputs).  What the tables need goes
+ * back to the device: the ghost stencils of the immersed-boundary fill, the
+ * surface-force nodes, and the reconstruction of the nodes a body has
+ * uncovered, which the device evaluates on level TO before the ghost fill.
+ * The field never crosses PCIe.
  *
  *   artra_solid_step          solid_dynamics.c:37-48
  *   Kinematics, SolveInertia  solid_dynamics.c:172-212, linear_system.c:24-174
  *   Collisions                solid_dynamics.c:213-370
  *   Motion                    solid_dynamics.c:371-404
+ *   ClassifyOnDevice          immersed_boundary.c:65-264 on the device
  *   artra_hip_refresh_geometry  the device tables of a new classification
  *   artra_moving_step         solve.c:65-89 (one step of the time loop)
  */
@@ -271,6 +274,71 @@ static void Motion(Space *space, Real dt)
     }
 }
 
+int artra_geo_device_init(ArtraHipSolver *s)
+{
+    const Space *space = s->space;
+    const Partition *part = &space->part;
+    ArtraGeoSetup gs;
+    memset(&gs, 0, sizeof gs);
+    for (int d = 0; d < DIMS; ++d) {
+        gs.smin[d] = part->domain[d][MIN];
+        gs.d[d] = part->d[d];
+        gs.ng[d] = part->ng[d];
+    }
+    memcpy(gs.path, part->path, sizeof gs.path);
+    memcpy(gs.pathSep, part->pathSep, sizeof gs.pathSep);
+    gs.gl = part->gl;
+    int *flags = malloc((size_t)4 * s->N * sizeof(int));
+    if (NULL == flags) return artra_set_error(1, "out of host memory for %ld node flags", s->N);
+    for (long q = 0; q < s->N; ++q) {
+        const Node *nd = space->node + q;
+        flags[q] = nd->did;
+        flags[s->N + q] = nd->fid;
+        flags[2 * s->N + q] = nd->lid;
+        flags[3 * s->N + q] = nd->gst;
+    }
+    const int rc = artra_dev_geo_init(s->dev, &gs, flags, space->geo.totN);
+    free(flags);
+    return rc ? artra_dev_failed() : 0;
+}
+
+/* ComputeGeometricField over region on the device, the host's flags
+ * updated from it and the uncovered nodes listed in joins. */
+static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], ArtraGhostTable *joins)
+{
+    Space *space = s->space;
+    const Partition *part = &space->part;
+    const Geometry *geo = &space->geo;
+    ArtraGeoBody *bodies = calloc(geo->totN > 0 ? geo->totN : 1, sizeof *bodies);
+    long span = 1;
+    for (int d = 0; d < DIMS; ++d) {
+        const int lo = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
+        const int hi = (region[d][MAX] < part->ns[PIN][d][MAX]) ? region[d][MAX] : part->ns[PIN][d][MAX];
+        span *= (hi > lo) ? hi - lo : 0;
+    }
+    int *packed = malloc((size_t)(5 * span > 0 ? 5 * span : 1) * sizeof(int));
+    if (!bodies || !packed) {
+        free(bodies);
+        free(packed);
+        return artra_set_error(1, "out of host memory");
+    }
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        ArtraGeoBody *b = bodies + n;
+        b->moving = (1 != poly->state);
+        SearchBox(part, poly, b->box);
+        for (int d = 0; d < DIMS; ++d) b->O[d] = poly->O[d];
+        b->r = poly->r;
+        if (0 < poly->faceN) b->pv = artra_poly_view(poly);
+    }
+    long count = 0;
+    int rc = artra_dev_geo_classify(s->dev, bodies, geo->totN, region, packed, &count) ? artra_dev_failed() : 0;
+    if (!rc) rc = artra_apply_classification(space, region, packed, count, joins);
+    free(bodies);
+    free(packed);
+    return rc;
+}
+
 int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT])
 {
     const Space *space = s->space;
@@ -310,7 +378,7 @@ int artra_solid_step(ArtraHipSolver *s, Real now, Real dt)
     artra_scope_region(space, &s->scope, region);
     ArtraGhostTable joins;
     memset(&joins, 0, sizeof joins);
-    int rc = artra_compute_geometric_field(space, model, &joins, (const int (*)[LIMIT])region);
+    int rc = ClassifyOnDevice(s, (const int (*)[LIMIT])region, &joins);
     if (!rc) artra_scope_update(space, &s->scope, &joins, (const int (*)[LIMIT])region);
     if (!rc) rc = artra_hip_refresh_geometry(s, (const int (*)[LIMIT])region);
     if (!rc && artra_dev_reconstruct(s->dev, TO, &joins)) rc = artra_dev_failed();

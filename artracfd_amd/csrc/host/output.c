@@ -1231,7 +1231,9 @@ int artra_hip_read_restart(ArtraHipSolver *s, Time *time, const char *path)
     }
     if (moving) {
         artra_scope_free(&s->scope);
-        if (artra_scope_init(s->space, &s->scope) || artra_hip_refresh_geometry(s, NULL)) return 1;
+        if (artra_scope_init(s->space, &s->scope) || artra_geo_device_init(s) || artra_hip_refresh_geometry(s, NULL)) {
+            return 1;
+        }
     }
     time->stepC = hdr[6];
     time->dataC = hdr[7];

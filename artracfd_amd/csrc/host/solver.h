@@ -42,6 +42,9 @@ int artra_dev_failed(void);
  * flags after the bodies moved; region (may be NULL: everywhere) bounds the
  * nodes whose flags changed. */
 int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT]);
+/* The device's copy of the node classification, from the host's flags
+ * (at attach and after a restart of a moving case). */
+int artra_geo_device_init(ArtraHipSolver *s);
 /* EvolveSolidDynamics (solid_dynamics.c:37-48) over dt at time now. */
 int artra_solid_step(ArtraHipSolver *s, Real now, Real dt);
 /* One pass of the time loop's body with moving bodies (solve.c:65-89): the
