@@ -1350,6 +1350,123 @@ __global__ __launch_bounds__(256) void geo_interface(Grid g, GeoSetup gs, GeoBox
     fluid[q] = (0 == dq);
 }
 
+// The ghost table on the device (immersed_boundary.c:270-487 via
+// artra_build_ghost_table): one thread per node of a body's box; a ghost
+// of layer r takes the next entry of the table and kGhostSlot stencil
+// entries, found by the same search (radius 2 around the image point's node
+// over fluid nodes for an image-point ghost, radius 1 around itself over
+// the previous layer's ghosts of the body otherwise, growing to 3).
+constexpr int kGhostSlot = 343;
+
+struct GhostBuildArgs {
+    Grid g;
+    GeoSetup gs;
+    double dd[3];
+    double tinyL;
+    int ibmLayer;
+    int cap;                        // ghost entries available
+};
+
+struct GhostBody {
+    ArtraPolyView pv;
+    double O[3], r, V[3], W[3], cf, T;
+    int body;
+};
+
+__device__ __forceinline__ double GbInverse(double tiny, double d2)
+{
+    if (tiny > d2) d2 = tiny;
+    return 1.0 / d2;
+}
+
+__device__ __forceinline__ double GbDist2(const double a[3], const double b[3])
+{
+    const double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    return GcDot(d, d);
+}
+
+__global__ __launch_bounds__(256) void ib_build(GhostBuildArgs a, GeoBox b, int r, GhostBody body, const int *did,
+                                                const int *fid, const int *gst, GhostDev *ghosts, int *snode,
+                                                double *sw, int *counter, int *overflow)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= b.count) return;
+    const Grid &g = a.g;
+    int i, j, k;
+    const long q = GeoNode(g, b, t, i, j, k);
+    if (r != gst[q] || body.body + 1 != did[q]) return;
+    const int e = atomicAdd(counter, 1);
+    if (e >= a.cap) {
+        atomicOr(overflow, 1);
+        return;
+    }
+    GhostDev gh;
+    memset(&gh, 0, sizeof gh);
+    gh.node = (int)q;
+    gh.first = e * kGhostSlot;
+    double pG[3];
+    GeoPoint(a.gs, i, j, k, pG);
+    double c[3];                                    // the search's centre point and node
+    int nc[3], h, want, layer;
+    double pO[3], pI[3], N[3];
+    if (a.ibmLayer >= r) {
+        GcGeometricData(pG, fid[q], &body.pv, body.O, body.r, pO, pI, N);
+        for (int s = 0; s < 3; ++s) {
+            c[s] = pI[s];
+            nc[s] = GcMapNode(pI[s], a.gs.smin[s], a.dd[s], a.gs.ng[s]);
+        }
+        h = 2;
+        want = 0;
+        layer = 0;
+    } else {
+        for (int s = 0; s < 3; ++s) c[s] = pG[s];
+        nc[0] = i;
+        nc[1] = j;
+        nc[2] = k;
+        h = 1;
+        want = body.body + 1;
+        layer = r - 1;
+    }
+    int count = 0;
+    for (int rr = h; 0 == count; ++rr) {
+        if (rr > 3) {
+            atomicOr(overflow, 1);
+            return;
+        }
+        for (int kh = -rr; kh <= rr; ++kh)
+            for (int jh = -rr; jh <= rr; ++jh)
+                for (int ih = -rr; ih <= rr; ++ih) {
+                    const int x = nc[0] + ih, y = nc[1] + jh, z = nc[2] + kh;
+                    if (z < g.lo[2] || z >= g.hi[2] || y < g.lo[1] || y >= g.hi[1] || x < g.lo[0] || x >= g.hi[0])
+                        continue;
+                    const long qq = ((long)z * g.ny + y) * g.nx + x;
+                    if (want != did[qq]) continue;
+                    if (layer > 0 && layer != gst[qq]) continue;
+                    double ph[3];
+                    GeoPoint(a.gs, x, y, z, ph);
+                    snode[gh.first + count] = (int)qq;
+                    sw[gh.first + count] = GbInverse(a.tinyL, GbDist2(c, ph));
+                    ++count;
+                }
+    }
+    gh.count = count;
+    if (a.ibmLayer >= r) {
+        gh.kind = 0;
+        gh.wO = GbInverse(a.tinyL, GbDist2(pI, pO));
+        const double arm[3] = {pO[0] - body.O[0], pO[1] - body.O[1], pO[2] - body.O[2]};
+        GcCross(body.W, arm, gh.Vs);
+        for (int s = 0; s < 3; ++s) gh.Vs[s] = body.V[s] + gh.Vs[s];
+        for (int s = 0; s < 3; ++s) gh.N[s] = N[s];
+        GcOrthogonalSpace(N, gh.Ta, gh.Tb);
+        gh.noslip = (0.0 < body.cf);
+        gh.isothermal = !(0.0 > body.T);
+        gh.wallT = body.T;
+    } else {
+        gh.kind = 1;
+    }
+    ghosts[e] = gh;
+}
+
 // A table of filled nodes on the device with its stencil entries; the
 // buffers grow as needed and are reused when the table is rebuilt.
 struct IbTable {
@@ -1872,6 +1989,7 @@ struct ArtraDevice {
     int *geoPacked = nullptr;       // [4][region nodes] for the host's copy
     long geoPackedCap = 0;
     int *geoMoving = nullptr;       // [bodies]: 1 for a body that moves
+    int *geoCounters = nullptr;     // ghost-table build: entries taken, overflow
     GeoSetup geo{};
     std::vector<GeoBodyDev> geoBodies;
     std::vector<int> layerStart;
@@ -2720,6 +2838,7 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->geoFlags);
     hipFree(dev->geoPacked);
     hipFree(dev->geoMoving);
+    hipFree(dev->geoCounters);
     for (GeoBodyDev &b : dev->geoBodies) {
         hipFree(b.f); hipFree(b.e); hipFree(b.v); hipFree(b.Nf); hipFree(b.Ne); hipFree(b.Nv);
     }
@@ -3982,6 +4101,98 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
                            dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
     *count = rb.count;
+    return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, double dd[3], double tinyL,
+                           int ibmLayer, int *fallback)
+{
+    *fallback = 0;
+    if (!dev->geoFlags) return Fail("moving-body classification not set up");
+    ++dev->configGen;
+    const Grid &g = dev->g;
+    const long N = g.N;
+    const int *did = dev->geoFlags, *fid = did + N, *gst = fid + 2 * N;
+    IbTable &tb = dev->ib;
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    // room for the last table's ghosts and half again
+    const int want = std::max(4096, tb.count + tb.count / 2);
+    if (want > tb.ghostCap || (long)want * kGhostSlot > tb.stencilCap) {
+        hipFree(tb.ghosts);
+        hipFree(tb.snode);
+        hipFree(tb.sw);
+        tb = IbTable();
+        HIP_TRY(hipMalloc(&tb.ghosts, (size_t)want * sizeof(GhostDev)));
+        HIP_TRY(hipMalloc(&tb.snode, (size_t)want * kGhostSlot * sizeof(int)));
+        HIP_TRY(hipMalloc(&tb.sw, (size_t)want * kGhostSlot * sizeof(double)));
+        tb.ghostCap = want;
+        tb.stencilCap = want * kGhostSlot;
+    }
+    if (!dev->geoCounters) HIP_TRY(hipMalloc(&dev->geoCounters, 2 * sizeof(int)));
+    HIP_TRY(hipMemsetAsync(dev->geoCounters, 0, 2 * sizeof(int), dev->stream));
+    GhostBuildArgs a;
+    a.g = g;
+    a.gs = dev->geo;
+    for (int s = 0; s < 3; ++s) a.dd[s] = dd[s];
+    a.tinyL = tinyL;
+    a.ibmLayer = ibmLayer;
+    a.cap = tb.ghostCap;
+    const int layers = dev->geo.gl;
+    std::vector<int> layerEnd(layers + 1, 0);
+    for (int r = 1; r <= layers; ++r) {
+        for (int n = 0; n < nbody; ++n) {
+            const ArtraGeoBody &bd = bodies[n];
+            GeoBox bb;
+            bb.count = 1;
+            for (int s = 0; s < 3; ++s) {
+                bb.lo[s] = bd.box[s][0];
+                bb.n[s] = bd.box[s][1] > bd.box[s][0] ? bd.box[s][1] - bd.box[s][0] : 0;
+                bb.count *= bb.n[s];
+            }
+            if (bb.count == 0) continue;
+            GhostBody gbody;
+            gbody.pv = bd.pv;
+            if (bd.pv.faceN > 0) {
+                const GeoBodyDev &gb = dev->geoBodies[n];
+                if (!gb.uploaded) return Fail("body %d has no device arrays", n);
+                gbody.pv.f = (const int (*)[3])gb.f;
+                gbody.pv.e = (const int (*)[4])gb.e;
+                gbody.pv.v = (const double (*)[3])gb.v;
+                gbody.pv.Nf = (const double (*)[3])gb.Nf;
+                gbody.pv.Ne = (const double (*)[3])gb.Ne;
+                gbody.pv.Nv = (const double (*)[3])gb.Nv;
+            }
+            for (int s = 0; s < 3; ++s) {
+                gbody.O[s] = bd.O[s];
+                gbody.V[s] = bd.V[s];
+                gbody.W[s] = bd.W[s];
+            }
+            gbody.r = bd.r;
+            gbody.cf = bd.cf;
+            gbody.T = bd.T;
+            gbody.body = n;
+            hipLaunchKernelGGL(ib_build, dim3((unsigned)((bb.count + 255) / 256)), dim3(256), 0, dev->stream, a, bb, r,
+                               gbody, did, fid, gst, tb.ghosts, tb.snode, tb.sw, dev->geoCounters,
+                               dev->geoCounters + 1);
+            HIP_TRY(hipGetLastError());
+        }
+        int c[2];
+        HIP_TRY(hipMemcpyAsync(c, dev->geoCounters, sizeof c, hipMemcpyDeviceToHost, dev->stream));
+        HIP_TRY(hipStreamSynchronize(dev->stream));
+        if (c[1] || c[0] > tb.ghostCap) {
+            tb.count = std::min(c[0], tb.ghostCap);  // the next build allocates room for them
+            *fallback = 1;
+            return 0;
+        }
+        layerEnd[r] = c[0];
+    }
+    tb.count = layerEnd[layers];
+    dev->layerStart.assign(layerEnd.begin(), layerEnd.end());
+    dev->ibImageOnly = (ibmLayer >= layers);
     return 0;
 }
 

@@ -109,7 +109,17 @@ typedef struct {
     int box[DIMS][LIMIT];
     double O[DIMS], r;
     ArtraPolyView pv;
+    double V[DIMS], W[DIMS];  /* velocities of level TO (ghost wall velocity) */
+    double cf, T;             /* friction (> 0 no-slip), wall temperature (< 0 adiabatic) */
 } ArtraGeoBody;
+
+/* The immersed-boundary ghost table built on the device from its flags
+ * (artra_build_ghost_table's entries; within a layer the order may
+ * differ, which the fill does not depend on).  *fallback = 1 when a
+ * stencil search needs more than radius 3 (the caller then builds the
+ * table on the host). */
+int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, double dd[DIMS], double tinyL,
+                           int ibmLayer, int *fallback);
 
 /* The flags did, fid, lid, gst ([4][N], host order) onto the device. */
 int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *flags, int nbody);

@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include "artra_host.h"
+#include "geomcore.h"
 
 /* ------------------------------------------------------------------ errors */
 static _Thread_local char errorText[512];
@@ -117,18 +118,7 @@ void artra_normalize(int n, Real by, Real *v)
  * zeroes the normal's smallest component (ties resolved toward z, then y). */
 void artra_orthogonal_space(const Real N[DIMS], Real Ta[DIMS], Real Tb[DIMS])
 {
-    int small = Z;
-    if (fabs(N[small]) > fabs(N[Y])) small = Y;
-    if (fabs(N[small]) > fabs(N[X])) small = X;
-    if (X == small) {
-        Ta[X] = 0.0; Ta[Y] = -N[Z]; Ta[Z] = N[Y];
-    } else if (Y == small) {
-        Ta[X] = N[Z]; Ta[Y] = 0.0; Ta[Z] = -N[X];
-    } else {
-        Ta[X] = -N[Y]; Ta[Y] = N[X]; Ta[Z] = 0.0;
-    }
-    artra_normalize(DIMS, artra_norm(Ta), Ta);
-    artra_cross(Ta, N, Tb);
+    GcOrthogonalSpace(N, Ta, Tb);
 }
 
 /* Primitive (rho, u, v, w, p, T) from conservative (cfd_commons.c:296-305). */

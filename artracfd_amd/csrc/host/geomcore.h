@@ -9,6 +9,8 @@
 #ifndef ARTRA_GEOMCORE_H_
 #define ARTRA_GEOMCORE_H_
 
+#include <math.h>
+
 #include "artra_types.h"
 
 #ifdef __HIPCC__
@@ -210,6 +212,56 @@ ARTRA_HD int GcInsideOfFacet(const Real p[DIMS], int cid, const ArtraPolyView *p
     GcClosestOnFacet(p, cid, pv, pi, N);
     const Real r[DIMS] = {p[X] - pi[X], p[Y] - pi[Y], p[Z] - pi[Z]};
     return !(0.0 < GcDot(r, N));
+}
+
+ARTRA_HD void GcCross(const Real a[DIMS], const Real b[DIMS], Real out[DIMS])
+{
+    out[X] = a[Y] * b[Z] - a[Z] * b[Y];
+    out[Y] = a[Z] * b[X] - a[X] * b[Z];
+    out[Z] = a[X] * b[Y] - a[Y] * b[X];
+}
+
+/* Node of a coordinate (cfd_commons.c:352-355). */
+ARTRA_HD int GcMapNode(Real s, Real smin, Real dds, int ng)
+{
+    return (int)((s - smin) * dds + 0.5) + ng;
+}
+
+/* Tangent pair of a unit normal (cfd_commons.c:431-458): the first tangent
+ * zeroes the normal's smallest component (ties resolved toward z, then y). */
+ARTRA_HD void GcOrthogonalSpace(const Real N[DIMS], Real Ta[DIMS], Real Tb[DIMS])
+{
+    int small = Z;
+    if (fabs(N[small]) > fabs(N[Y])) small = Y;
+    if (fabs(N[small]) > fabs(N[X])) small = X;
+    if (X == small) {
+        Ta[X] = 0.0; Ta[Y] = -N[Z]; Ta[Z] = N[Y];
+    } else if (Y == small) {
+        Ta[X] = N[Z]; Ta[Y] = 0.0; Ta[Z] = -N[X];
+    } else {
+        Ta[X] = -N[Y]; Ta[Y] = N[X]; Ta[Z] = 0.0;
+    }
+    const Real norm = sqrt(GcDot(Ta, Ta));
+    for (int s = 0; s < DIMS; ++s) Ta[s] = Ta[s] / norm;
+    GcCross(Ta, N, Tb);
+}
+
+/* Boundary point pi, image point pm = 2 pi - p and normal N of a ghost node
+ * (computational_geometry.c:689-783): a sphere (faceN <= 0: centre O,
+ * radius r) or facet fid of a triangulated body. */
+ARTRA_HD void GcGeometricData(const Real p[DIMS], int fid, const ArtraPolyView *pv, const Real O[DIMS], Real r,
+                              Real pi[DIMS], Real pm[DIMS], Real N[DIMS])
+{
+    if (0 >= pv->faceN) {
+        for (int s = 0; s < DIMS; ++s) N[s] = p[s] - O[s];
+        Real dist = sqrt(GcDot(N, N));
+        for (int s = 0; s < DIMS; ++s) N[s] = N[s] / dist;
+        dist = r - dist;
+        for (int s = 0; s < DIMS; ++s) pi[s] = p[s] + dist * N[s];
+    } else {
+        GcClosestOnFacet(p, fid, pv, pi, N);
+    }
+    for (int s = 0; s < DIMS; ++s) pm[s] = pi[s] + pi[s] - p[s];
 }
 
 #endif /* ARTRA_GEOMCORE_H_ */

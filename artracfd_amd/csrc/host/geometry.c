@@ -452,17 +452,13 @@ int artra_point_in_polyhedron(const Real p[DIMS], const Polyhedron *poly, const 
 void artra_compute_geometric_data(const Real p[DIMS], int fid, const Polyhedron *poly, Real pi[DIMS],
                                   Real pm[DIMS], Real N[DIMS])
 {
-    if (0 >= poly->faceN) {
-        for (int s = 0; s < DIMS; ++s) N[s] = p[s] - poly->O[s];
-        Real dist = artra_norm(N);
-        artra_normalize(DIMS, dist, N);
-        dist = poly->r - dist;
-        for (int s = 0; s < DIMS; ++s) pi[s] = p[s] + dist * N[s];
+    ArtraPolyView pv;
+    if (0 < poly->faceN) {
+        pv = artra_poly_view(poly);
     } else {
-        const ArtraPolyView pv = artra_poly_view(poly);
-        GcClosestOnFacet(p, fid, &pv, pi, N);
+        memset(&pv, 0, sizeof pv);
     }
-    for (int s = 0; s < DIMS; ++s) pm[s] = pi[s] + pi[s] - p[s];
+    GcGeometricData(p, fid, &pv, poly->O, poly->r, pi, pm, N);
 }
 
 /* ------------------------------------------------- node classification */

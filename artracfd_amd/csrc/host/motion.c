@@ -302,14 +302,38 @@ int artra_geo_device_init(ArtraHipSolver *s)
     return rc ? artra_dev_failed() : 0;
 }
 
+/* The bodies as the device's classification and ghost table take them. */
+static ArtraGeoBody *GeoBodies(const Space *space)
+{
+    const Geometry *geo = &space->geo;
+    ArtraGeoBody *bodies = calloc(geo->totN > 0 ? geo->totN : 1, sizeof *bodies);
+    if (NULL == bodies) return NULL;
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        ArtraGeoBody *b = bodies + n;
+        b->moving = (1 != poly->state);
+        SearchBox(&space->part, poly, b->box);
+        for (int d = 0; d < DIMS; ++d) {
+            b->O[d] = poly->O[d];
+            b->V[d] = poly->V[TO][d];
+            b->W[d] = poly->W[TO][d];
+        }
+        b->r = poly->r;
+        b->cf = poly->cf;
+        b->T = poly->T;
+        if (0 < poly->faceN) b->pv = artra_poly_view(poly);
+    }
+    return bodies;
+}
+
 /* ComputeGeometricField over region on the device, the host's flags
  * updated from it and the uncovered nodes listed in joins. */
-static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], ArtraGhostTable *joins)
+static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], const ArtraGeoBody *bodies,
+                            ArtraGhostTable *joins)
 {
     Space *space = s->space;
     const Partition *part = &space->part;
     const Geometry *geo = &space->geo;
-    ArtraGeoBody *bodies = calloc(geo->totN > 0 ? geo->totN : 1, sizeof *bodies);
     long span = 1;
     for (int d = 0; d < DIMS; ++d) {
         const int lo = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
@@ -317,26 +341,33 @@ static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], Ar
         span *= (hi > lo) ? hi - lo : 0;
     }
     int *packed = malloc((size_t)(5 * span > 0 ? 5 * span : 1) * sizeof(int));
-    if (!bodies || !packed) {
-        free(bodies);
-        free(packed);
-        return artra_set_error(1, "out of host memory");
-    }
-    for (int n = 0; n < geo->totN; ++n) {
-        const Polyhedron *poly = geo->poly + n;
-        ArtraGeoBody *b = bodies + n;
-        b->moving = (1 != poly->state);
-        SearchBox(part, poly, b->box);
-        for (int d = 0; d < DIMS; ++d) b->O[d] = poly->O[d];
-        b->r = poly->r;
-        if (0 < poly->faceN) b->pv = artra_poly_view(poly);
-    }
+    if (!packed) return artra_set_error(1, "out of host memory");
     long count = 0;
     int rc = artra_dev_geo_classify(s->dev, bodies, geo->totN, region, packed, &count) ? artra_dev_failed() : 0;
     if (!rc) rc = artra_apply_classification(space, region, packed, count, joins);
-    free(bodies);
     free(packed);
     return rc;
+}
+
+/* The device tables after a classification on the device: the ghost table
+ * built there (on the host when a stencil search reaches beyond radius 3),
+ * the surface-force table from the host's flags.  The fluid flags are
+ * already current on the device. */
+static int TablesOnDevice(ArtraHipSolver *s, const ArtraGeoBody *bodies)
+{
+    const Space *space = s->space;
+    const Partition *part = &space->part;
+    double dd[DIMS] = {part->dd[X], part->dd[Y], part->dd[Z]};
+    int fallback = 0;
+    if (artra_dev_build_ghosts(s->dev, bodies, space->geo.totN, dd, part->tinyL, s->model->ibmLayer, &fallback)) {
+        return artra_dev_failed();
+    }
+    if (fallback) {
+        artra_free_ghost_table(&s->ghosts);
+        if (artra_build_ghost_table(space, s->model, &s->ghosts)) return 1;
+        if (artra_dev_set_ghosts(s->dev, &s->ghosts)) return artra_dev_failed();
+    }
+    return artra_build_force_table(s);
 }
 
 int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT])
@@ -378,9 +409,12 @@ int artra_solid_step(ArtraHipSolver *s, Real now, Real dt)
     artra_scope_region(space, &s->scope, region);
     ArtraGhostTable joins;
     memset(&joins, 0, sizeof joins);
-    int rc = ClassifyOnDevice(s, (const int (*)[LIMIT])region, &joins);
+    ArtraGeoBody *bodies = GeoBodies(space);
+    int rc = bodies ? 0 : artra_set_error(1, "out of host memory");
+    if (!rc) rc = ClassifyOnDevice(s, (const int (*)[LIMIT])region, bodies, &joins);
     if (!rc) artra_scope_update(space, &s->scope, &joins, (const int (*)[LIMIT])region);
-    if (!rc) rc = artra_hip_refresh_geometry(s, (const int (*)[LIMIT])region);
+    if (!rc) rc = TablesOnDevice(s, bodies);
+    free(bodies);
     if (!rc && artra_dev_reconstruct(s->dev, TO, &joins)) rc = artra_dev_failed();
     if (!rc && artra_dev_treat_immersed(s->dev, TO)) rc = artra_dev_failed();
     artra_free_ghost_table(&joins);
