@@ -58,6 +58,7 @@ WORKLOADS = {
     "shockvortex64": "shock-vortex interaction",
     "tgv128": "viscous Taylor-Green vortex",
     "cylinder": "shock diffraction over a cylinder (immersed boundary)",
+    "cylinder_moving": "shock over a translating, spinning cylinder (moving immersed boundary, solid dynamics)",
     "sod": "Sod shock tube",
 }
 
