@@ -1204,7 +1204,9 @@ typedef ArtraGeoSetup GeoSetup;
 struct GeoBodyDev {
     int faceN = 0, edgeN = 0, vertN = 0;
     int *f = nullptr, *e = nullptr;
-    double *v = nullptr, *Nf = nullptr, *Ne = nullptr, *Nv = nullptr;
+    double *v = nullptr, *Nf = nullptr, *Ne = nullptr, *Nv = nullptr;   // parts of block
+    double *block = nullptr;
+    double *stage = nullptr;        // pinned host copy of block for the upload
     bool uploaded = false;
 };
 
@@ -1351,12 +1353,13 @@ __global__ __launch_bounds__(256) void geo_interface(Grid g, GeoSetup gs, GeoBox
 }
 
 // The ghost table on the device (immersed_boundary.c:270-487 via
-// artra_build_ghost_table): one thread per node of a body's box; a ghost
-// of layer r takes the next entry of the table and kGhostSlot stencil
+// artra_build_ghost_table): a wavefront per 64 nodes of a body's box; a
+// ghost of layer r takes the next entry of the table and kGhostSlot stencil
 // entries, found by the same search (radius 2 around the image point's node
 // over fluid nodes for an image-point ghost, radius 1 around itself over
 // the previous layer's ghosts of the body otherwise, growing to 3).
 constexpr int kGhostSlot = 343;
+constexpr int kGeoCounters = 16;    // ghost-table build counters: taken, overflow, layer ends
 
 struct GhostBuildArgs {
     Grid g;
@@ -1389,82 +1392,137 @@ __global__ __launch_bounds__(256) void ib_build(GhostBuildArgs a, GeoBox b, int 
                                                 const int *fid, const int *gst, GhostDev *ghosts, int *snode,
                                                 double *sw, int *counter, int *overflow)
 {
+    // each wavefront covers 64 nodes of the box and searches their ghosts'
+    // stencils one at a time, the lanes scanning a search box together:
+    // candidate c of the (2 rr + 1)^3 box is (kh, jh, ih) in k-j-i order,
+    // and the found nodes take consecutive entries in that order (a ballot
+    // prefix), as the serial search appends them
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= b.count) return;
     const Grid &g = a.g;
-    int i, j, k;
-    const long q = GeoNode(g, b, t, i, j, k);
-    if (r != gst[q] || body.body + 1 != did[q]) return;
-    const int e = atomicAdd(counter, 1);
-    if (e >= a.cap) {
-        atomicOr(overflow, 1);
-        return;
+    const int lane = __lane_id();
+    int i = 0, j = 0, k = 0;
+    bool mine = false;
+    if (t < b.count) {
+        const long q = GeoNode(g, b, t, i, j, k);
+        mine = (r == gst[q] && body.body + 1 == did[q]);
     }
-    GhostDev gh;
-    memset(&gh, 0, sizeof gh);
-    gh.node = (int)q;
-    gh.first = e * kGhostSlot;
-    double pG[3];
-    GeoPoint(a.gs, i, j, k, pG);
-    double c[3];                                    // the search's centre point and node
-    int nc[3], h, want, layer;
-    double pO[3], pI[3], N[3];
-    if (a.ibmLayer >= r) {
-        GcGeometricData(pG, fid[q], &body.pv, body.O, body.r, pO, pI, N);
+    // each ghost's search centre on its own lane, all at once: the image
+    // point (through the body's surface) or the node itself
+    const bool image = (a.ibmLayer >= r);
+    double c[3] = {0.0, 0.0, 0.0}, pO[3] = {0.0, 0.0, 0.0}, pI[3] = {0.0, 0.0, 0.0}, N[3] = {0.0, 0.0, 0.0};
+    int nc[3] = {i, j, k};
+    if (mine) {
+        double pG[3];
+        GeoPoint(a.gs, i, j, k, pG);
+        if (image) {
+            GcGeometricData(pG, fid[((long)k * g.ny + j) * g.nx + i], &body.pv, body.O, body.r, pO, pI, N);
+            for (int s = 0; s < 3; ++s) {
+                c[s] = pI[s];
+                nc[s] = GcMapNode(pI[s], a.gs.smin[s], a.dd[s], a.gs.ng[s]);
+            }
+        } else {
+            for (int s = 0; s < 3; ++s) c[s] = pG[s];
+        }
+    }
+    const int h = image ? 2 : 1, want = image ? 0 : body.body + 1, layer = image ? 0 : r - 1;
+    unsigned long long todo = __ballot(mine);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the wavefront's ghosts take consecutive entries from one reservation
+    int e = 0;
+    if (lane == 0 && todo) e = atomicAdd(counter, __popcll(todo));
+    e = __shfl(e, 0) - 1;
+    int myEntry = -1, myCount = 0;
+    while (todo) {
+        const int src = __ffsll((long long)todo) - 1;
+        todo &= todo - 1;
+        ++e;
+        if (e >= a.cap) {
+            if (lane == 0) atomicOr(overflow, 1);
+            continue;
+        }
+        double cs[3];
+        int ns[3];
         for (int s = 0; s < 3; ++s) {
-            c[s] = pI[s];
-            nc[s] = GcMapNode(pI[s], a.gs.smin[s], a.dd[s], a.gs.ng[s]);
+            cs[s] = __shfl(c[s], src);
+            ns[s] = __shfl(nc[s], src);
         }
-        h = 2;
-        want = 0;
-        layer = 0;
-    } else {
-        for (int s = 0; s < 3; ++s) c[s] = pG[s];
-        nc[0] = i;
-        nc[1] = j;
-        nc[2] = k;
-        h = 1;
-        want = body.body + 1;
-        layer = r - 1;
-    }
-    int count = 0;
-    for (int rr = h; 0 == count; ++rr) {
-        if (rr > 3) {
-            atomicOr(overflow, 1);
-            return;
-        }
-        for (int kh = -rr; kh <= rr; ++kh)
-            for (int jh = -rr; jh <= rr; ++jh)
-                for (int ih = -rr; ih <= rr; ++ih) {
-                    const int x = nc[0] + ih, y = nc[1] + jh, z = nc[2] + kh;
-                    if (z < g.lo[2] || z >= g.hi[2] || y < g.lo[1] || y >= g.hi[1] || x < g.lo[0] || x >= g.hi[0])
-                        continue;
-                    const long qq = ((long)z * g.ny + y) * g.nx + x;
-                    if (want != did[qq]) continue;
-                    if (layer > 0 && layer != gst[qq]) continue;
+        const int first = e * kGhostSlot;
+        int count = 0;
+        bool lost = false;
+        for (int rr = h; 0 == count; ++rr) {
+            if (rr > 3) {
+                lost = true;
+                break;
+            }
+            const int w = 2 * rr + 1, cand = w * w * w;
+            // every chunk's flags are loaded before the first ballot, so the
+            // loads of a search overlap
+            constexpr int kChunks = (kGhostSlot + 63) / 64;
+            int qv[kChunks];
+            bool hv[kChunks];
+#pragma unroll
+            for (int ch = 0; ch < kChunks; ++ch) {
+                const int cc = ch * 64 + lane;
+                hv[ch] = false;
+                qv[ch] = -1;
+                if (cc < cand) {
+                    const int x = ns[0] + cc % w - rr, y = ns[1] + (cc / w) % w - rr, z = ns[2] + cc / (w * w) - rr;
+                    if (z >= g.lo[2] && z < g.hi[2] && y >= g.lo[1] && y < g.hi[1] && x >= g.lo[0] && x < g.hi[0])
+                        qv[ch] = (int)(((long)z * g.ny + y) * g.nx + x);
+                }
+                if (qv[ch] >= 0) {
+                    const int dq = did[qv[ch]], gq2 = gst[qv[ch]];
+                    hv[ch] = (want == dq) && (layer == 0 || layer == gq2);
+                }
+            }
+#pragma unroll
+            for (int ch = 0; ch < kChunks; ++ch) {
+                if (ch * 64 >= cand) break;
+                const unsigned long long hits = __ballot(hv[ch]);
+                if (hv[ch]) {
+                    const int cc = ch * 64 + lane;
+                    const int x = ns[0] + cc % w - rr, y = ns[1] + (cc / w) % w - rr, z = ns[2] + cc / (w * w) - rr;
                     double ph[3];
                     GeoPoint(a.gs, x, y, z, ph);
-                    snode[gh.first + count] = (int)qq;
-                    sw[gh.first + count] = GbInverse(a.tinyL, GbDist2(c, ph));
-                    ++count;
+                    const int pos = count + __popcll(hits & below);
+                    snode[first + pos] = qv[ch];
+                    sw[first + pos] = GbInverse(a.tinyL, GbDist2(cs, ph));
                 }
+                count += __popcll(hits);
+            }
+        }
+        if (lost) {
+            if (lane == 0) atomicOr(overflow, 1);
+            continue;
+        }
+        if (lane == src) {
+            myEntry = e;
+            myCount = count;
+        }
     }
-    gh.count = count;
-    if (a.ibmLayer >= r) {
-        gh.kind = 0;
-        gh.wO = GbInverse(a.tinyL, GbDist2(pI, pO));
-        const double arm[3] = {pO[0] - body.O[0], pO[1] - body.O[1], pO[2] - body.O[2]};
-        GcCross(body.W, arm, gh.Vs);
-        for (int s = 0; s < 3; ++s) gh.Vs[s] = body.V[s] + gh.Vs[s];
-        for (int s = 0; s < 3; ++s) gh.N[s] = N[s];
-        GcOrthogonalSpace(N, gh.Ta, gh.Tb);
-        gh.noslip = (0.0 < body.cf);
-        gh.isothermal = !(0.0 > body.T);
-        gh.wallT = body.T;
-    } else {
-        gh.kind = 1;
+    // each ghost's record from its own lane
+    if (myEntry >= 0) {
+        GhostDev gh;
+        memset(&gh, 0, sizeof gh);
+        gh.node = (int)(((long)k * g.ny + j) * g.nx + i);
+        gh.first = myEntry * kGhostSlot;
+        gh.count = myCount;
+        if (image) {
+            gh.kind = 0;
+            gh.wO = GbInverse(a.tinyL, GbDist2(pI, pO));
+            const double arm[3] = {pO[0] - body.O[0], pO[1] - body.O[1], pO[2] - body.O[2]};
+            GcCross(body.W, arm, gh.Vs);
+            for (int s = 0; s < 3; ++s) gh.Vs[s] = body.V[s] + gh.Vs[s];
+            for (int s = 0; s < 3; ++s) gh.N[s] = N[s];
+            GcOrthogonalSpace(N, gh.Ta, gh.Tb);
+            gh.noslip = (0.0 < body.cf);
+            gh.isothermal = !(0.0 > body.T);
+            gh.wallT = body.T;
+        } else {
+            gh.kind = 1;
+        }
+        ghosts[myEntry] = gh;
     }
-    ghosts[e] = gh;
 }
 
 // A table of filled nodes on the device with its stencil entries; the
@@ -1986,10 +2044,12 @@ struct ArtraDevice {
     // the node classification of moving bodies (did, fid, lid, gst), kept
     // on the device and reclassified there after every move
     int *geoFlags = nullptr;        // [4][N]
-    int *geoPacked = nullptr;       // [4][region nodes] for the host's copy
+    int *geoPacked = nullptr;       // [5][region nodes] for the host's copy
+    int *geoPackedHost = nullptr;   // that copy (pinned)
     long geoPackedCap = 0;
     int *geoMoving = nullptr;       // [bodies]: 1 for a body that moves
-    int *geoCounters = nullptr;     // ghost-table build: entries taken, overflow
+    int *geoCounters = nullptr;     // ghost-table build: entries taken, overflow, each layer's end
+    int *geoCountsHost = nullptr;   // their pinned host copy
     GeoSetup geo{};
     std::vector<GeoBodyDev> geoBodies;
     std::vector<int> layerStart;
@@ -2837,10 +2897,12 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->flagStage);
     hipFree(dev->geoFlags);
     hipFree(dev->geoPacked);
+    hipHostFree(dev->geoPackedHost);
     hipFree(dev->geoMoving);
     hipFree(dev->geoCounters);
+    hipHostFree(dev->geoCountsHost);
     for (GeoBodyDev &b : dev->geoBodies) {
-        hipFree(b.f); hipFree(b.e); hipFree(b.v); hipFree(b.Nf); hipFree(b.Ne); hipFree(b.Nv);
+        hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage);
     }
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
@@ -3987,7 +4049,7 @@ int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *
     HIP_TRY(hipMemcpy(dev->geoFlags, flags, (size_t)4 * N * sizeof(int), hipMemcpyHostToDevice));
     if ((int)dev->geoBodies.size() != nbody) {
         for (GeoBodyDev &b : dev->geoBodies) {
-            hipFree(b.f); hipFree(b.e); hipFree(b.v); hipFree(b.Nf); hipFree(b.Ne); hipFree(b.Nv);
+            hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage);
         }
         dev->geoBodies.assign(nbody, GeoBodyDev());
         hipFree(dev->geoMoving);
@@ -3999,9 +4061,10 @@ int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *
 }
 
 int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int region[3][2],
-                           int *packed, long *count)
+                           const int **packed, long *count)
 {
     *count = 0;
+    *packed = nullptr;
     if (!dev->geoFlags || nbody != (int)dev->geoBodies.size()) return Fail("moving-body classification not set up");
     ++dev->configGen;
     dev->allFluid = 0;                              // bodies present: the stage-by-stage sweeps
@@ -4023,20 +4086,26 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
             gb.vertN = bd.pv.vertN;
             HIP_TRY(hipMalloc(&gb.f, (size_t)gb.faceN * 3 * sizeof(int)));
             HIP_TRY(hipMalloc(&gb.e, (size_t)gb.edgeN * 4 * sizeof(int)));
-            HIP_TRY(hipMalloc(&gb.v, vb));
-            HIP_TRY(hipMalloc(&gb.Nf, fb));
-            HIP_TRY(hipMalloc(&gb.Ne, eb));
-            HIP_TRY(hipMalloc(&gb.Nv, vb));
+            HIP_TRY(hipMalloc(&gb.block, 2 * vb + fb + eb));
+            HIP_TRY(hipHostMalloc(&gb.stage, 2 * vb + fb + eb));
+            gb.v = gb.block;
+            gb.Nf = gb.v + vb / sizeof(double);
+            gb.Ne = gb.Nf + fb / sizeof(double);
+            gb.Nv = gb.Ne + eb / sizeof(double);
             HIP_TRY(hipMemcpy(gb.f, bd.pv.f, (size_t)gb.faceN * 3 * sizeof(int), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(gb.e, bd.pv.e, (size_t)gb.edgeN * 4 * sizeof(int), hipMemcpyHostToDevice));
         } else if (gb.faceN != bd.pv.faceN || gb.edgeN != bd.pv.edgeN || gb.vertN != bd.pv.vertN) {
             return Fail("body %d changed its facet count", n);
         }
         if (!gb.uploaded || bd.moving) {
-            HIP_TRY(hipMemcpy(gb.v, bd.pv.v, vb, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(gb.Nf, bd.pv.Nf, fb, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(gb.Ne, bd.pv.Ne, eb, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(gb.Nv, bd.pv.Nv, vb, hipMemcpyHostToDevice));
+            // the stream is idle here (synchronized above): the staging copy
+            // of the last upload is free
+            char *st = (char *)gb.stage;
+            memcpy(st, bd.pv.v, vb);
+            memcpy(st + vb, bd.pv.Nf, fb);
+            memcpy(st + vb + fb, bd.pv.Ne, eb);
+            memcpy(st + vb + fb + eb, bd.pv.Nv, vb);
+            HIP_TRY(hipMemcpyAsync(gb.block, gb.stage, 2 * vb + fb + eb, hipMemcpyHostToDevice, dev->stream));
         }
         gb.uploaded = true;
     }
@@ -4054,9 +4123,12 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
     if (rb.count == 0) return 0;
     if (5 * rb.count > dev->geoPackedCap) {
         hipFree(dev->geoPacked);
+        hipHostFree(dev->geoPackedHost);
         dev->geoPacked = nullptr;
+        dev->geoPackedHost = nullptr;
         dev->geoPackedCap = 0;
         HIP_TRY(hipMalloc(&dev->geoPacked, (size_t)5 * rb.count * sizeof(int)));
+        HIP_TRY(hipHostMalloc(&dev->geoPackedHost, (size_t)5 * rb.count * sizeof(int)));
         dev->geoPackedCap = 5 * rb.count;
     }
     const GeoSetup gs = dev->geo;
@@ -4097,9 +4169,10 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
     hipLaunchKernelGGL(geo_interface, dim3((unsigned)((rb.count + 255) / 256)), dim3(256), 0, dev->stream, g, gs, rb,
                        did, fid, lid, gst, dev->fluid, dev->geoPacked);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(packed, dev->geoPacked, (size_t)5 * rb.count * sizeof(int), hipMemcpyDeviceToHost,
-                           dev->stream));
+    HIP_TRY(hipMemcpyAsync(dev->geoPackedHost, dev->geoPacked, (size_t)5 * rb.count * sizeof(int),
+                           hipMemcpyDeviceToHost, dev->stream));
     HIP_TRY(hipStreamSynchronize(dev->stream));
+    *packed = dev->geoPackedHost;
     *count = rb.count;
     return 0;
 }
@@ -4118,10 +4191,12 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
     const long N = g.N;
     const int *did = dev->geoFlags, *fid = did + N, *gst = fid + 2 * N;
     IbTable &tb = dev->ib;
-    HIP_TRY(hipStreamSynchronize(dev->stream));
+    const int layers = dev->geo.gl;
+    if (layers + 3 > kGeoCounters) return Fail("%d ghost layers exceed the table's %d", layers, kGeoCounters - 3);
     // room for the last table's ghosts and half again
     const int want = std::max(4096, tb.count + tb.count / 2);
     if (want > tb.ghostCap || (long)want * kGhostSlot > tb.stencilCap) {
+        HIP_TRY(hipStreamSynchronize(dev->stream));
         hipFree(tb.ghosts);
         hipFree(tb.snode);
         hipFree(tb.sw);
@@ -4132,7 +4207,8 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
         tb.ghostCap = want;
         tb.stencilCap = want * kGhostSlot;
     }
-    if (!dev->geoCounters) HIP_TRY(hipMalloc(&dev->geoCounters, 2 * sizeof(int)));
+    if (!dev->geoCounters) HIP_TRY(hipMalloc(&dev->geoCounters, kGeoCounters * sizeof(int)));
+    if (!dev->geoCountsHost) HIP_TRY(hipHostMalloc(&dev->geoCountsHost, kGeoCounters * sizeof(int)));
     HIP_TRY(hipMemsetAsync(dev->geoCounters, 0, 2 * sizeof(int), dev->stream));
     GhostBuildArgs a;
     a.g = g;
@@ -4141,8 +4217,6 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
     a.tinyL = tinyL;
     a.ibmLayer = ibmLayer;
     a.cap = tb.ghostCap;
-    const int layers = dev->geo.gl;
-    std::vector<int> layerEnd(layers + 1, 0);
     for (int r = 1; r <= layers; ++r) {
         for (int n = 0; n < nbody; ++n) {
             const ArtraGeoBody &bd = bodies[n];
@@ -4180,16 +4254,21 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
                                dev->geoCounters + 1);
             HIP_TRY(hipGetLastError());
         }
-        int c[2];
-        HIP_TRY(hipMemcpyAsync(c, dev->geoCounters, sizeof c, hipMemcpyDeviceToHost, dev->stream));
-        HIP_TRY(hipStreamSynchronize(dev->stream));
-        if (c[1] || c[0] > tb.ghostCap) {
-            tb.count = std::min(c[0], tb.ghostCap);  // the next build allocates room for them
-            *fallback = 1;
-            return 0;
-        }
-        layerEnd[r] = c[0];
+        // the table's end after layer r, kept on the device until all are built
+        HIP_TRY(hipMemcpyAsync(dev->geoCounters + 2 + r, dev->geoCounters, sizeof(int), hipMemcpyDeviceToDevice,
+                               dev->stream));
     }
+    int *c = dev->geoCountsHost;
+    HIP_TRY(hipMemcpyAsync(c, dev->geoCounters, (size_t)(layers + 3) * sizeof(int), hipMemcpyDeviceToHost,
+                           dev->stream));
+    HIP_TRY(hipStreamSynchronize(dev->stream));
+    if (c[1] || c[0] > tb.ghostCap) {
+        tb.count = std::min(c[0], tb.ghostCap);  // the next build allocates room for them
+        *fallback = 1;
+        return 0;
+    }
+    std::vector<int> layerEnd(layers + 1, 0);
+    for (int r = 1; r <= layers; ++r) layerEnd[r] = c[2 + r];
     tb.count = layerEnd[layers];
     dev->layerStart.assign(layerEnd.begin(), layerEnd.end());
     dev->ibImageOnly = (ibmLayer >= layers);

@@ -125,10 +125,11 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
 int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *flags, int nbody);
 /* ComputeGeometricField's three passes (immersed_boundary.c:65-264) over
  * region on the device (the domain field over each moving body's box), the
- * sweeps' fluid flags updated; packed[5][*count] returns the region's new
- * did, fid, lid, gst and previous domain field in k-j-i order. */
+ * sweeps' fluid flags updated; *packed points to [5][*count] ints, the
+ * region's new did, fid, lid, gst and previous domain field in k-j-i order,
+ * held by the device until its next classification. */
 int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int region[DIMS][LIMIT],
-                           int *packed, long *count);
+                           const int **packed, long *count);
 
 /* replay the device march's step from a captured graph (1, the default) or
  * launch its kernels one by one (0) */

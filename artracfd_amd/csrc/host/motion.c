@@ -332,21 +332,10 @@ static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], co
                             ArtraGhostTable *joins)
 {
     Space *space = s->space;
-    const Partition *part = &space->part;
-    const Geometry *geo = &space->geo;
-    long span = 1;
-    for (int d = 0; d < DIMS; ++d) {
-        const int lo = (region[d][MIN] > part->ns[PIN][d][MIN]) ? region[d][MIN] : part->ns[PIN][d][MIN];
-        const int hi = (region[d][MAX] < part->ns[PIN][d][MAX]) ? region[d][MAX] : part->ns[PIN][d][MAX];
-        span *= (hi > lo) ? hi - lo : 0;
-    }
-    int *packed = malloc((size_t)(5 * span > 0 ? 5 * span : 1) * sizeof(int));
-    if (!packed) return artra_set_error(1, "out of host memory");
+    const int *packed = NULL;
     long count = 0;
-    int rc = artra_dev_geo_classify(s->dev, bodies, geo->totN, region, packed, &count) ? artra_dev_failed() : 0;
-    if (!rc) rc = artra_apply_classification(space, region, packed, count, joins);
-    free(packed);
-    return rc;
+    if (artra_dev_geo_classify(s->dev, bodies, space->geo.totN, region, &packed, &count)) return artra_dev_failed();
+    return artra_apply_classification(space, region, packed, count, joins);
 }
 
 /* The device tables after a classification on the device: the ghost table

@@ -162,7 +162,7 @@ SIDE_WORKLOADS = (
     ("tgv128", 20, "tgv64", 4, "sweep_x/sweep_march with the viscous flux (stage-by-stage sweeps)", 176.6),
     ("cylinder", 40, "cyl64", 100, "sweep_x/sweep_march with the immersed-boundary fill (stage-by-stage sweeps)",
      126.0),
-    ("cylinder_moving", 20, "cyl64_moving", 10,
+    ("cylinder_moving", 40, "cyl64_moving", 10,
      "sweep_x/sweep_march with the immersed-boundary fill; solid half-steps (surface force, kinematics, motion, "
      "reclassification, ghost tables) between the fluid steps", 126.0),
 )

@@ -292,7 +292,7 @@ def cases():
                                 geo=moving_geo_text(spheres=[dict(O=(0.05, 0, 0), r=0.2, V=(-0.6, 0.2, 0.1),
                                                                   W=(0, 0, 2), rho=2.7, T=0.0038, cf=1)]))
     # the benchmark's immersed cylinder (512x512x4) moving the same way, 20 steps
-    c["cylinder_moving"] = dict(case=case_text(((0, 0, 0), (2, 2, 0.016), (512, 512, 4)), (0, 1e9, 0.6, 20, 1, 0),
+    c["cylinder_moving"] = dict(case=case_text(((0, 0, 0), (2, 2, 0.016), (512, 512, 4)), (0, 1e9, 0.6, 60, 1, 0),
                                                (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
                                                [inflow(*CYL_POST), OUTFLOW, slip(), slip(), PERIODIC, PERIODIC],
                                                [plane((0.6, 0, 0), (-1, 0, 0), CYL_POST)]),
