@@ -1207,6 +1207,8 @@ struct GeoBodyDev {
     double *v = nullptr, *Nf = nullptr, *Ne = nullptr, *Nv = nullptr;   // parts of block
     double *block = nullptr;
     double *stage = nullptr;        // pinned host copy of block for the upload
+    double4 *sph = nullptr;         // each facet's bounding sphere (centre, radius)
+    double4 *bsph = nullptr;        // each 32-facet block's bounding sphere
     bool uploaded = false;
 };
 
@@ -1270,12 +1272,96 @@ __global__ __launch_bounds__(256) void geo_sphere(Grid g, GeoSetup gs, GeoBox b,
 }
 
 // Pass 2 for a triangulated body (computational_geometry.c:463-506): one
-// wavefront per node; lane l measures facets l, l + 64, ... keeping its first
-// facet at its minimum, the lanes' (distance, facet) pairs reduce to the
-// lexicographic minimum (the first facet at the global minimum, as the
-// serial scan), and the pseudo-normal at that facet decides.
+// wavefront per node; the lanes measure the facets that bounding spheres
+// (of 32-facet blocks, then of facets) cannot rule out, the lanes'
+// (distance, facet) pairs reduce to the lexicographic minimum (the first
+// facet at the global minimum, as the serial scan), and the pseudo-normal
+// at that facet decides.
+// Bounding spheres for the closest-facet search below: per facet its
+// centroid and radius, per block of 32 consecutive facets (a half-wavefront
+// here) the mean of their centroids and the farthest reach of their spheres
+// from it.
+__global__ __launch_bounds__(256) void geo_spheres(ArtraPolyView pv, double4 *sph, double4 *bsph)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = n < pv.faceN;
+    double c[3] = {0.0, 0.0, 0.0}, r2 = 0.0;
+    if (on) {
+        const double *v0 = pv.v[pv.f[n][0]], *v1 = pv.v[pv.f[n][1]], *v2 = pv.v[pv.f[n][2]];
+        for (int s = 0; s < 3; ++s) c[s] = (v0[s] + v1[s] + v2[s]) / 3.0;
+        const double *vs[3] = {v0, v1, v2};
+        for (int q = 0; q < 3; ++q) {
+            const double d[3] = {vs[q][0] - c[0], vs[q][1] - c[1], vs[q][2] - c[2]};
+            r2 = fmax(r2, d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        }
+        sph[n] = make_double4(c[0], c[1], c[2], sqrt(r2));
+    }
+    const double r = sqrt(r2);
+    // the block's mean centroid and reach over its half-wavefront
+    double m[3] = {c[0], c[1], c[2]};
+    int cnt = on ? 1 : 0;
+    for (int off = 16; off > 0; off >>= 1) {
+        for (int s = 0; s < 3; ++s) m[s] += __shfl_xor(m[s], off, 32);
+        cnt += __shfl_xor(cnt, off, 32);
+    }
+    if (cnt == 0) return;                           // the whole half-wavefront past the facets
+    for (int s = 0; s < 3; ++s) m[s] /= (double)cnt;
+    double reach = 0.0;
+    if (on) {
+        const double d[3] = {c[0] - m[0], c[1] - m[1], c[2] - m[2]};
+        reach = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + r;
+    }
+    for (int off = 16; off > 0; off >>= 1) reach = fmax(reach, __shfl_xor(reach, off, 32));
+    if ((n & 31) == 0) bsph[n >> 5] = make_double4(m[0], m[1], m[2], reach);
+}
+
+// Pass 2 for a triangulated body (computational_geometry.c:463-506): one
+// wavefront per node; the lanes measure the facets that bounding spheres
+// (of 32-facet blocks, then of facets) cannot rule out, the lanes'
+// (distance, facet) pairs reduce to the lexicographic minimum (the first
+// facet at the global minimum, as the serial scan), and the pseudo-normal
+// at that facet decides.
+// A bounding sphere per facet (centroid, radius) for the closest-facet
+// search below.
+__global__ __launch_bounds__(256) void geo_spheres(ArtraPolyView pv, double4 *sph)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= pv.faceN) return;
+    const double *v0 = pv.v[pv.f[n][0]], *v1 = pv.v[pv.f[n][1]], *v2 = pv.v[pv.f[n][2]];
+    double c[3], r2 = 0.0;
+    for (int s = 0; s < 3; ++s) c[s] = (v0[s] + v1[s] + v2[s]) / 3.0;
+    const double *vs[3] = {v0, v1, v2};
+    for (int q = 0; q < 3; ++q) {
+        const double d[3] = {vs[q][0] - c[0], vs[q][1] - c[1], vs[q][2] - c[2]};
+        r2 = fmax(r2, d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    }
+    sph[n] = make_double4(c[0], c[1], c[2], sqrt(r2));
+}
+
+// A bounding sphere per block of 32 consecutive facets: the mean of their
+// centroids and the farthest reach of their spheres from it.
+__global__ __launch_bounds__(64) void geo_blocks(int faceN, const double4 *sph, double4 *bsph)
+{
+    const int bk = blockIdx.x * blockDim.x + threadIdx.x;
+    const int first = bk * 32, last = min(first + 32, faceN);
+    if (first >= faceN) return;
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int n = first; n < last; ++n) {
+        c[0] += sph[n].x;
+        c[1] += sph[n].y;
+        c[2] += sph[n].z;
+    }
+    for (int s = 0; s < 3; ++s) c[s] /= (double)(last - first);
+    double r = 0.0;
+    for (int n = first; n < last; ++n) {
+        const double d[3] = {sph[n].x - c[0], sph[n].y - c[1], sph[n].z - c[2]};
+        r = fmax(r, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) + sph[n].w);
+    }
+    bsph[bk] = make_double4(c[0], c[1], c[2], r);
+}
+
 __global__ __launch_bounds__(256) void geo_mesh(Grid g, GeoSetup gs, GeoBox b, int *did, int *fid, ArtraPolyView pv,
-                                                int body)
+                                                const double4 *sph, const double4 *bsph, int body)
 {
     const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     if (t >= b.count) return;                       // whole wavefronts leave together
@@ -1285,13 +1371,58 @@ __global__ __launch_bounds__(256) void geo_mesh(Grid g, GeoSetup gs, GeoBox b, i
     if (0 != did[q]) return;                        // the same node on every lane
     double p[3];
     GeoPoint(gs, i, j, k, p);
+    // a bound first: the squared distance U to the nearest of the facet
+    // blocks' first centroids is at least the minimum; a block or facet
+    // whose bounding sphere lies farther than U (with a margin far above
+    // rounding) cannot hold the minimum and is skipped; the others are
+    // measured, each lane keeping the lexicographic minimum of (distance,
+    // facet), which the serial scan's first facet at the minimum is
+    const int nblk = (pv.faceN + 31) >> 5;
+    double U = DBL_MAX, R2 = 0.0;
+    for (int bk = lane; bk < nblk; bk += 64) {
+        const double4 c = sph[bk * 32];
+        const double d[3] = {p[0] - c.x, p[1] - c.y, p[2] - c.z};
+        U = fmin(U, d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        R2 = fmax(R2, bsph[bk].w * bsph[bk].w);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        U = fmin(U, __shfl_xor(U, off));
+        R2 = fmax(R2, __shfl_xor(R2, off));
+    }
+    const double thr = U + 1e-6 * (U + R2);
     double best = FLT_MAX;                          // the serial scan's start: facet 0 at FLT_MAX
     int cid = 0;
-    for (int n = lane; n < pv.faceN; n += 64) {
-        const double d2 = GcFacetDistance2(p, n, &pv);
-        if (best > d2) {
-            best = d2;
-            cid = n;
+    for (int base = 0; base < nblk; base += 64) {
+        bool live = false;
+        if (base + lane < nblk) {
+            const double4 c = bsph[base + lane];
+            const double d[3] = {p[0] - c.x, p[1] - c.y, p[2] - c.z};
+            const double dc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const double gap = dc - c.w - 1e-9 * (dc + c.w);
+            live = !(gap > 0.0 && gap * gap > thr);
+        }
+        unsigned long long m = __ballot(live);
+        while (m) {                                 // two blocks at a time, a half-wavefront each
+            const int b0 = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            int b1 = -1;
+            if (m) {
+                b1 = __ffsll((long long)m) - 1;
+                m &= m - 1;
+            }
+            const int mb = (lane < 32) ? b0 : b1;
+            const int n = (base + mb) * 32 + (lane & 31);
+            if (mb < 0 || n >= pv.faceN) continue;
+            const double4 c = sph[n];
+            const double d[3] = {p[0] - c.x, p[1] - c.y, p[2] - c.z};
+            const double dc = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            const double gap = dc - c.w - 1e-9 * (dc + c.w);
+            if (gap > 0.0 && gap * gap > thr) continue;
+            const double d2 = GcFacetDistance2(p, n, &pv);
+            if (d2 < best || (d2 == best && n < cid)) {
+                best = d2;
+                cid = n;
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -2018,6 +2149,7 @@ struct ArtraDevice {
     double *forceTerms = nullptr;   // [forceCount][10]
     double *forceSums = nullptr;    // [forceBodyCount][12]
     int forceCount = 0, forceBodyCount = 0;
+    int forceNodeCap = 0, forceBodyCap = 0;     // allocated entries, kept across rebuilds
     int *probeNodes = nullptr;      // gather buffers, grown as needed
     double *probePrim = nullptr;
     int probeCap = 0;
@@ -2902,7 +3034,7 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->geoCounters);
     hipHostFree(dev->geoCountsHost);
     for (GeoBodyDev &b : dev->geoBodies) {
-        hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage);
+        hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage); hipFree(b.sph); hipFree(b.bsph);
     }
     hipFree(dev->vmax);
     hipFree(dev->marchBuf);
@@ -3160,25 +3292,34 @@ int artra_dev_max_speed(ArtraDevice *dev, const double floor[3], double vmax[3])
 int artra_dev_set_force_table(ArtraDevice *dev, const ArtraForceNode *nodes, int count, const ArtraForceBody *bodies,
                               int nbody)
 {
+    // the tables keep their allocations while they fit (moving bodies
+    // rebuild them every half-step); the copies below wait for the stream
     HIP_TRY(hipStreamSynchronize(dev->stream));
-    hipFree(dev->forceNodes);
-    hipFree(dev->forceBodies);
-    hipFree(dev->forceTerms);
-    hipFree(dev->forceSums);
-    dev->forceNodes = nullptr;
-    dev->forceBodies = nullptr;
-    dev->forceTerms = nullptr;
-    dev->forceSums = nullptr;
     dev->forceCount = dev->forceBodyCount = 0;
     if (nbody <= 0) return 0;
-    HIP_TRY(hipMalloc(&dev->forceBodies, nbody * sizeof(ArtraForceBody)));
-    HIP_TRY(hipMalloc(&dev->forceSums, (size_t)nbody * 12 * sizeof(double)));
-    HIP_TRY(hipMemcpy(dev->forceBodies, bodies, nbody * sizeof(ArtraForceBody), hipMemcpyHostToDevice));
-    if (count > 0) {
-        HIP_TRY(hipMalloc(&dev->forceNodes, count * sizeof(ArtraForceNode)));
-        HIP_TRY(hipMalloc(&dev->forceTerms, (size_t)count * 10 * sizeof(double)));
-        HIP_TRY(hipMemcpy(dev->forceNodes, nodes, count * sizeof(ArtraForceNode), hipMemcpyHostToDevice));
+    if (nbody > dev->forceBodyCap) {
+        hipFree(dev->forceBodies);
+        hipFree(dev->forceSums);
+        dev->forceBodies = nullptr;
+        dev->forceSums = nullptr;
+        dev->forceBodyCap = 0;
+        HIP_TRY(hipMalloc(&dev->forceBodies, nbody * sizeof(ArtraForceBody)));
+        HIP_TRY(hipMalloc(&dev->forceSums, (size_t)nbody * 12 * sizeof(double)));
+        dev->forceBodyCap = nbody;
     }
+    HIP_TRY(hipMemcpy(dev->forceBodies, bodies, nbody * sizeof(ArtraForceBody), hipMemcpyHostToDevice));
+    if (count > dev->forceNodeCap) {
+        hipFree(dev->forceNodes);
+        hipFree(dev->forceTerms);
+        dev->forceNodes = nullptr;
+        dev->forceTerms = nullptr;
+        dev->forceNodeCap = 0;
+        const int cap = count + count / 2;
+        HIP_TRY(hipMalloc(&dev->forceNodes, (size_t)cap * sizeof(ArtraForceNode)));
+        HIP_TRY(hipMalloc(&dev->forceTerms, (size_t)cap * 10 * sizeof(double)));
+        dev->forceNodeCap = cap;
+    }
+    if (count > 0) HIP_TRY(hipMemcpy(dev->forceNodes, nodes, count * sizeof(ArtraForceNode), hipMemcpyHostToDevice));
     dev->forceCount = count;
     dev->forceBodyCount = nbody;
     return 0;
@@ -4049,7 +4190,7 @@ int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *
     HIP_TRY(hipMemcpy(dev->geoFlags, flags, (size_t)4 * N * sizeof(int), hipMemcpyHostToDevice));
     if ((int)dev->geoBodies.size() != nbody) {
         for (GeoBodyDev &b : dev->geoBodies) {
-            hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage);
+            hipFree(b.f); hipFree(b.e); hipFree(b.block); hipHostFree(b.stage); hipFree(b.sph); hipFree(b.bsph);
         }
         dev->geoBodies.assign(nbody, GeoBodyDev());
         hipFree(dev->geoMoving);
@@ -4087,6 +4228,8 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
             HIP_TRY(hipMalloc(&gb.f, (size_t)gb.faceN * 3 * sizeof(int)));
             HIP_TRY(hipMalloc(&gb.e, (size_t)gb.edgeN * 4 * sizeof(int)));
             HIP_TRY(hipMalloc(&gb.block, 2 * vb + fb + eb));
+            HIP_TRY(hipMalloc(&gb.sph, (size_t)gb.faceN * sizeof(double4)));
+            HIP_TRY(hipMalloc(&gb.bsph, (size_t)((gb.faceN + 31) / 32) * sizeof(double4)));
             HIP_TRY(hipHostMalloc(&gb.stage, 2 * vb + fb + eb));
             gb.v = gb.block;
             gb.Nf = gb.v + vb / sizeof(double);
@@ -4106,6 +4249,12 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
             memcpy(st + vb + fb, bd.pv.Ne, eb);
             memcpy(st + vb + fb + eb, bd.pv.Nv, vb);
             HIP_TRY(hipMemcpyAsync(gb.block, gb.stage, 2 * vb + fb + eb, hipMemcpyHostToDevice, dev->stream));
+            ArtraPolyView dv = bd.pv;
+            dv.f = (const int (*)[3])gb.f;
+            dv.v = (const double (*)[3])gb.v;
+            hipLaunchKernelGGL(geo_spheres, dim3((unsigned)((gb.faceN + 255) / 256)), dim3(256), 0, dev->stream, dv,
+                               gb.sph, gb.bsph);
+            HIP_TRY(hipGetLastError());
         }
         gb.uploaded = true;
     }
@@ -4162,7 +4311,7 @@ int artra_dev_geo_classify(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
             pv.Ne = (const double (*)[3])gb.Ne;
             pv.Nv = (const double (*)[3])gb.Nv;
             hipLaunchKernelGGL(geo_mesh, dim3((unsigned)((bb.count + 3) / 4)), dim3(256), 0, dev->stream, g, gs, bb,
-                               did, fid, pv, n);
+                               did, fid, pv, gb.sph, gb.bsph, n);
         }
         HIP_TRY(hipGetLastError());
     }
