@@ -1656,6 +1656,88 @@ __global__ __launch_bounds__(256) void ib_build(GhostBuildArgs a, GeoBox b, int 
     }
 }
 
+// The surface-force table of one body on the device (the host's
+// artra_build_force_table over the device's flags): one workgroup scans the
+// body's node box in k-j-i order and appends its layer-2 ghosts in that
+// order (a block-wide prefix of ballots), with the normal, arm and distance
+// of each; it counts the body's interfacial layer-2 nodes.  counters[0] is
+// the table's running end, counters[1 + 2 slot] and [2 + 2 slot] the
+// body's ghost and interfacial counts.
+constexpr int kForceBuildThreads = 1024;
+
+__global__ __launch_bounds__(kForceBuildThreads) void force_build(Grid g, GeoSetup gs, GeoBox b, GhostBody body,
+                                                                  int slot, int viscous, const int *did,
+                                                                  const int *fid, const int *lid, const int *gst,
+                                                                  ArtraForceNode *nodes, ArtraForceBody *bodies,
+                                                                  int cap, int *counters)
+{
+    __shared__ int waveHits[kForceBuildThreads / 64];
+    __shared__ int lidCount;
+    const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int first = counters[0];
+    if (tid == 0) lidCount = 0;
+    __syncthreads();
+    int end = first;
+    for (long start = 0; start < b.count; start += kForceBuildThreads) {
+        const long t = start + tid;
+        bool hit = false;
+        int i = 0, j = 0, k = 0;
+        long q = 0;
+        if (t < b.count) {
+            q = GeoNode(g, b, t, i, j, k);
+            const int d = did[q];
+            if (body.body + 1 == d) {
+                hit = (2 == gst[q]);
+                if (2 == lid[q]) atomicAdd(&lidCount, 1);
+            }
+        }
+        const unsigned long long m = __ballot(hit);
+        __syncthreads();                            // the last chunk's prefix is read
+        if (lane == 0) waveHits[wave] = __popcll(m);
+        __syncthreads();
+        int before = end, total = 0;
+        for (int w = 0; w < kForceBuildThreads / 64; ++w) {
+            if (w < wave) before += waveHits[w];
+            total += waveHits[w];
+        }
+        if (hit) {
+            const int pos = before + __popcll(m & below);
+            if (pos < cap) {
+                double pG[3], pO[3], pI[3], N[3];
+                GeoPoint(gs, i, j, k, pG);
+                GcGeometricData(pG, fid[q], &body.pv, body.O, body.r, pO, pI, N);
+                ArtraForceNode e;
+                e.node = (int)q;
+                e.body = slot;
+                for (int s = 0; s < 3; ++s) {
+                    e.N[s] = N[s];
+                    e.r[s] = pO[s] - body.O[s];
+                }
+                e.dist = sqrt(GbDist2(pG, pO));
+                nodes[pos] = e;
+            }
+        }
+        end += total;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        counters[0] = end;
+        counters[1 + 2 * slot] = end - first;
+        counters[2 + 2 * slot] = lidCount;
+        ArtraForceBody fb;
+        fb.first = first;
+        fb.count = end - first;
+        fb.viscous = viscous;
+        fb.pad = 0;
+        for (int s = 0; s < 3; ++s) {
+            fb.V[s] = body.V[s];
+            fb.W[s] = body.W[s];
+        }
+        bodies[slot] = fb;
+    }
+}
+
 // A table of filled nodes on the device with its stencil entries; the
 // buffers grow as needed and are reused when the table is rebuilt.
 struct IbTable {
@@ -2150,6 +2232,9 @@ struct ArtraDevice {
     double *forceSums = nullptr;    // [forceBodyCount][12]
     int forceCount = 0, forceBodyCount = 0;
     int forceNodeCap = 0, forceBodyCap = 0;     // allocated entries, kept across rebuilds
+    int *forceCounters = nullptr;   // device-built force table: running end, per body counts
+    int *forceCountsHost = nullptr; // their pinned host copy
+    int forceCounterCap = 0;
     int *probeNodes = nullptr;      // gather buffers, grown as needed
     double *probePrim = nullptr;
     int probeCap = 0;
@@ -3042,6 +3127,8 @@ void artra_dev_destroy(ArtraDevice *dev)
     hipFree(dev->forceBodies);
     hipFree(dev->forceTerms);
     hipFree(dev->forceSums);
+    hipFree(dev->forceCounters);
+    hipHostFree(dev->forceCountsHost);
     hipFree(dev->probeNodes);
     hipFree(dev->probePrim);
     FreeIbTable(dev->ib);
@@ -4422,6 +4509,99 @@ int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbo
     dev->layerStart.assign(layerEnd.begin(), layerEnd.end());
     dev->ibImageOnly = (ibmLayer >= layers);
     return 0;
+}
+
+
+int artra_dev_build_force_table(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int *slotBody,
+                                const int (*box)[3][2], const int *viscous, int nb, int *gstN, int *lidN)
+{
+    if (!dev->geoFlags) return Fail("moving-body classification not set up");
+    dev->forceCount = dev->forceBodyCount = 0;
+    if (nb <= 0) return 0;
+    const long N = dev->g.N;
+    const int *did = dev->geoFlags, *fid = did + N, *lid = fid + N, *gst = lid + N;
+    if (nb > dev->forceBodyCap || 1 + 2 * nb > dev->forceCounterCap) {
+        HIP_TRY(hipStreamSynchronize(dev->stream));
+        hipFree(dev->forceBodies);
+        hipFree(dev->forceSums);
+        hipFree(dev->forceCounters);
+        hipHostFree(dev->forceCountsHost);
+        dev->forceBodies = nullptr;
+        dev->forceSums = nullptr;
+        dev->forceCounters = nullptr;
+        dev->forceCountsHost = nullptr;
+        dev->forceBodyCap = dev->forceCounterCap = 0;
+        HIP_TRY(hipMalloc(&dev->forceBodies, nb * sizeof(ArtraForceBody)));
+        HIP_TRY(hipMalloc(&dev->forceSums, (size_t)nb * 12 * sizeof(double)));
+        HIP_TRY(hipMalloc(&dev->forceCounters, (size_t)(1 + 2 * nb) * sizeof(int)));
+        HIP_TRY(hipHostMalloc(&dev->forceCountsHost, (size_t)(1 + 2 * nb) * sizeof(int)));
+        dev->forceBodyCap = nb;
+        dev->forceCounterCap = 1 + 2 * nb;
+    }
+    // a second pass when the table outgrew its allocation
+    for (int pass = 0; pass < 2; ++pass) {
+        HIP_TRY(hipMemsetAsync(dev->forceCounters, 0, sizeof(int), dev->stream));
+        for (int b = 0; b < nb; ++b) {
+            const int n = slotBody[b];
+            if (n < 0 || n >= nbody) return Fail("force body %d out of range", n);
+            const ArtraGeoBody &bd = bodies[n];
+            GeoBox bb;
+            bb.count = 1;
+            for (int s = 0; s < 3; ++s) {
+                bb.lo[s] = box[b][s][0];
+                bb.n[s] = box[b][s][1] > box[b][s][0] ? box[b][s][1] - box[b][s][0] : 0;
+                bb.count *= bb.n[s];
+            }
+            GhostBody gbody;
+            gbody.pv = bd.pv;
+            if (bd.pv.faceN > 0) {
+                const GeoBodyDev &gb = dev->geoBodies[n];
+                if (!gb.uploaded) return Fail("body %d has no device arrays", n);
+                gbody.pv.f = (const int (*)[3])gb.f;
+                gbody.pv.e = (const int (*)[4])gb.e;
+                gbody.pv.v = (const double (*)[3])gb.v;
+                gbody.pv.Nf = (const double (*)[3])gb.Nf;
+                gbody.pv.Ne = (const double (*)[3])gb.Ne;
+                gbody.pv.Nv = (const double (*)[3])gb.Nv;
+            }
+            for (int s = 0; s < 3; ++s) {
+                gbody.O[s] = bd.O[s];
+                gbody.V[s] = bd.V[s];
+                gbody.W[s] = bd.W[s];
+            }
+            gbody.r = bd.r;
+            gbody.cf = bd.cf;
+            gbody.T = bd.T;
+            gbody.body = n;
+            hipLaunchKernelGGL(force_build, dim3(1), dim3(kForceBuildThreads), 0, dev->stream, dev->g, dev->geo, bb,
+                               gbody, b, viscous[b], did, fid, lid, gst, dev->forceNodes, dev->forceBodies,
+                               dev->forceNodeCap, dev->forceCounters);
+            HIP_TRY(hipGetLastError());
+        }
+        int *c = dev->forceCountsHost;
+        HIP_TRY(hipMemcpyAsync(c, dev->forceCounters, (size_t)(1 + 2 * nb) * sizeof(int), hipMemcpyDeviceToHost,
+                               dev->stream));
+        HIP_TRY(hipStreamSynchronize(dev->stream));
+        if (c[0] <= dev->forceNodeCap) {
+            for (int b = 0; b < nb; ++b) {
+                gstN[b] = c[1 + 2 * b];
+                lidN[b] = c[2 + 2 * b];
+            }
+            dev->forceCount = c[0];
+            dev->forceBodyCount = nb;
+            return 0;
+        }
+        hipFree(dev->forceNodes);
+        hipFree(dev->forceTerms);
+        dev->forceNodes = nullptr;
+        dev->forceTerms = nullptr;
+        dev->forceNodeCap = 0;
+        const int cap = c[0] + c[0] / 2;
+        HIP_TRY(hipMalloc(&dev->forceNodes, (size_t)cap * sizeof(ArtraForceNode)));
+        HIP_TRY(hipMalloc(&dev->forceTerms, (size_t)cap * 10 * sizeof(double)));
+        dev->forceNodeCap = cap;
+    }
+    return Fail("surface-force table did not fit after growing");
 }
 
 }  // extern "C"

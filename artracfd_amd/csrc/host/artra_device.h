@@ -121,6 +121,14 @@ typedef struct {
 int artra_dev_build_ghosts(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, double dd[DIMS], double tinyL,
                            int ibmLayer, int *fallback);
 
+/* The surface-force table built on the device from its flags: the nodes,
+ * their order and geometry of artra_build_force_table.  Slot b integrates
+ * body slotBody[b] over its node box box[b] with the wall shear when
+ * viscous[b]; gstN[b] and lidN[b] return its layer-2 ghost and interfacial
+ * node counts. */
+int artra_dev_build_force_table(ArtraDevice *dev, const ArtraGeoBody *bodies, int nbody, const int *slotBody,
+                                const int (*box)[DIMS][LIMIT], const int *viscous, int nb, int *gstN, int *lidN);
+
 /* The flags did, fid, lid, gst ([4][N], host order) onto the device. */
 int artra_dev_geo_init(ArtraDevice *dev, const ArtraGeoSetup *setup, const int *flags, int nbody);
 /* ComputeGeometricField's three passes (immersed_boundary.c:65-264) over

@@ -338,10 +338,56 @@ static int ClassifyOnDevice(ArtraHipSolver *s, const int region[DIMS][LIMIT], co
     return artra_apply_classification(space, region, packed, count, joins);
 }
 
+/* The surface-force table built on the device from its flags
+ * (artra_build_force_table's nodes, order and geometry); the host keeps
+ * each body's counts for the calibration. */
+static int ForceTableOnDevice(ArtraHipSolver *s, const ArtraGeoBody *bodies)
+{
+    const Space *space = s->space;
+    const Geometry *geo = &space->geo;
+    free(s->force);
+    s->force = NULL;
+    s->forceBodies = 0;
+    int nb = 0;
+    for (int n = 0; n < geo->totN; ++n) nb += (0 >= geo->poly[n].state);
+    if (0 == nb) return artra_dev_set_force_table(s->dev, NULL, 0, NULL, 0) ? artra_dev_failed() : 0;
+    s->force = calloc((size_t)nb, sizeof *s->force);
+    int *ints = malloc((size_t)4 * nb * sizeof(int));
+    int (*box)[DIMS][LIMIT] = malloc((size_t)nb * sizeof *box);
+    if (!s->force || !ints || !box) {
+        free(ints);
+        free(box);
+        return artra_set_error(1, "out of host memory for the surface-force table");
+    }
+    int *slot = ints, *viscous = ints + nb, *gstN = ints + 2 * nb, *lidN = ints + 3 * nb;
+    int b = 0;
+    for (int n = 0; n < geo->totN; ++n) {
+        const Polyhedron *poly = geo->poly + n;
+        if (0 < poly->state) continue;
+        slot[b] = n;
+        viscous[b] = (0.0 < s->model->refMu) && (0.0 < poly->cf);
+        artra_force_body_box(&space->part, poly, box[b]);
+        ++b;
+    }
+    int rc = artra_dev_build_force_table(s->dev, bodies, geo->totN, slot, (const int (*)[DIMS][LIMIT])box, viscous,
+                                         nb, gstN, lidN) ? artra_dev_failed() : 0;
+    if (!rc) {
+        for (b = 0; b < nb; ++b) {
+            s->force[b].body = slot[b];
+            s->force[b].gstN = gstN[b];
+            s->force[b].lidN = lidN[b];
+        }
+        s->forceBodies = nb;
+    }
+    free(ints);
+    free(box);
+    return rc;
+}
+
 /* The device tables after a classification on the device: the ghost table
  * built there (on the host when a stencil search reaches beyond radius 3),
- * the surface-force table from the host's flags.  The fluid flags are
- * already current on the device. */
+ * the surface-force table likewise from the device's flags.  The fluid flags
+ * are already current on the device. */
 static int TablesOnDevice(ArtraHipSolver *s, const ArtraGeoBody *bodies)
 {
     const Space *space = s->space;
@@ -356,7 +402,7 @@ static int TablesOnDevice(ArtraHipSolver *s, const ArtraGeoBody *bodies)
         if (artra_build_ghost_table(space, s->model, &s->ghosts)) return 1;
         if (artra_dev_set_ghosts(s->dev, &s->ghosts)) return artra_dev_failed();
     }
-    return artra_build_force_table(s);
+    return ForceTableOnDevice(s, bodies);
 }
 
 int artra_hip_refresh_geometry(ArtraHipSolver *s, const int (*region)[LIMIT])

@@ -41,6 +41,11 @@ static void NodePoint(const Partition *part, int k, int j, int i, Real p[DIMS])
 }
 
 /* ------------------------------------------------------------ surface force */
+void artra_force_body_box(const Partition *part, const Polyhedron *poly, int box[DIMS][LIMIT])
+{
+    BodyBox(part, poly, PIN, box);
+}
+
 int artra_build_force_table(ArtraHipSolver *s)
 {
     const Space *space = s->space;

@@ -32,6 +32,9 @@ struct ArtraHipSolver {
     uint8_t *fluid;           /* [N] host copy of the device's fluid flags */
 };
 
+/* The node box of a body's surface-force scan: its bounding box over the
+ * interior range. */
+void artra_force_body_box(const Partition *part, const Polyhedron *poly, int box[DIMS][LIMIT]);
 /* Build the surface-force table of the solver's bodies and hand it to the
  * device (at attach). */
 int artra_build_force_table(ArtraHipSolver *s);
