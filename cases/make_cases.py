@@ -147,6 +147,49 @@ def cylinder_stl(path, cx, cy, r, z0, z1, segments):
     return len(tris)
 
 
+def icosphere_stl(path, cx, cy, cz, r, level):
+    """Closed sphere: an icosahedron whose facets are split in four `level`
+    times, the new vertices pushed onto the sphere; 20 * 4^level facets."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [tuple(q / math.sqrt(sum(c * c for c in v)) for q in v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def middle(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = [(verts[a][q] + verts[b][q]) / 2.0 for q in range(3)]
+                n = math.sqrt(sum(c * c for c in m))
+                verts.append(tuple(c / n for c in m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+        nxt = []
+        for a, b, c in faces:
+            ab, bc, ca = middle(a, b), middle(b, c), middle(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nxt
+    f32 = lambda v: struct.unpack("<f", struct.pack("<f", v))[0]
+    pts = [(f32(cx + r * v[0]), f32(cy + r * v[1]), f32(cz + r * v[2])) for v in verts]
+    with open(path, "wb") as fp:
+        fp.write(b"icosphere written by cases/make_cases.py".ljust(80, b" "))
+        fp.write(struct.pack("<I", len(faces)))
+        for a, b, c in faces:
+            t0, t1, t2 = pts[a], pts[b], pts[c]
+            e1 = [t1[q] - t0[q] for q in range(3)]
+            e2 = [t2[q] - t0[q] for q in range(3)]
+            n = (e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0])
+            fp.write(struct.pack("<3f", *n))
+            for v in (t0, t1, t2):
+                fp.write(struct.pack("<3f", *v))
+            fp.write(struct.pack("<H", 0))
+    return len(faces)
+
+
 OUTFLOW = ["outflow"]
 PERIODIC = ["periodic"]
 
@@ -291,6 +334,16 @@ def cases():
                                                probes=([(0.3, 0.1, -0.05)], [], 1, 1, (5, 5, 5, 5))),
                                 geo=moving_geo_text(spheres=[dict(O=(0.05, 0, 0), r=0.2, V=(-0.6, 0.2, 0.1),
                                                                   W=(0, 0, 2), rho=2.7, T=0.0038, cf=1)]))
+    # a triangulated sphere (1280 facets) translating and tumbling about a tilted axis through a
+    # wavy stream (psi 1): closest-facet searches in all three directions
+    c["ico_moving"] = dict(case=case_text(((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), (28, 28, 28)),
+                                          (0, 0.15, 0.6, 0, 2, 0), (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1,
+                                          ("1", "0.3", "0.1*sin(3*y)", "0", "1+0.1*cos(4*z)"),
+                                          [inflow(1, 0.3, 0, 0, 1), OUTFLOW, slip(), slip(), OUTFLOW, OUTFLOW],
+                                          probes=([(0.3, 0.1, -0.05)], [], 1, 1, (5, 5, 5, 5))),
+                           geo=moving_geo_text(stls=[("ico.stl", dict(r=0.21, V=(0.5, -0.3, 0.2),
+                                                                      W=(1.5, -2.0, 2.5), rho=2.0))]),
+                           ico=("ico.stl", 0.03, -0.02, 0.01, 0.21, 3))
     # the benchmark's immersed cylinder (512x512x4) moving the same way, 20 steps
     c["cylinder_moving"] = dict(case=case_text(((0, 0, 0), (2, 2, 0.016), (512, 512, 4)), (0, 1e9, 0.6, 60, 1, 0),
                                                (1, 1, 0, 0, 0, 1, 0), INVISCID, REF1, ("1.4", "0", "0", "0", "1"),
@@ -357,6 +410,10 @@ def main():
             name, cx, cy, r, z0, z1, seg = spec["stl"]
             nf = cylinder_stl(os.path.join(d, name), cx, cy, r, z0, z1, seg)
             assert nf == 4 * seg
+        if "ico" in spec:
+            name, cx, cy, cz, r, level = spec["ico"]
+            nf = icosphere_stl(os.path.join(d, name), cx, cy, cz, r, level)
+            assert nf == 20 * 4 ** level
     for tag in sorted(cases()):
         for line in cases()[tag]["case"].splitlines():
             assert len(line) < 100, (tag, line)

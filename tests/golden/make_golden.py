@@ -87,7 +87,7 @@ def make_vectors(work, seed=20261202, count=400):
     print("vectors", count, nf)
 
 
-SOLVE_CASES = ("cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight", "cyl64_moving",
+SOLVE_CASES = ("cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight", "cyl64_moving", "ico_moving",
                "spheres_collide", "sphere20_moving")
 SOLVE_TEXT = (".csv", ".log", ".pvd", ".case", ".state")   # stored as text; other outputs (.vts, .vtp, EnSight binaries) by hash
 

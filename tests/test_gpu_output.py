@@ -53,17 +53,18 @@ def _solve(name, tmp_path, exact):
 
 
 # inviscid cases: exact arithmetic reproduces the reference bit for bit
-BITWISE = ("cyl64_probes", "cyl64_ensight", "cyl64_moving", "spheres_collide")
+BITWISE = ("cyl64_probes", "cyl64_ensight", "cyl64_moving", "ico_moving", "spheres_collide")
 
 
 @pytest.mark.parametrize("name", ["cyl64_probes", "sphere20_probes", "cyl64_ensight", "sphere20_ensight",
-                                  "cyl64_moving", "spheres_collide", "sphere20_moving"])
+                                  "cyl64_moving", "ico_moving", "spheres_collide", "sphere20_moving"])
 @pytest.mark.parametrize("exact", [True, False])
 def test_solve_writes_the_reference_outputs(name, exact, tmp_path):
     """Exact arithmetic on the inviscid cases: every text file
     byte-identical, every field and geometry file the same SHA-256 as the
     reference's (ParaView and EnSight Gold writers) and, for the moving
-    bodies (a translating, spinning STL cylinder; two spheres that collide),
+    bodies (a translating, spinning STL cylinder; a tumbling 1280-facet STL
+    sphere; two spheres that collide),
     the same final positions, velocities and accelerations.  The viscous
     spheres, still and moving (the device's pow differs from glibc's in the
     last bit), and the fast arithmetic: the same files with the same numbers

@@ -192,6 +192,7 @@ def test_integration_link_recipe():
 @pytest.mark.parametrize("name, moves, motion", [
     ("cyl64_moving", 12, ("0.004", "0.0025", "0", "0.02")),     # sub-cell steps with spin
     ("cyl64_moving", 5, ("0.031", "-0.02", "0", "0.3")),         # two cells a move
+    ("ico_moving", 6, ("0.011", "-0.007", "0.005", "0.25")),     # a 1280-facet sphere, 3D facets
     ("spheres_collide", 10, ("0.012", "-0.006", "0.003", "0")),
     ("spheres_collide", 6, ("-0.05", "0.03", "0.02", "0")),
 ])
