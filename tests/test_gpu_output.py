@@ -132,7 +132,7 @@ def test_surface_force_call_matches_the_oracle(oracle_mod):
 
 
 @pytest.mark.parametrize("name,first,second", [("cyl64", 15, 25), ("shockvortex64", 6, 9), ("tgv16", 7, 8),
-                                               ("spheres_collide", 3, 5)])
+                                               ("spheres_collide", 3, 5), ("ico_moving", 7, 9)])
 def test_restart_continues_the_run_bit_for_bit(name, first, second, tmp_path):
     """March `first` steps, write the FP64 restart, continue `second` steps;
     a fresh solver reading the restart and marching `second` steps ends

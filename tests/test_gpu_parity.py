@@ -162,8 +162,10 @@ def test_reference_driver_through_dropin_symbols(name, tmp_path, oracle_mod):
 
 
 @pytest.mark.skipif(not os.path.exists(DROPIN), reason="the drop-in driver is built only where the reference is")
-def test_reference_solve_with_moving_bodies_through_dropin_symbols(tmp_path, oracle_mod):
-    """The reference's whole Solve with two colliding spheres, its own solid
+@pytest.mark.parametrize("name", ["spheres_collide", "ico_moving"])
+def test_reference_solve_with_moving_bodies_through_dropin_symbols(name, tmp_path, oracle_mod):
+    """The reference's whole Solve with moving bodies (two colliding spheres;
+    a tumbling triangulated sphere), its own solid
     dynamics (surface force, kinematics, collisions, motion) calling the
     library's ComputeGeometricField (classification and reconstruction of
     uncovered nodes on the caller's node array) and TreatImmersedBoundary,
@@ -172,7 +174,6 @@ def test_reference_solve_with_moving_bodies_through_dropin_symbols(tmp_path, ora
     reference's CPU run's bit for bit."""
     import shutil
     import subprocess
-    name = "spheres_collide"
     print(f"reference solve through the drop-in symbols: {name}", flush=True)
     fx = golden("solve_" + name)
     work = tmp_path / name
