@@ -1682,9 +1682,9 @@ __global__ __launch_bounds__(kForceBuildThreads) void force_build(Grid g, GeoSet
     for (long start = 0; start < b.count; start += kForceBuildThreads) {
         const long t = start + tid;
         bool hit = false;
-        int i = 0, j = 0, k = 0;
         long q = 0;
         if (t < b.count) {
+            int i, j, k;
             q = GeoNode(g, b, t, i, j, k);
             const int d = did[q];
             if (body.body + 1 == d) {
@@ -1703,24 +1703,28 @@ __global__ __launch_bounds__(kForceBuildThreads) void force_build(Grid g, GeoSet
         }
         if (hit) {
             const int pos = before + __popcll(m & below);
-            if (pos < cap) {
-                double pG[3], pO[3], pI[3], N[3];
-                GeoPoint(gs, i, j, k, pG);
-                GcGeometricData(pG, fid[q], &body.pv, body.O, body.r, pO, pI, N);
-                ArtraForceNode e;
-                e.node = (int)q;
-                e.body = slot;
-                for (int s = 0; s < 3; ++s) {
-                    e.N[s] = N[s];
-                    e.r[s] = pO[s] - body.O[s];
-                }
-                e.dist = sqrt(GbDist2(pG, pO));
-                nodes[pos] = e;
-            }
+            if (pos < cap) nodes[pos].node = (int)q;    // the geometry once every entry is placed
         }
         end += total;
     }
     __syncthreads();
+    // each entry's normal, arm and distance, all entries at once
+    for (int e = first + tid; e < end && e < cap; e += kForceBuildThreads) {
+        const long q = nodes[e].node;
+        const int i = (int)(q % g.nx), j = (int)((q / g.nx) % g.ny), k = (int)(q / ((long)g.nx * g.ny));
+        double pG[3], pO[3], pI[3], N[3];
+        GeoPoint(gs, i, j, k, pG);
+        GcGeometricData(pG, fid[q], &body.pv, body.O, body.r, pO, pI, N);
+        ArtraForceNode rec;
+        rec.node = (int)q;
+        rec.body = slot;
+        for (int s = 0; s < 3; ++s) {
+            rec.N[s] = N[s];
+            rec.r[s] = pO[s] - body.O[s];
+        }
+        rec.dist = sqrt(GbDist2(pG, pO));
+        nodes[e] = rec;
+    }
     if (tid == 0) {
         counters[0] = end;
         counters[1 + 2 * slot] = end - first;
